@@ -1,0 +1,162 @@
+"""Benchmark of the photon-transport + development + scan forward on one MI355X.
+
+Workload: frame24mp_trix_cubic -- a 6000x4000 frame as 247 tiles of 320 px (edge tiles
+partial), Tri-X cubic emulsion per tile, 128 photons per pixel (3.072e9 photon
+paths), supersample 2 (16 scan rays per pixel, 24 Mpix out).  One step renders the
+whole frame: transport -> development -> scan -> OTF, with the per-tile emulsions
+(crystals + BVH) already resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|batched]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_HBM_GBPS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, exposure, tile_px=320):
+    """The oracle (C restatement of the reference's Rayon path, chunks = 8 * threads)
+    timed on this host's cores over tile (0, 0) of the frame: synthesis excluded,
+    transport + development + scan included."""
+    import numpy as np
+    from oracle import oracle as O
+
+    e, o, x = cfg["emulsion"], cfg["emulsion"]["optics"], cfg["exposure"]
+    threads = os.cpu_count() or 1
+    cs = O.synthesize(W=tile_px, H=tile_px, D=e["depth"], r_min=e["r_min"],
+                      mean_radius=e["mean_crystal_radius"], size_sigma=e["size_variance"],
+                      morph=e["morphology"], aspect=e["tabular_aspect_ratio"],
+                      base_sens=e["base_sensitivity"], sens_sigma=e["sensitivity_sigma"],
+                      min_radius=e["min_radius"], min_half_thickness=e["min_half_thickness"],
+                      seed=cfg["seed"])
+    emap = np.ascontiguousarray(exposure[:tile_px, :tile_px])
+    t0 = time.perf_counter()
+    counts, st = O.expose(cs, emap, W=tile_px, H=tile_px, D=e["depth"], mu_a=o["absorption_coeff"],
+                          mu_s=o["scattering_coeff"], g=o["anisotropy_g"], n_refr=o["refractive_index"],
+                          r_ahu=o["antihalation_reflectance"], od_ahu=o["antihalation_density"],
+                          mu_abs=x["crystal_absorption_coeff"], theta_max=x["illumination_angle_rad"],
+                          max_bounces=x["max_bounces"], seed=cfg["seed"] + 1, sampler=0,
+                          photons_per_area=float(x["photons_per_pixel"]), threads=threads)
+    t1 = time.perf_counter()
+    d = cfg["development"]
+    dev, fe, _, _ = O.develop(cs, counts, d["threshold_photons"], d["filament_expansion"],
+                              d["clumping_proximity_factor"])
+    cs.developed, cs.fe = dev, fe
+    t2 = time.perf_counter()
+    s = cfg["scan"]
+    O.scan(cs, tile_px, tile_px, e["depth"], tile_px, tile_px, spp=s["samples_per_pixel"],
+           ss=s["supersample"], sigma=s["silver_opacity"], seed=cfg["seed"] + 2, threads=threads)
+    t3 = time.perf_counter()
+    photons = st["launched"]
+    return {
+        "value": photons / (t3 - t0) / 1e6,
+        "unit": "Mphoton-paths/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"tile (0,0) {tile_px}x{tile_px} px of the frame, {photons} photon paths "
+                   f"(reference chunking, {st['chunks']} chunks), develop, scan ss="
+                   f"{s['supersample']}; synthesis excluded"),
+        "transport_s": t1 - t0, "develop_s": t2 - t1, "scan_s": t3 - t2,
+        "scan_mpix_per_s": tile_px * tile_px / (t3 - t2) / 1e6,
+        "transport_mpaths_per_s": photons / (t1 - t0) / 1e6,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["plain", "batched"], default="plain")
+    ap.add_argument("--workload", default="frame24mp_trix_cubic")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(tiles would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import fgrain_amd as F
+    from fgrain_amd import workloads as W
+
+    cfg = W.load_config(args.workload)
+    engine = F.GpuEngine.new()
+    log(f"engine: {engine.adapter_name()}")
+    t = time.perf_counter()
+    exposure = W.workload_exposure(cfg)
+    frame = W.Frame(engine, cfg, exposure)
+    log(f"frame ready: {frame.crystals} crystals in {time.perf_counter() - t:.1f} s (host synthesis)")
+    mode = W.Frame.BATCHED if args.mode == "batched" else W.Frame.PLAIN
+    for i in range(args.warmup):
+        st = frame.render(mode)
+        log(f"warmup {i}: {st['ms_total']:.1f} ms")
+    agg = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = frame.render(mode)
+        log(f"step {i}: {st['ms_total']:.1f} ms (transport kernels {st['ms_kernel_transport']:.1f} ms, "
+            f"scan kernels {st['ms_kernel_scan']:.1f} ms)")
+        if agg is None:
+            agg = dict(st)
+        else:
+            for k in ("ms_kernel_transport", "ms_kernel_scan", "ms_transport", "ms_develop", "ms_scan"):
+                agg[k] += st[k]
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed / args.steps * 1e3
+    photons = agg["photons"]
+    pixels = agg["pixels"]
+    # dominant kernel: transport; algorithmic bytes per launch from the device counters
+    # of the same reference-order traversal (4 B exposure + 32 B per node + 48 B per
+    # crystal record + 8 B per absorbing hit)
+    alg_bytes = 4 * photons + 32 * agg["nodes_visited"] + 48 * agg["prims_tested"] + 8 * agg["abs_events"]
+    roof = None
+    if agg["ms_kernel_transport"] > 0:
+        ach = alg_bytes * args.steps / (agg["ms_kernel_transport"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
+                "frac": ach / MI355X_HBM_GBPS, "traffic": None}
+    out = {
+        "metric": "Mphoton-paths/s + scanned Mpix/s, 24MP Tri-X, 1 GPU; adjoint/forward time ratio",
+        "value": photons / (ms_step * 1e-3) / 1e6,
+        "unit": "Mphoton-paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "frame": [frame.width, frame.height],
+                   "tiles": agg["tiles"], "photons_per_pixel": cfg["exposure"]["photons_per_pixel"],
+                   "supersample": cfg["scan"]["supersample"], "mode": args.mode,
+                   "crystals": agg["crystals"]},
+        "scanned_mpix_per_s": pixels / (ms_step * 1e-3) / 1e6,
+        "transport_kernel_mpaths_per_s": photons * args.steps / (agg["ms_kernel_transport"] * 1e-3) / 1e6,
+        "scan_kernel_mpix_per_s": pixels * args.steps / (agg["ms_kernel_scan"] * 1e-3) / 1e6,
+        "ms_breakdown": {k: agg[k] / args.steps for k in ("ms_transport", "ms_develop", "ms_scan",
+                                                          "ms_kernel_transport", "ms_kernel_scan")},
+        "roofline": roof,
+        "transport_per_path": {"bytes": alg_bytes / photons, "nodes": agg["nodes_visited"] / photons,
+                               "prims": agg["prims_tested"] / photons, "bounces": agg["bounces"] / photons},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, host cores) ...")
+        out["cpu_baseline"] = cpu_baseline(cfg, exposure)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

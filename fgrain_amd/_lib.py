@@ -1,0 +1,181 @@
+"""ctypes binding of libfgrain_hip.so (include/fgrain_hip.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading fails with
+an ImportError naming the missing file when the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfgrain_hip.so")
+
+FG_OK = 0
+STATUS_NAMES = {
+    0: "Ok", 1: "AdapterNotFound", 2: "DeviceRequestFailed", 3: "ChannelError",
+    4: "BufferMapError", 5: "BufferRangeError", 6: "PipelineError", 7: "InvalidArgument",
+}
+
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u32p = C.POINTER(C.c_uint32)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+
+
+class Optics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "refractive_index", "absorption_coeff", "scattering_coeff", "anisotropy_g",
+        "antihalation_reflectance", "antihalation_density")]
+
+
+class EmulsionConfigC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "width", "height", "depth", "r_min", "mean_crystal_radius", "size_variance",
+        "tabular_aspect_ratio", "base_sensitivity", "sensitivity_sigma", "min_radius",
+        "min_half_thickness")] + [("morphology", C.c_uint32), ("k_samples", C.c_uint32),
+                                  ("optics", Optics)]
+
+
+class ExposureConfigC(C.Structure):
+    _fields_ = [("photons_per_sq_unit", C.c_double), ("illumination_angle_rad", C.c_double),
+                ("crystal_absorption_coeff", C.c_double), ("max_bounces", C.c_uint32),
+                ("photons_per_pixel", C.c_uint32)]
+
+
+class ExposeStatsC(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "launched", "bounces", "nodes_visited", "prims_tested", "abs_events", "abs_total")] + [
+        ("kernel_ms", C.c_float), ("pad_", C.c_float)]
+
+
+class DevelopmentConfigC(C.Structure):
+    _fields_ = [("threshold_photons", C.c_uint32), ("pad_", C.c_uint32),
+                ("filament_expansion", C.c_double), ("clumping_proximity_factor", C.c_double),
+                ("developer_contrast", C.c_double)]
+
+
+class DevelopmentReportC(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "total_crystals", "developed_crystals", "fixed_crystals", "total_clumps",
+        "max_clump_size")] + [("avg_clump_size", C.c_double)]
+
+
+class ScanConfigC(C.Structure):
+    _fields_ = [("width_px", C.c_uint32), ("height_px", C.c_uint32),
+                ("samples_per_pixel", C.c_uint32), ("mode", C.c_uint32),
+                ("silver_opacity", C.c_double), ("supersample", C.c_uint32), ("otf", C.c_uint32)]
+
+
+class TileParamsC(C.Structure):
+    _fields_ = [("film_w", C.c_double), ("film_h", C.c_double), ("film_d", C.c_double),
+                ("width_px", C.c_uint32), ("height_px", C.c_uint32),
+                ("samples_per_pixel", C.c_uint32), ("morphology", C.c_uint32),
+                ("silver_opacity", C.c_double), ("seed", C.c_uint64),
+                ("r_min", C.c_double), ("mean_crystal_radius", C.c_double),
+                ("size_variance", C.c_double), ("tabular_aspect_ratio", C.c_double),
+                ("base_sensitivity", C.c_double), ("exposure", C.c_double),
+                ("threshold_photons", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("filament_expansion", C.c_double), ("clumping_proximity_factor", C.c_double),
+                ("optics", Optics), ("photons_per_sq_unit", C.c_double),
+                ("illumination_angle_rad", C.c_double), ("crystal_absorption_coeff", C.c_double),
+                ("depth", C.c_double)]
+
+
+class TileStatsC(C.Structure):
+    _fields_ = [("crystals", C.c_uint64), ("developed", C.c_uint64), ("clumps", C.c_uint64),
+                ("photons", C.c_uint64), ("ms_transport", C.c_float), ("ms_develop", C.c_float),
+                ("ms_scan", C.c_float), ("ms_total", C.c_float)]
+
+
+class FrameDescC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tile", C.c_uint32),
+                ("threads", C.c_uint32), ("pixel_size", C.c_double),
+                ("emulsion", EmulsionConfigC), ("exposure", ExposureConfigC),
+                ("development", DevelopmentConfigC), ("scan", ScanConfigC),
+                ("seed", C.c_uint64), ("exposure_map", _f32p), ("density_mult", _f32p)]
+
+
+class FrameStatsC(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "tiles", "crystals", "developed", "clumps", "photons", "rays", "pixels", "bounces",
+        "nodes_visited", "prims_tested", "abs_events")] + [
+        (n, C.c_float) for n in ("ms_total", "ms_transport", "ms_develop", "ms_scan",
+                                 "ms_kernel_transport", "ms_kernel_scan")]
+
+
+# every entry point of include/fgrain_hip.h: name -> (restype, argtypes)
+_vp = C.c_void_p
+_st = C.c_int
+SIGNATURES = {
+    "fg_abi_version": (C.c_uint32, []),
+    "fg_last_error": (C.c_char_p, []),
+    "fg_engine_create": (_st, [C.POINTER(_vp)]),
+    "fg_engine_destroy": (None, [_vp]),
+    "fg_engine_adapter_name": (C.c_char_p, [_vp]),
+    "fg_get_engine": (_st, [C.POINTER(_vp)]),
+    "fg_emulsion_synthesize": (_st, [C.POINTER(EmulsionConfigC), C.c_uint64, C.POINTER(_vp)]),
+    "fg_emulsion_from_crystals": (_st, [C.POINTER(EmulsionConfigC), C.c_int64, _f64p, _f64p,
+                                        _f64p, _i32p, _f64p, C.POINTER(_vp)]),
+    "fg_emulsion_destroy": (None, [_vp]),
+    "fg_emulsion_count": (C.c_int64, [_vp]),
+    "fg_emulsion_get_crystals": (_st, [_vp, _f64p, _f64p, _f64p, _i32p, _f64p]),
+    "fg_emulsion_get_bvh": (_st, [_vp, _i64p, _f64p, _f64p, _u32p, _u32p, _u32p, _u32p, _u32p]),
+    "fg_emulsion_get_device_layout": (_st, [_vp, _vp, _vp]),
+    "fg_emulsion_get_state": (_st, [_vp, _u32p, _u8p, _f64p, _i64p]),
+    "fg_emulsion_set_state": (_st, [_vp, _u32p, _u8p, _f64p]),
+    "fg_expose": (_st, [_vp, _vp, _f32p, C.c_uint32, C.c_uint32, C.POINTER(ExposureConfigC),
+                        C.c_uint64, C.POINTER(ExposeStatsC)]),
+    "fg_develop": (_st, [_vp, _vp, C.POINTER(DevelopmentConfigC), C.POINTER(DevelopmentReportC)]),
+    "fg_scan": (_st, [_vp, _vp, C.POINTER(ScanConfigC), C.c_uint64, _f32p]),
+    "fg_traverse_all_hits": (_st, [_vp, _vp, C.c_int64, _f32p, _f32p, C.c_int32, _i32p, _i32p,
+                                   _f32p, _f32p]),
+    "fg_traverse_first_hit": (_st, [_vp, _vp, C.c_int64, _f32p, _f32p, _i32p, _f32p, _f32p]),
+    "fg_tile_params_init": (_st, [C.POINTER(TileParamsC), C.c_char_p, C.c_double, C.c_uint32,
+                                  C.c_uint32, C.c_double, C.c_double, C.c_uint64]),
+    "fg_render_tile_all_gpu": (_st, [_vp, _f32p, C.POINTER(TileParamsC), _f32p,
+                                     C.POINTER(TileStatsC)]),
+    "fg_frame_create": (_st, [_vp, C.POINTER(FrameDescC), C.POINTER(_vp)]),
+    "fg_frame_destroy": (None, [_vp]),
+    "fg_frame_render": (_st, [_vp, _vp, C.c_uint32, C.POINTER(FrameStatsC)]),
+    "fg_frame_read": (_st, [_vp, _vp, _f32p]),
+    "fg_frame_read_counts": (_st, [_vp, _vp, C.c_int64, _u32p]),
+    "fg_frame_crystal_count": (C.c_int64, [_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"fgrain_amd: HIP library missing at {LIB_PATH}; "
+                              "run __graft_entry__.build() (hipcc, gfx950) first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class GpuError(RuntimeError):
+    """GpuError (src/error.rs:9-33): status code + payload string."""
+
+    def __init__(self, status: int, msg: str):
+        self.status = status
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+
+
+def check(status: int):
+    if status != FG_OK:
+        raise GpuError(status, lib().fg_last_error().decode(errors="replace"))
+
+
+def ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)

@@ -1,0 +1,992 @@
+// C ABI of the MI355X engine (include/fgrain_hip.h): engine lifecycle, the library
+// stages, the drop-in tile render and device-resident frames.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+
+using namespace fg;
+
+// ------------------------------------------------------------------ errors -------
+static thread_local std::string g_last_error;
+
+static fg_status fail(fg_status s, const std::string &msg)
+{
+    g_last_error = msg;
+    return s;
+}
+
+#define HIP_TRY(call, status)                                                          \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(status, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        const size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) cap = b;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+namespace fg {
+struct DeviceEmulsion {
+    DevBuf nodes, recs, counts;
+    uint32_t n_nodes = 0;
+    int64_t n = 0;
+};
+}  // namespace fg
+
+struct fg_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string name;
+    std::mutex mu;
+    DevBuf emap, stats, scan_nodes, scan_recs, out, out2, rays_o, rays_d, hits_n, hits_i,
+        hits_t0, hits_t1;
+};
+
+struct fg_emulsion {
+    Emulsion em;
+};
+
+// ------------------------------------------------------------------ engine -------
+extern "C" uint32_t fg_abi_version(void) { return FG_ABI_VERSION; }
+
+extern "C" const char *fg_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" fg_status fg_engine_create(fg_engine **out)
+{
+    if (!out) return fail(FG_ERR_INVALID_ARG, "fg_engine_create: null output");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(FG_ERR_ADAPTER_NOT_FOUND,
+                    std::string("no HIP device: ") + (e != hipSuccess ? hipGetErrorString(e) : "0 devices"));
+    fg_engine *E = new fg_engine();
+    e = hipGetDevice(&E->device);
+    if (e == hipSuccess) e = hipSetDevice(E->device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, E->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev1);
+    if (e != hipSuccess) {
+        delete E;
+        return fail(FG_ERR_DEVICE_REQUEST, std::string("HIP device setup: ") + hipGetErrorString(e));
+    }
+    E->name = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+              std::to_string(prop.multiProcessorCount) + " CUs)";
+    *out = E;
+    return FG_OK;
+}
+
+extern "C" void fg_engine_destroy(fg_engine *E)
+{
+    if (!E) return;
+    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->out, &E->out2,
+                      &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1})
+        b->release();
+    if (E->ev0) (void)hipEventDestroy(E->ev0);
+    if (E->ev1) (void)hipEventDestroy(E->ev1);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+}
+
+extern "C" const char *fg_engine_adapter_name(const fg_engine *E) { return E ? E->name.c_str() : ""; }
+
+static std::once_flag g_engine_once;
+static fg_engine *g_engine = nullptr;
+static fg_status g_engine_status = FG_OK;
+static std::string g_engine_msg;
+
+extern "C" fg_status fg_get_engine(fg_engine **out)
+{
+    if (!out) return fail(FG_ERR_INVALID_ARG, "fg_get_engine: null output");
+    std::call_once(g_engine_once, [] {
+        g_engine_status = fg_engine_create(&g_engine);
+        if (g_engine_status != FG_OK) g_engine_msg = g_last_error;
+    });
+    *out = g_engine;
+    if (g_engine_status != FG_OK) return fail(g_engine_status, g_engine_msg);
+    return FG_OK;
+}
+
+// ---------------------------------------------------------------- emulsion -------
+static bool valid_cfg(const fg_emulsion_config *c, std::string &msg)
+{
+    if (!c) { msg = "null emulsion config"; return false; }
+    if (!(c->width >= 0 && c->height >= 0 && c->depth > 0)) { msg = "emulsion extents must be >= 0 (depth > 0)"; return false; }
+    if (!(c->r_min > 0)) { msg = "r_min must be > 0"; return false; }
+    if (c->morphology > 2) { msg = "morphology must be 0, 1 or 2"; return false; }
+    if (c->morphology == 1 && !(c->tabular_aspect_ratio > 0)) { msg = "tabular aspect ratio must be > 0"; return false; }
+    const double cells = (c->width / (c->r_min / sqrt(3.0)) + 1) * (c->height / (c->r_min / sqrt(3.0)) + 1) *
+                         (c->depth / (c->r_min / sqrt(3.0)) + 1);
+    if (cells > 4e9) { msg = "emulsion too large for one Poisson grid"; return false; }
+    return true;
+}
+
+extern "C" fg_status fg_emulsion_synthesize(const fg_emulsion_config *cfg, uint64_t seed,
+                                            fg_emulsion **out)
+{
+    std::string msg;
+    if (!out || !valid_cfg(cfg, msg)) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_synthesize: " + msg);
+    fg_emulsion *E = new fg_emulsion();
+    synthesize(*cfg, seed, E->em);
+    if (!finalize_emulsion(E->em, msg)) {
+        delete E;
+        return fail(FG_ERR_INVALID_ARG, "fg_emulsion_synthesize: " + msg);
+    }
+    *out = E;
+    return FG_OK;
+}
+
+extern "C" fg_status fg_emulsion_from_crystals(const fg_emulsion_config *cfg, int64_t n,
+                                               const double *pos, const double *dims,
+                                               const double *quat, const int32_t *morph,
+                                               const double *sens, fg_emulsion **out)
+{
+    std::string msg;
+    if (!out || n < 0 || (n > 0 && (!pos || !dims || !quat || !morph || !sens)) || !cfg)
+        return fail(FG_ERR_INVALID_ARG, "fg_emulsion_from_crystals: null or negative argument");
+    if (n > (int64_t)(1u << 27)) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_from_crystals: too many crystals");
+    for (int64_t i = 0; i < n; i++)
+        if (morph[i] < 0 || morph[i] > 2) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_from_crystals: bad morphology");
+    fg_emulsion *E = new fg_emulsion();
+    Emulsion &em = E->em;
+    em.cfg = *cfg;
+    em.n = n;
+    em.pos.assign(pos, pos + 3 * n);
+    em.dims.assign(dims, dims + 3 * n);
+    em.quat.assign(quat, quat + 4 * n);
+    em.morph.assign(morph, morph + n);
+    em.sens.assign(sens, sens + n);
+    if (!finalize_emulsion(em, msg)) {
+        delete E;
+        return fail(FG_ERR_INVALID_ARG, "fg_emulsion_from_crystals: " + msg);
+    }
+    *out = E;
+    return FG_OK;
+}
+
+static void free_device(Emulsion &em)
+{
+    if (em.dev) {
+        em.dev->nodes.release();
+        em.dev->recs.release();
+        em.dev->counts.release();
+        delete em.dev;
+        em.dev = nullptr;
+    }
+}
+
+extern "C" void fg_emulsion_destroy(fg_emulsion *E)
+{
+    if (!E) return;
+    free_device(E->em);
+    delete E;
+}
+
+extern "C" int64_t fg_emulsion_count(const fg_emulsion *E) { return E ? E->em.n : 0; }
+
+extern "C" fg_status fg_emulsion_get_crystals(const fg_emulsion *E, double *pos, double *dims,
+                                              double *quat, int32_t *morph, double *sens)
+{
+    if (!E) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_get_crystals: null emulsion");
+    const Emulsion &em = E->em;
+    if (pos) memcpy(pos, em.pos.data(), em.pos.size() * sizeof(double));
+    if (dims) memcpy(dims, em.dims.data(), em.dims.size() * sizeof(double));
+    if (quat) memcpy(quat, em.quat.data(), em.quat.size() * sizeof(double));
+    if (morph) memcpy(morph, em.morph.data(), em.morph.size() * sizeof(int32_t));
+    if (sens) memcpy(sens, em.sens.data(), em.sens.size() * sizeof(double));
+    return FG_OK;
+}
+
+extern "C" fg_status fg_emulsion_get_bvh(const fg_emulsion *E, int64_t *n_nodes, double *bmin,
+                                         double *bmax, uint32_t *left, uint32_t *right,
+                                         uint32_t *offset, uint32_t *count, uint32_t *indices)
+{
+    if (!E || !n_nodes) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_get_bvh: null argument");
+    const RefBvh &b = E->em.bvh;
+    const int64_t nn = b.nodes();
+    *n_nodes = nn;
+    if (bmin) memcpy(bmin, b.mn.data(), (size_t)nn * 3 * sizeof(double));
+    if (bmax) memcpy(bmax, b.mx.data(), (size_t)nn * 3 * sizeof(double));
+    if (left) memcpy(left, b.left.data(), (size_t)nn * sizeof(uint32_t));
+    if (right) memcpy(right, b.right.data(), (size_t)nn * sizeof(uint32_t));
+    if (offset) memcpy(offset, b.offset.data(), (size_t)nn * sizeof(uint32_t));
+    if (count) memcpy(count, b.count.data(), (size_t)nn * sizeof(uint32_t));
+    if (indices) memcpy(indices, b.idx.data(), b.idx.size() * sizeof(uint32_t));
+    return FG_OK;
+}
+
+extern "C" fg_status fg_emulsion_get_device_layout(const fg_emulsion *E, void *nodes, void *records)
+{
+    if (!E) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_get_device_layout: null emulsion");
+    if (nodes) memcpy(nodes, E->em.nodes.data(), E->em.nodes.size() * sizeof(Node));
+    if (records) memcpy(records, E->em.recs.data(), E->em.recs.size() * sizeof(Crystal));
+    return FG_OK;
+}
+
+extern "C" fg_status fg_emulsion_get_state(const fg_emulsion *E, uint32_t *counts,
+                                           uint8_t *developed, double *fe, int64_t *clump_id)
+{
+    if (!E) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_get_state: null emulsion");
+    const Emulsion &em = E->em;
+    if (counts) memcpy(counts, em.counts.data(), em.counts.size() * sizeof(uint32_t));
+    if (developed) memcpy(developed, em.developed.data(), em.developed.size());
+    if (fe) memcpy(fe, em.fe.data(), em.fe.size() * sizeof(double));
+    if (clump_id) memcpy(clump_id, em.clump.data(), em.clump.size() * sizeof(int64_t));
+    return FG_OK;
+}
+
+extern "C" fg_status fg_emulsion_set_state(fg_emulsion *E, const uint32_t *counts,
+                                           const uint8_t *developed, const double *fe)
+{
+    if (!E) return fail(FG_ERR_INVALID_ARG, "fg_emulsion_set_state: null emulsion");
+    Emulsion &em = E->em;
+    if (counts) em.counts.assign(counts, counts + em.n);
+    if (developed) em.developed.assign(developed, developed + em.n);
+    if (fe) em.fe.assign(fe, fe + em.n);
+    em.dev_state_dirty = true;
+    return FG_OK;
+}
+
+// ------------------------------------------------------------- device helpers ---
+static fg_status ensure_device(fg_engine *E, Emulsion &em)
+{
+    if (em.dev) return FG_OK;
+    em.dev = new DeviceEmulsion();
+    DeviceEmulsion &d = *em.dev;
+    d.n = em.n;
+    d.n_nodes = (uint32_t)em.nodes.size();
+    HIP_TRY(d.nodes.ensure(em.nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(d.recs.ensure(em.recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(d.counts.ensure((size_t)em.n * sizeof(uint32_t)), FG_ERR_BUFFER_RANGE);
+    if (!em.nodes.empty())
+        HIP_TRY(hipMemcpyAsync(d.nodes.p, em.nodes.data(), em.nodes.size() * sizeof(Node),
+                               hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    if (!em.recs.empty())
+        HIP_TRY(hipMemcpyAsync(d.recs.p, em.recs.data(), em.recs.size() * sizeof(Crystal),
+                               hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    return FG_OK;
+}
+
+static void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const fg_exposure_config &x,
+                           uint32_t tw, uint32_t th, uint64_t seed)
+{
+    const fg_optics &o = ec.optics;
+    const double mu_t = o.absorption_coeff + o.scattering_coeff;
+    const double r = (1.0 - o.refractive_index) / (1.0 + o.refractive_index);
+    a.W = (float)ec.width;
+    a.H = (float)ec.height;
+    a.D = (float)ec.depth;
+    a.pw = (float)(ec.width / (double)std::max(tw, 1u));
+    a.ph = (float)(ec.height / (double)std::max(th, 1u));
+    a.mu_t = (float)mu_t;
+    a.albedo = (float)(mu_t > 0.0 ? o.scattering_coeff / mu_t : 0.0);
+    a.r0 = (float)(r * r);
+    a.hal_eff = (float)(o.antihalation_reflectance * pow(10.0, -2.0 * o.antihalation_density));
+    a.mu_abs = (float)x.crystal_absorption_coeff;
+    a.theta_max = (float)x.illumination_angle_rad;
+    a.g = (float)o.anisotropy_g;
+    a.step_cap = (float)(ec.depth * 1.5);
+    a.max_bounces = x.max_bounces;
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    a.tw = std::max(tw, 1u);
+    a.th = std::max(th, 1u);
+    a.ppp = x.photons_per_pixel;
+    if (x.photons_per_pixel > 0)
+        a.n_photons = (uint64_t)x.photons_per_pixel * a.tw * a.th;
+    else
+        a.n_photons = (uint64_t)llround(ec.width * ec.height * x.photons_per_sq_unit);
+}
+
+static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, uint32_t tw,
+                               uint32_t th, const fg_exposure_config &cfg, uint64_t seed,
+                               fg_expose_stats *stats)
+{
+    fg_status s = ensure_device(E, em);
+    if (s != FG_OK) return s;
+    DeviceEmulsion &d = *em.dev;
+    TransportArgs a{};
+    a.nodes = d.nodes.as<Node>();
+    a.recs = d.recs.as<float4>();
+    a.n_nodes = d.n_nodes;
+    fill_transport(a, em.cfg, cfg, emap ? tw : 1, emap ? th : 1, seed);
+    if (emap) {
+        HIP_TRY(E->emap.ensure((size_t)tw * th * sizeof(float)), FG_ERR_BUFFER_RANGE);
+        HIP_TRY(hipMemcpyAsync(E->emap.p, emap, (size_t)tw * th * sizeof(float), hipMemcpyHostToDevice,
+                               E->stream), FG_ERR_BUFFER_MAP);
+        a.emap = E->emap.as<float>();
+        a.estride = tw;
+    }
+    HIP_TRY(E->stats.ensure(8 * sizeof(unsigned long long)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipMemsetAsync(E->stats.p, 0, 8 * sizeof(unsigned long long), E->stream), FG_ERR_BUFFER_MAP);
+    if (em.n > 0) HIP_TRY(hipMemsetAsync(d.counts.p, 0, (size_t)em.n * 4, E->stream), FG_ERR_BUFFER_MAP);
+    a.counts = d.counts.as<uint32_t>();
+    a.stats = E->stats.as<unsigned long long>();
+    HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(launch_transport(a, E->stream), FG_ERR_PIPELINE);
+    HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
+    std::vector<uint32_t> c((size_t)em.n);
+    unsigned long long st[8];
+    if (em.n > 0)
+        HIP_TRY(hipMemcpyAsync(c.data(), d.counts.p, (size_t)em.n * 4, hipMemcpyDeviceToHost, E->stream),
+                FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipMemcpyAsync(st, E->stats.p, sizeof st, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    for (size_t k = 0; k < c.size(); k++) em.counts[em.bvh.idx[k]] += c[k];
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        stats->launched = a.n_photons;
+        stats->bounces = st[0];
+        stats->nodes_visited = st[1];
+        stats->prims_tested = st[2];
+        stats->abs_events = st[3];
+        stats->abs_total = st[4];
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
+        stats->kernel_ms = ms;
+    }
+    return FG_OK;
+}
+
+static bool valid_exposure(const fg_exposure_config *c, std::string &msg)
+{
+    if (!c) { msg = "null exposure config"; return false; }
+    if (!(c->illumination_angle_rad >= 0.0)) { msg = "illumination angle must be >= 0"; return false; }
+    if (!(c->photons_per_sq_unit >= 0.0)) { msg = "photon density must be >= 0"; return false; }
+    return true;
+}
+
+extern "C" fg_status fg_expose(fg_engine *E, fg_emulsion *em, const float *emap, uint32_t tw,
+                               uint32_t th, const fg_exposure_config *cfg, uint64_t seed,
+                               fg_expose_stats *stats)
+{
+    std::string msg;
+    if (!E || !em) return fail(FG_ERR_INVALID_ARG, "fg_expose: null engine or emulsion");
+    if (!valid_exposure(cfg, msg)) return fail(FG_ERR_INVALID_ARG, "fg_expose: " + msg);
+    if (emap && (tw == 0 || th == 0)) return fail(FG_ERR_INVALID_ARG, "fg_expose: empty exposure map");
+    if (cfg->photons_per_pixel > 0 && !emap)
+        return fail(FG_ERR_INVALID_ARG, "fg_expose: photons_per_pixel needs an exposure map");
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    return expose_locked(E, em->em, emap, tw, th, *cfg, seed, stats);
+}
+
+extern "C" fg_status fg_develop(fg_engine *E, fg_emulsion *em, const fg_development_config *cfg,
+                                fg_development_report *report)
+{
+    if (!E || !em || !cfg) return fail(FG_ERR_INVALID_ARG, "fg_develop: null argument");
+    std::lock_guard<std::mutex> lk(E->mu);
+    fg_development_report rep;
+    develop_host(em->em, *cfg, rep);
+    if (report) *report = rep;
+    return FG_OK;
+}
+
+// Developed-crystal BVH (transmittance.rs:82-89) in the threaded layout.
+static bool build_scan_bvh(const Emulsion &em, std::vector<Node> &nodes, std::vector<Crystal> &recs,
+                           std::string &msg)
+{
+    std::vector<uint32_t> subset;
+    for (int64_t i = 0; i < em.n; i++)
+        if (em.developed[(size_t)i]) subset.push_back((uint32_t)i);
+    RefBvh b;
+    build_ref_bvh((int64_t)subset.size(), em.pos.data(), em.dims.data(), em.developed.data(),
+                  em.fe.data(), subset.data(), b);
+    if (!flatten_bvh(b, nodes, msg)) return false;
+    // records in leaf order of the subset BVH
+    RefBvh mapped = b;
+    for (auto &k : mapped.idx) k = subset[k];
+    make_records(em, mapped, em.developed.data(), em.fe.data(), recs);
+    return true;
+}
+
+static bool valid_scan(const fg_scan_config *c, std::string &msg)
+{
+    if (!c) { msg = "null scan config"; return false; }
+    if (c->mode > 3) { msg = "scan mode must be 0..3"; return false; }
+    if (c->supersample > 8) { msg = "supersample must be <= 8"; return false; }
+    if (c->samples_per_pixel > 1024) { msg = "samples_per_pixel must be <= 1024"; return false; }
+    if ((uint64_t)c->width_px * c->height_px > (1ull << 31)) { msg = "scan grid too large"; return false; }
+    return true;
+}
+
+static fg_status scan_locked(fg_engine *E, const Emulsion &em, const fg_scan_config &cfg, uint64_t seed,
+                             float *out, float *kernel_ms)
+{
+    std::vector<Node> nodes;
+    std::vector<Crystal> recs;
+    std::string msg;
+    if (!build_scan_bvh(em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "scan: " + msg);
+    HIP_TRY(E->scan_nodes.ensure(nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->scan_recs.ensure(recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
+    if (!nodes.empty())
+        HIP_TRY(hipMemcpyAsync(E->scan_nodes.p, nodes.data(), nodes.size() * sizeof(Node),
+                               hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    if (!recs.empty())
+        HIP_TRY(hipMemcpyAsync(E->scan_recs.p, recs.data(), recs.size() * sizeof(Crystal),
+                               hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    const size_t npx = (size_t)cfg.width_px * cfg.height_px;
+    HIP_TRY(E->out.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
+    ScanArgs a{};
+    a.nodes = E->scan_nodes.as<Node>();
+    a.recs = E->scan_recs.as<float4>();
+    a.n_nodes = (uint32_t)nodes.size();
+    a.w_px = cfg.width_px;
+    a.h_px = cfg.height_px;
+    const uint32_t spp = std::max(cfg.samples_per_pixel, 1u);
+    a.g = std::max((uint32_t)llround(sqrt((double)spp)), 1u);
+    a.ss = std::max(cfg.supersample, 1u);
+    a.mode = cfg.mode;
+    a.pw = (float)(em.cfg.width / (double)cfg.width_px);
+    a.ph = (float)(em.cfg.height / (double)cfg.height_px);
+    a.z0 = (float)(em.cfg.depth + 0.1);
+    a.sigma = (float)cfg.silver_opacity;
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    a.out = E->out.as<float>();
+    a.ostride = cfg.width_px;
+    HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(launch_scan(a, E->stream), FG_ERR_PIPELINE);
+    HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
+    const float *res = E->out.as<float>();
+    if (cfg.otf) {
+        HIP_TRY(E->out2.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
+        HIP_TRY(launch_otf3(E->out.as<float>(), E->out2.as<float>(), cfg.width_px, cfg.height_px, E->stream),
+                FG_ERR_PIPELINE);
+        res = E->out2.as<float>();
+    }
+    HIP_TRY(hipMemcpyAsync(out, res, npx * sizeof(float), hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    if (kernel_ms) (void)hipEventElapsedTime(kernel_ms, E->ev0, E->ev1);
+    return FG_OK;
+}
+
+extern "C" fg_status fg_scan(fg_engine *E, const fg_emulsion *em, const fg_scan_config *cfg,
+                             uint64_t seed, float *out)
+{
+    std::string msg;
+    if (!E || !em || !out) return fail(FG_ERR_INVALID_ARG, "fg_scan: null argument");
+    if (!valid_scan(cfg, msg)) return fail(FG_ERR_INVALID_ARG, "fg_scan: " + msg);
+    if (cfg->width_px == 0 || cfg->height_px == 0) return FG_OK;
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    return scan_locked(E, em->em, *cfg, seed, out, nullptr);
+}
+
+static fg_status traverse(fg_engine *E, const fg_emulsion *emw, int64_t n_rays, const float *o,
+                          const float *d, int32_t max_hits, int32_t *n_hits, int32_t *idx, float *t0,
+                          float *t1, bool first)
+{
+    if (!E || !emw || n_rays < 0 || (n_rays > 0 && (!o || !d || !idx || !t0 || !t1)) ||
+        (!first && (max_hits <= 0 || !n_hits)))
+        return fail(FG_ERR_INVALID_ARG, "traverse: bad argument");
+    if (n_rays == 0) return FG_OK;
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    Emulsion &em = const_cast<Emulsion &>(emw->em);
+    fg_status s = ensure_device(E, em);
+    if (s != FG_OK) return s;
+    const int64_t per = first ? 1 : max_hits;
+    HIP_TRY(E->rays_o.ensure((size_t)n_rays * 12), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->rays_d.ensure((size_t)n_rays * 12), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->hits_n.ensure((size_t)n_rays * 4), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->hits_i.ensure((size_t)(n_rays * per) * 4), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->hits_t0.ensure((size_t)(n_rays * per) * 4), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->hits_t1.ensure((size_t)(n_rays * per) * 4), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipMemcpyAsync(E->rays_o.p, o, (size_t)n_rays * 12, hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipMemcpyAsync(E->rays_d.p, d, (size_t)n_rays * 12, hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    DeviceEmulsion &dv = *em.dev;
+    if (first)
+        HIP_TRY(launch_first_hit(dv.nodes.as<Node>(), dv.n_nodes, dv.recs.as<float4>(), E->rays_o.as<float>(),
+                                 E->rays_d.as<float>(), n_rays, E->hits_i.as<int32_t>(), E->hits_t0.as<float>(),
+                                 E->hits_t1.as<float>(), E->stream), FG_ERR_PIPELINE);
+    else
+        HIP_TRY(launch_all_hits(dv.nodes.as<Node>(), dv.n_nodes, dv.recs.as<float4>(), E->rays_o.as<float>(),
+                                E->rays_d.as<float>(), n_rays, max_hits, E->hits_n.as<int32_t>(),
+                                E->hits_i.as<int32_t>(), E->hits_t0.as<float>(), E->hits_t1.as<float>(),
+                                E->stream), FG_ERR_PIPELINE);
+    const size_t m = (size_t)(n_rays * per);
+    if (!first)
+        HIP_TRY(hipMemcpyAsync(n_hits, E->hits_n.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, E->stream),
+                FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipMemcpyAsync(idx, E->hits_i.p, m * 4, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipMemcpyAsync(t0, E->hits_t0.p, m * 4, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipMemcpyAsync(t1, E->hits_t1.p, m * 4, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    // leaf slots -> crystal indices
+    for (int64_t r = 0; r < n_rays; r++) {
+        const int64_t nh = first ? 1 : std::min<int64_t>(n_hits[r], max_hits);
+        for (int64_t k = 0; k < nh; k++) {
+            int32_t &v = idx[r * per + k];
+            if (v >= 0) v = (int32_t)em.bvh.idx[(size_t)v];
+        }
+    }
+    return FG_OK;
+}
+
+extern "C" fg_status fg_traverse_all_hits(fg_engine *E, const fg_emulsion *em, int64_t n_rays,
+                                          const float *o, const float *d, int32_t max_hits,
+                                          int32_t *n_hits, int32_t *idx, float *t0, float *t1)
+{
+    return traverse(E, em, n_rays, o, d, max_hits, n_hits, idx, t0, t1, false);
+}
+
+extern "C" fg_status fg_traverse_first_hit(fg_engine *E, const fg_emulsion *em, int64_t n_rays,
+                                           const float *o, const float *d, int32_t *idx, float *t0,
+                                           float *t1)
+{
+    return traverse(E, em, n_rays, o, d, 1, nullptr, idx, t0, t1, true);
+}
+
+// ------------------------------------------------------------- tile render -------
+extern "C" fg_status fg_tile_params_init(fg_tile_params *p, const char *preset, double grain_size_px,
+                                         uint32_t tw, uint32_t th, double exposure, double push,
+                                         uint64_t tile_seed)
+{
+    if (!p || !preset || !(grain_size_px > 0))
+        return fail(FG_ERR_INVALID_ARG, "fg_tile_params_init: bad argument");
+    std::string s(preset);
+    for (auto &c : s) c = (char)tolower((unsigned char)c);
+    const bool tmax = s == "t-max" || s == "tmax";
+    const bool hp5 = s == "hp5" || s == "hp5-plus";
+    const bool named1600 = s == "tri-x-1600" || s == "trix-1600" || s == "caffenol";
+    // effective push (image_pipeline.rs:368-374)
+    const double ep = (s.find("1600") != std::string::npos || s == "caffenol") ? std::max(push, 2.0)
+                                                                              : std::max(push, 0.0);
+    memset(p, 0, sizeof *p);
+    const double px_um = 1.0 / grain_size_px;
+    p->film_w = tw * px_um;
+    p->film_h = th * px_um;
+    p->width_px = tw;
+    p->height_px = th;
+    p->samples_per_pixel = 4;
+    p->seed = tile_seed;
+    p->exposure = exposure;
+    p->threshold_photons = 4;
+    p->max_bounces = 48;
+    // emulsion preset as render_tile_cpu_full selects it (image_pipeline.rs:925-938)
+    fg_optics o{1.534, 0.003, 0.050, 0.80, 0.14, 0.80};
+    if (tmax) {
+        p->depth = 8.0; p->r_min = 0.9; p->mean_crystal_radius = 0.40; p->size_variance = 0.20;
+        p->morphology = 1; p->tabular_aspect_ratio = 6.5; p->base_sensitivity = 0.8;
+        o = {1.534, 0.002, 0.035, 0.85, 0.08, 1.10};
+    } else if (hp5) {
+        p->depth = 12.5; p->r_min = 1.1; p->mean_crystal_radius = 0.50; p->size_variance = 0.30;
+        p->morphology = 2; p->tabular_aspect_ratio = 1.0; p->base_sensitivity = 1.1;
+        o = {1.534, 0.0025, 0.045, 0.82, 0.12, 0.85};
+    } else if (named1600 || ep >= 1.5) {
+        p->depth = 14.5; p->r_min = 1.1; p->mean_crystal_radius = 0.52; p->size_variance = 0.38;
+        p->morphology = 0; p->tabular_aspect_ratio = 1.0; p->base_sensitivity = 1.35;
+        o = {1.534, 0.0032, 0.052, 0.80, 0.14, 0.80};
+    } else {
+        p->depth = 14.0; p->r_min = 1.2; p->mean_crystal_radius = 0.55; p->size_variance = 0.40;
+        p->morphology = 0; p->tabular_aspect_ratio = 1.0; p->base_sensitivity = 1.2;
+    }
+    p->film_d = p->depth;
+    p->optics = o;
+    // development (image_pipeline.rs:963-976)
+    p->filament_expansion = tmax ? 1.10 + 0.04 * ep : hp5 ? 1.15 + 0.05 * ep : 1.20 + 0.06 * ep;
+    p->clumping_proximity_factor = tmax ? 1.10 + 0.03 * ep : hp5 ? 1.18 + 0.04 * ep : 1.25 + 0.05 * ep;
+    // exposure (image_pipeline.rs:942-947) and scan (image_pipeline.rs:996-1002)
+    p->photons_per_sq_unit = 6.0 * std::max(exposure, 0.05);
+    p->illumination_angle_rad = 0.12;
+    p->crystal_absorption_coeff = 1.3;
+    p->silver_opacity = 0.25;
+    return FG_OK;
+}
+
+static fg_emulsion_config tile_emulsion_config(const fg_tile_params &p)
+{
+    fg_emulsion_config c{};
+    c.width = p.film_w;
+    c.height = p.film_h;
+    c.depth = p.depth > 0 ? p.depth : p.film_d;
+    c.r_min = p.r_min;
+    c.mean_crystal_radius = p.mean_crystal_radius;
+    c.size_variance = p.size_variance;
+    c.tabular_aspect_ratio = p.tabular_aspect_ratio;
+    c.base_sensitivity = p.base_sensitivity;
+    c.sensitivity_sigma = 0.15;
+    c.min_radius = 0.15;
+    c.min_half_thickness = 0.05;
+    c.morphology = p.morphology;
+    c.k_samples = 30;
+    c.optics = p.optics;
+    return c;
+}
+
+using Clock = std::chrono::steady_clock;
+static float ms_since(Clock::time_point t0)
+{
+    return std::chrono::duration<float, std::milli>(Clock::now() - t0).count();
+}
+
+extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img, const fg_tile_params *p,
+                                            float *out, fg_tile_stats *stats)
+{
+    if (!E || !p) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: null argument");
+    if ((uint64_t)p->width_px * p->height_px == 0) return FG_OK;
+    if (!tile_img || !out) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: null buffer");
+    const fg_emulsion_config ec = tile_emulsion_config(*p);
+    std::string msg;
+    if (!valid_cfg(&ec, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: " + msg);
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    const auto t0 = Clock::now();
+    fg_emulsion em;
+    synthesize(ec, p->seed, em.em);
+    if (!finalize_emulsion(em.em, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: " + msg);
+    fg_exposure_config xc{};
+    xc.photons_per_sq_unit = p->photons_per_sq_unit;
+    xc.illumination_angle_rad = p->illumination_angle_rad;
+    xc.crystal_absorption_coeff = p->crystal_absorption_coeff;
+    xc.max_bounces = p->max_bounces;
+    xc.photons_per_pixel = 0;
+    fg_expose_stats xs;
+    const auto t1 = Clock::now();
+    fg_status s = expose_locked(E, em.em, tile_img, p->width_px, p->height_px, xc, p->seed + 1, &xs);
+    free_device(em.em);
+    if (s != FG_OK) return s;
+    const auto t2 = Clock::now();
+    fg_development_config dc{p->threshold_photons, 0, p->filament_expansion, p->clumping_proximity_factor, 0.85};
+    fg_development_report rep;
+    develop_host(em.em, dc, rep);
+    const auto t3 = Clock::now();
+    fg_scan_config sc{p->width_px, p->height_px, p->samples_per_pixel, FG_SCAN_DNEG, p->silver_opacity, 1, 0};
+    s = scan_locked(E, em.em, sc, p->seed + 2, out, nullptr);
+    if (s != FG_OK) return s;
+    if (stats) {
+        stats->crystals = (uint64_t)em.em.n;
+        stats->developed = rep.developed_crystals;
+        stats->clumps = rep.total_clumps;
+        stats->photons = xs.launched;
+        stats->ms_transport = std::chrono::duration<float, std::milli>(t2 - t1).count();
+        stats->ms_develop = std::chrono::duration<float, std::milli>(t3 - t2).count();
+        stats->ms_scan = ms_since(t3);
+        stats->ms_total = ms_since(t0);
+    }
+    return FG_OK;
+}
+
+// -------------------------------------------------------------------- frames -----
+struct TileRec {
+    uint32_t x0, y0, tw, th;
+    uint64_t seed;
+    int64_t crystal_base, node_base;
+    uint32_t n_nodes;
+    int64_t n;
+};
+
+struct fg_frame {
+    fg_frame_desc desc{};
+    std::vector<float> emap;
+    std::vector<TileRec> tiles;
+    std::vector<Emulsion> ems;
+    int64_t n_crystals = 0, n_nodes = 0;
+    DevBuf d_nodes, d_recs, d_counts, d_emap, d_dneg, d_dneg2, d_stats, d_scan_nodes, d_scan_recs;
+    bool rendered = false;
+    bool otf_applied = false;
+};
+
+extern "C" int64_t fg_frame_crystal_count(const fg_frame *F) { return F ? F->n_crystals : 0; }
+
+extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg_frame **out)
+{
+    if (!E || !desc || !out || !desc->exposure_map)
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: null argument");
+    if (desc->width == 0 || desc->height == 0 || desc->tile == 0 || !(desc->pixel_size > 0))
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: empty frame or tile");
+    std::string msg;
+    if (!valid_exposure(&desc->exposure, msg) || !valid_scan(&desc->scan, msg))
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: " + msg);
+    fg_frame *F = new fg_frame();
+    F->desc = *desc;
+    F->emap.assign(desc->exposure_map, desc->exposure_map + (size_t)desc->width * desc->height);
+    F->desc.exposure_map = nullptr;
+    F->desc.density_mult = nullptr;
+    const uint32_t nx = (desc->width + desc->tile - 1) / desc->tile;
+    const uint32_t ny = (desc->height + desc->tile - 1) / desc->tile;
+    std::vector<double> mult((size_t)nx * ny, 1.0);
+    if (desc->density_mult)
+        for (size_t k = 0; k < mult.size(); k++) mult[k] = desc->density_mult[k];
+    for (uint32_t ty = 0; ty < ny; ty++)
+        for (uint32_t tx = 0; tx < nx; tx++) {
+            TileRec t{};
+            t.x0 = tx * desc->tile;
+            t.y0 = ty * desc->tile;
+            t.tw = std::min(desc->tile, desc->width - t.x0);
+            t.th = std::min(desc->tile, desc->height - t.y0);
+            const uint64_t idx = (uint64_t)ty * nx + tx;
+            // tile seed (image_pipeline.rs:430-434)
+            t.seed = desc->seed + idx * 73856093ull + (uint64_t)tx * 19349663ull + (uint64_t)ty * 83492791ull;
+            F->tiles.push_back(t);
+        }
+    F->ems.resize(F->tiles.size());
+    std::atomic<size_t> next{0};
+    std::atomic<bool> bad{false};
+    std::string err;
+    std::mutex emu;
+    unsigned nt = desc->threads ? desc->threads : std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min<unsigned>(nt, (unsigned)F->tiles.size());
+    auto work = [&]() {
+        for (;;) {
+            const size_t k = next.fetch_add(1);
+            if (k >= F->tiles.size()) break;
+            const TileRec &t = F->tiles[k];
+            fg_emulsion_config c = desc->emulsion;
+            c.width = t.tw * desc->pixel_size;
+            c.height = t.th * desc->pixel_size;
+            c.r_min = desc->emulsion.r_min / cbrt(std::max(mult[k], 1e-3));
+            synthesize(c, t.seed, F->ems[k]);
+            std::string m;
+            if (!finalize_emulsion(F->ems[k], m)) {
+                std::lock_guard<std::mutex> g(emu);
+                bad = true;
+                err = m;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned i = 0; i < nt; i++) pool.emplace_back(work);
+    for (auto &th : pool) th.join();
+    if (bad) {
+        delete F;
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: " + err);
+    }
+    for (size_t k = 0; k < F->tiles.size(); k++) {
+        TileRec &t = F->tiles[k];
+        t.crystal_base = F->n_crystals;
+        t.node_base = F->n_nodes;
+        t.n = F->ems[k].n;
+        t.n_nodes = (uint32_t)F->ems[k].nodes.size();
+        F->n_crystals += t.n;
+        F->n_nodes += t.n_nodes;
+    }
+    std::lock_guard<std::mutex> lk(E->mu);
+    auto cleanup = [&](fg_status s) { fg_frame_destroy(F); return s; };
+    hipError_t e = hipSetDevice(E->device);
+    if (e == hipSuccess) e = F->d_nodes.ensure((size_t)F->n_nodes * sizeof(Node));
+    if (e == hipSuccess) e = F->d_recs.ensure((size_t)F->n_crystals * sizeof(Crystal));
+    if (e == hipSuccess) e = F->d_counts.ensure((size_t)F->n_crystals * 4);
+    if (e == hipSuccess) e = F->d_emap.ensure(F->emap.size() * 4);
+    if (e == hipSuccess) e = F->d_dneg.ensure(F->emap.size() * 4);
+    if (e == hipSuccess) e = F->d_dneg2.ensure(F->emap.size() * 4);
+    if (e == hipSuccess) e = F->d_stats.ensure(64);
+    if (e != hipSuccess) return cleanup(fail(FG_ERR_BUFFER_RANGE, std::string("frame allocation: ") + hipGetErrorString(e)));
+    for (size_t k = 0; k < F->tiles.size() && e == hipSuccess; k++) {
+        const TileRec &t = F->tiles[k];
+        if (t.n_nodes)
+            e = hipMemcpy(F->d_nodes.as<Node>() + t.node_base, F->ems[k].nodes.data(), t.n_nodes * sizeof(Node),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess && t.n)
+            e = hipMemcpy(F->d_recs.as<Crystal>() + t.crystal_base, F->ems[k].recs.data(),
+                          (size_t)t.n * sizeof(Crystal), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(F->d_emap.p, F->emap.data(), F->emap.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(fail(FG_ERR_BUFFER_MAP, std::string("frame upload: ") + hipGetErrorString(e)));
+    *out = F;
+    return FG_OK;
+}
+
+extern "C" void fg_frame_destroy(fg_frame *F)
+{
+    if (!F) return;
+    for (DevBuf *b : {&F->d_nodes, &F->d_recs, &F->d_counts, &F->d_emap, &F->d_dneg, &F->d_dneg2,
+                      &F->d_stats, &F->d_scan_nodes, &F->d_scan_recs})
+        b->release();
+    delete F;
+}
+
+// Plain per-tile path: per tile one transport launch, a synchronisation and readback of
+// the latent counts, development and the developed-crystal BVH on the host, one scan
+// launch and a synchronisation -- the shape of render_tile_all_gpu called per tile.
+static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *st)
+{
+    const fg_frame_desc &D = F->desc;
+    std::vector<uint32_t> c;
+    float ms_tk = 0, ms_sk = 0;
+    double t_tr = 0, t_dev = 0, t_sc = 0;
+    HIP_TRY(hipMemsetAsync(F->d_stats.p, 0, 64, E->stream), FG_ERR_BUFFER_MAP);
+    for (size_t k = 0; k < F->tiles.size(); k++) {
+        const TileRec &t = F->tiles[k];
+        Emulsion &em = F->ems[k];
+        auto a0 = Clock::now();
+        TransportArgs a{};
+        a.nodes = F->d_nodes.as<Node>() + t.node_base;
+        a.recs = reinterpret_cast<const float4 *>(F->d_recs.as<Crystal>() + t.crystal_base);
+        a.n_nodes = t.n_nodes;
+        fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1);
+        a.emap = F->d_emap.as<float>() + (size_t)t.y0 * D.width + t.x0;
+        a.estride = D.width;
+        a.counts = F->d_counts.as<uint32_t>() + t.crystal_base;
+        a.stats = F->d_stats.as<unsigned long long>();
+        if (t.n) HIP_TRY(hipMemsetAsync(a.counts, 0, (size_t)t.n * 4, E->stream), FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
+        HIP_TRY(launch_transport(a, E->stream), FG_ERR_PIPELINE);
+        HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
+        c.resize((size_t)t.n);
+        if (t.n)
+            HIP_TRY(hipMemcpyAsync(c.data(), a.counts, (size_t)t.n * 4, hipMemcpyDeviceToHost, E->stream),
+                    FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
+        ms_tk += ms;
+        st->photons += a.n_photons;
+        auto a1 = Clock::now();
+        for (size_t j = 0; j < c.size(); j++) em.counts[em.bvh.idx[j]] = c[j];
+        fg_development_report rep;
+        develop_host(em, D.development, rep);
+        st->developed += rep.developed_crystals;
+        st->clumps += rep.total_clumps;
+        auto a2 = Clock::now();
+        std::vector<Node> nodes;
+        std::vector<Crystal> recs;
+        std::string msg;
+        if (!build_scan_bvh(em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "scan: " + msg);
+        HIP_TRY(F->d_scan_nodes.ensure(nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
+        HIP_TRY(F->d_scan_recs.ensure(recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
+        if (!nodes.empty())
+            HIP_TRY(hipMemcpyAsync(F->d_scan_nodes.p, nodes.data(), nodes.size() * sizeof(Node),
+                                   hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+        if (!recs.empty())
+            HIP_TRY(hipMemcpyAsync(F->d_scan_recs.p, recs.data(), recs.size() * sizeof(Crystal),
+                                   hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+        ScanArgs s{};
+        s.nodes = F->d_scan_nodes.as<Node>();
+        s.recs = F->d_scan_recs.as<float4>();
+        s.n_nodes = (uint32_t)nodes.size();
+        s.w_px = t.tw;
+        s.h_px = t.th;
+        const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
+        s.g = std::max((uint32_t)llround(sqrt((double)spp)), 1u);
+        s.ss = std::max(D.scan.supersample, 1u);
+        s.mode = D.scan.mode;
+        s.pw = (float)(em.cfg.width / (double)t.tw);
+        s.ph = (float)(em.cfg.height / (double)t.th);
+        s.z0 = (float)(em.cfg.depth + 0.1);
+        s.sigma = (float)D.scan.silver_opacity;
+        s.key0 = (uint32_t)(t.seed + 2);
+        s.key1 = (uint32_t)((t.seed + 2) >> 32);
+        s.out = F->d_dneg.as<float>() + (size_t)t.y0 * D.width + t.x0;
+        s.ostride = D.width;
+        HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
+        HIP_TRY(launch_scan(s, E->stream), FG_ERR_PIPELINE);
+        HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
+        HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+        (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
+        ms_sk += ms;
+        st->rays += (uint64_t)t.tw * t.th * s.g * s.g * s.ss * s.ss;
+        auto a3 = Clock::now();
+        t_tr += std::chrono::duration<double, std::milli>(a1 - a0).count();
+        t_dev += std::chrono::duration<double, std::milli>(a2 - a1).count();
+        t_sc += std::chrono::duration<double, std::milli>(a3 - a2).count();
+    }
+    unsigned long long cnt[8];
+    HIP_TRY(hipMemcpy(cnt, F->d_stats.p, 64, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+    st->bounces = cnt[0];
+    st->nodes_visited = cnt[1];
+    st->prims_tested = cnt[2];
+    st->abs_events = cnt[3];
+    st->ms_kernel_transport = ms_tk;
+    st->ms_kernel_scan = ms_sk;
+    st->ms_transport = (float)t_tr;
+    st->ms_develop = (float)t_dev;
+    st->ms_scan = (float)t_sc;
+    return FG_OK;
+}
+
+extern "C" fg_status fg_frame_render(fg_engine *E, fg_frame *F, uint32_t mode, fg_frame_stats *stats)
+{
+    if (!E || !F) return fail(FG_ERR_INVALID_ARG, "fg_frame_render: null argument");
+    if (mode != FG_FRAME_PLAIN && mode != FG_FRAME_BATCHED)
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_render: unknown mode");
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    fg_frame_stats st{};
+    st.tiles = F->tiles.size();
+    st.crystals = (uint64_t)F->n_crystals;
+    st.pixels = (uint64_t)F->desc.width * F->desc.height;
+    const auto t0 = Clock::now();
+    fg_status s;
+    if (mode == FG_FRAME_PLAIN) s = frame_render_plain(E, F, &st);
+    else return fail(FG_ERR_PIPELINE, "fg_frame_render: batched mode not available in this build");
+    if (s != FG_OK) return s;
+    F->otf_applied = false;
+    if (F->desc.scan.otf) {
+        HIP_TRY(launch_otf3(F->d_dneg.as<float>(), F->d_dneg2.as<float>(), F->desc.width, F->desc.height,
+                            E->stream), FG_ERR_PIPELINE);
+        F->otf_applied = true;
+    }
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    st.ms_total = ms_since(t0);
+    F->rendered = true;
+    if (stats) *stats = st;
+    return FG_OK;
+}
+
+extern "C" fg_status fg_frame_read(fg_engine *E, const fg_frame *F, float *out)
+{
+    if (!E || !F || !out) return fail(FG_ERR_INVALID_ARG, "fg_frame_read: null argument");
+    if (!F->rendered) return fail(FG_ERR_INVALID_ARG, "fg_frame_read: frame not rendered");
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    const void *src = F->otf_applied ? F->d_dneg2.p : F->d_dneg.p;
+    HIP_TRY(hipMemcpy(out, src, F->emap.size() * 4, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+    return FG_OK;
+}
+
+extern "C" fg_status fg_frame_read_counts(fg_engine *E, const fg_frame *F, int64_t n, uint32_t *counts)
+{
+    if (!E || !F || !counts || n != F->n_crystals) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_counts: bad argument");
+    if (!F->rendered) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_counts: frame not rendered");
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    std::vector<uint32_t> c((size_t)n);
+    if (n) HIP_TRY(hipMemcpy(c.data(), F->d_counts.p, (size_t)n * 4, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+    for (size_t k = 0; k < F->tiles.size(); k++) {
+        const TileRec &t = F->tiles[k];
+        const Emulsion &em = F->ems[k];
+        for (int64_t j = 0; j < t.n; j++) counts[t.crystal_base + em.bvh.idx[(size_t)j]] = c[(size_t)(t.crystal_base + j)];
+    }
+    return FG_OK;
+}

@@ -1,0 +1,160 @@
+// Device layouts and shared inline math of the MI355X photon-transport + scan path.
+//
+// Everything here is compiled both for the host (C++) and for gfx950 (HIP).  The
+// floating-point operation order of the geometry functions is fixed (the library is
+// built with -ffp-contract=off), so a host evaluation of the same function returns
+// the same bits as the kernel.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define FG_HD __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#define FG_HD inline
+#endif
+
+namespace fg {
+
+// Philox counter layouts (key = (lo32(seed), hi32(seed))):
+//   photon stream: ctr = (lo32(photon), hi32(photon), draw >> 2, kTagPhoton), word draw & 3
+//   scan jitter:   ctr = (pixel, ray >> 1, kTagScan, 0), words 2*(ray & 1) (x) and +1 (y)
+//   sequential:    ctr = (lo32(block), hi32(block), 0, kTagSeq); u64 = w0 | w1 << 32
+constexpr uint32_t kTagPhoton = 0x50484f54u;
+constexpr uint32_t kTagScan = 0x5343414eu;
+constexpr uint32_t kTagSeq = 0x53455120u;
+
+// 32-byte BVH node in threaded ("skip-link") pre-order: a node's right subtree follows
+// it directly and its left subtree follows the right one, so walking "hit -> i + 1,
+// miss or leaf done -> skip" visits nodes in exactly the order of a depth-first
+// traversal that pops the right child first, with no stack.
+struct alignas(32) Node {
+    float lo[3];
+    float hi[3];
+    uint32_t skip;   // next node when this subtree is finished or missed (n_nodes = end)
+    uint32_t prim;   // kInterior, or (first_prim << 4) | count for a leaf
+};
+constexpr uint32_t kInterior = 0xFFFFFFFFu;
+constexpr uint32_t kMaxLeafCount = 15u;
+
+// 48-byte crystal record in BVH leaf order: three 16-byte vectors.
+struct alignas(16) Crystal {
+    float c[3];
+    float sens;
+    float q[4];      // unit quaternion (w, i, j, k)
+    float h[3];      // semi-axes times filament expansion
+    uint32_t morph;  // 0 cubic, 1 tabular, 2 octahedral
+};
+
+static_assert(sizeof(Node) == 32, "node must be 32 bytes");
+static_assert(sizeof(Crystal) == 48, "crystal record must be 48 bytes");
+
+// ------------------------------------------------------------------ Philox4x32-10
+FG_HD uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+// Philox4x32 with 10 rounds (Salmon, Moraes, Dror, Shaw, SC'11).
+FG_HD void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int r = 0; r < 10; r++) {
+        const uint32_t h0 = mulhi32(0xD2511F53u, c[0]), l0 = 0xD2511F53u * c[0];
+        const uint32_t h1 = mulhi32(0xCD9E8D57u, c[2]), l1 = 0xCD9E8D57u * c[2];
+        const uint32_t n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = l1;
+        c[2] = n2;
+        c[3] = l0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// 24-bit uniform in [0, 1), exactly representable in float
+FG_HD float u24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
+
+// ------------------------------------------------------------------ geometry (f32)
+// UnitQuaternion::inverse_transform_vector (crystal.rs:151-154): rotation by the
+// conjugate, t = 2 (u' x v), out = (t w + u' x t) + v with u' = -u.
+FG_HD void qinv_rotate(const float q[4], float vx, float vy, float vz, float &ox, float &oy,
+                       float &oz)
+{
+    const float w = q[0], ux = -q[1], uy = -q[2], uz = -q[3];
+    const float tx = (uy * vz - uz * vy) * 2.0f;
+    const float ty = (uz * vx - ux * vz) * 2.0f;
+    const float tz = (ux * vy - uy * vx) * 2.0f;
+    const float cx = uy * tz - uz * ty;
+    const float cy = uz * tx - ux * tz;
+    const float cz = ux * ty - uy * tx;
+    ox = (tx * w + cx) + vx;
+    oy = (ty * w + cy) + vy;
+    oz = (tz * w + cz) + vz;
+}
+
+// HalideCrystal::intersect_ray (crystal.rs:140-217) on a crystal record; returns true
+// and (t0, t1) = (max(t_enter, 0), t_exit) on a hit.
+FG_HD bool crystal_hit(const float c[3], const float q[4], const float h[3], uint32_t morph,
+                       float ox, float oy, float oz, float dx, float dy, float dz, float &t0,
+                       float &t1)
+{
+    float lx, ly, lz, ex, ey, ez;
+    qinv_rotate(q, ox - c[0], oy - c[1], oz - c[2], lx, ly, lz);
+    qinv_rotate(q, dx, dy, dz, ex, ey, ez);
+    if (morph == 0u) {
+        const float ix = 1.0f / ex, iy = 1.0f / ey, iz = 1.0f / ez;
+        const float a1 = (-h[0] - lx) * ix, a2 = (h[0] - lx) * ix;
+        const float a3 = (-h[1] - ly) * iy, a4 = (h[1] - ly) * iy;
+        const float a5 = (-h[2] - lz) * iz, a6 = (h[2] - lz) * iz;
+        const float tmin = fmaxf(fmaxf(fminf(a1, a2), fminf(a3, a4)), fminf(a5, a6));
+        const float tmax = fminf(fminf(fmaxf(a1, a2), fmaxf(a3, a4)), fmaxf(a5, a6));
+        if (tmax >= tmin && tmax > 0.0f) {
+            t0 = fmaxf(tmin, 0.0f);
+            t1 = tmax;
+            return true;
+        }
+        return false;
+    }
+    const float px = lx / h[0], py = ly / h[1], pz = lz / h[2];
+    const float qx = ex / h[0], qy = ey / h[1], qz = ez / h[2];
+    const float a = qx * qx + qy * qy + qz * qz;
+    const float b = 2.0f * (px * qx + py * qy + pz * qz);
+    const float cc = (px * px + py * py + pz * pz) - 1.0f;
+    const float disc = b * b - 4.0f * a * cc;
+    if (disc < 0.0f) return false;
+    const float s = sqrtf(disc);
+    const float r0 = (-b - s) / (2.0f * a);
+    const float r1 = (-b + s) / (2.0f * a);
+    const float te = fminf(r0, r1), tx = fmaxf(r0, r1);
+    if (tx > 0.0f) {
+        t0 = fmaxf(te, 0.0f);
+        t1 = tx;
+        return true;
+    }
+    return false;
+}
+
+// AABB::intersect_ray (bvh.rs:96-124)
+FG_HD bool box_hit(const float lo[3], const float hi[3], float ox, float oy, float oz,
+                   float ix, float iy, float iz)
+{
+    const float t1x = (lo[0] - ox) * ix, t2x = (hi[0] - ox) * ix;
+    const float tminx = fminf(t1x, t2x), tmaxx = fmaxf(t1x, t2x);
+    const float t1y = (lo[1] - oy) * iy, t2y = (hi[1] - oy) * iy;
+    const float tminy = fminf(t1y, t2y), tmaxy = fmaxf(t1y, t2y);
+    const float t1z = (lo[2] - oz) * iz, t2z = (hi[2] - oz) * iz;
+    const float tminz = fminf(t1z, t2z), tmaxz = fmaxf(t1z, t2z);
+    const float te = fmaxf(fmaxf(tminx, tminy), tminz);
+    const float tx = fminf(fminf(tmaxx, tmaxy), tmaxz);
+    return tx >= te && tx > 0.0f;
+}
+
+}  // namespace fg

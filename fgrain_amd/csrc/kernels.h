@@ -1,0 +1,54 @@
+// Launch interfaces of the gfx950 kernels (kernels.hip).  Host code passes device
+// pointers and plain parameter blocks; every launcher returns the hipError_t of its
+// launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fg_layout.h"
+
+namespace fg {
+
+struct TransportArgs {
+    const Node *nodes;
+    const float4 *recs;      // 3 float4 per crystal record, leaf order
+    uint32_t n_nodes;
+    uint32_t pad0;
+    const float *emap;       // exposure map (may be null: weight 1)
+    uint32_t tw, th, estride;
+    uint32_t ppp;            // > 0: stratified, photon p in pixel p / ppp
+    uint32_t *counts;        // latent counts, leaf order
+    unsigned long long *stats;  // bounces, nodes, prims, abs_events, abs_total
+    float W, H, D, pw, ph;
+    float mu_t, albedo, r0, hal_eff, mu_abs, theta_max, g, step_cap;
+    uint32_t max_bounces;
+    uint32_t key0, key1;
+    uint64_t n_photons;
+};
+
+struct ScanArgs {
+    const Node *nodes;
+    const float4 *recs;
+    uint32_t n_nodes;
+    uint32_t w_px, h_px;     // output grid
+    uint32_t g, ss;          // strata per side, apertures per side
+    uint32_t mode;
+    float pw, ph, z0;        // pixel size, ray origin height (D + 0.1)
+    float sigma;
+    uint32_t key0, key1;
+    float *out;              // w_px * h_px
+    uint32_t ostride;        // row stride of out
+};
+
+hipError_t launch_transport(const TransportArgs &a, hipStream_t s);
+hipError_t launch_scan(const ScanArgs &a, hipStream_t s);
+hipError_t launch_all_hits(const Node *nodes, uint32_t n_nodes, const float4 *recs,
+                           const float *o, const float *d, int64_t n_rays, int32_t max_hits,
+                           int32_t *n_hits, int32_t *slot, float *t0, float *t1,
+                           hipStream_t s);
+hipError_t launch_first_hit(const Node *nodes, uint32_t n_nodes, const float4 *recs,
+                            const float *o, const float *d, int64_t n_rays, int32_t *slot,
+                            float *t0, float *t1, hipStream_t s);
+hipError_t launch_otf3(const float *in, float *out, uint32_t w, uint32_t h, hipStream_t s);
+
+}  // namespace fg

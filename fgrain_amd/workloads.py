@@ -1,0 +1,151 @@
+"""Synthetic workloads (committed JSON configs) and device-resident frames.
+
+A workload names its sizes: ``tile320_trix_cubic`` (one 320x320 px tile) and
+``frame24mp_trix_cubic`` (6000x4000 px as 247 tiles).  Exposures are synthetic,
+generated from the seed in the config; no image files are read.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, ptr
+from .api import (DevelopmentConfig, EmulsionConfig, ExposureConfig, GelatinOptics, GpuEngine,
+                  Morphology, ScanConfig, ScanMode)
+
+CONFIG_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs")
+
+
+def load_config(name: str) -> dict:
+    with open(os.path.join(CONFIG_DIR, name + ".json")) as f:
+        return json.load(f)
+
+
+def emulsion_config(cfg: dict, width: float, height: float) -> EmulsionConfig:
+    e = cfg["emulsion"]
+    o = e["optics"]
+    return EmulsionConfig(
+        (width, height, e["depth"]), e["r_min"], e["mean_crystal_radius"], e["size_variance"],
+        Morphology(e["morphology"]), e["tabular_aspect_ratio"],
+        GelatinOptics(o["refractive_index"], o["absorption_coeff"], o["scattering_coeff"],
+                      o["anisotropy_g"], o["antihalation_reflectance"], o["antihalation_density"]),
+        e["base_sensitivity"], e["sensitivity_sigma"], e["min_radius"], e["min_half_thickness"],
+        e["k_samples"])
+
+
+def exposure_config(cfg: dict) -> ExposureConfig:
+    x = cfg["exposure"]
+    return ExposureConfig(0.0, x["illumination_angle_rad"], x["crystal_absorption_coeff"],
+                          x["max_bounces"], x["photons_per_pixel"])
+
+
+def development_config(cfg: dict) -> DevelopmentConfig:
+    d = cfg["development"]
+    return DevelopmentConfig(d["threshold_photons"], d["filament_expansion"],
+                             d["clumping_proximity_factor"], 0.85)
+
+
+def scan_config(cfg: dict, w: int, h: int, mode=ScanMode.NegativeDensity) -> ScanConfig:
+    s = cfg["scan"]
+    return ScanConfig(w, h, s["samples_per_pixel"], mode, s["silver_opacity"], s["supersample"],
+                      bool(s.get("otf", 0)))
+
+
+def ramp_exposure(w: int, h: int, lo: float, hi: float) -> np.ndarray:
+    """Linear horizontal ramp, first column lo, last column hi."""
+    x = np.arange(w, dtype=np.float64)
+    row = lo + (hi - lo) * x / max(w - 1, 1)
+    return np.ascontiguousarray(np.broadcast_to(row.astype(np.float32), (h, w)))
+
+
+def blob_exposure(w: int, h: int, seed: int, blobs=8, sigma=(200.0, 800.0), amp=(0.2, 0.8),
+                  lo=0.01, hi=4.0) -> np.ndarray:
+    """Seeded 8-bit luminance: vertical 0..1 ramp plus Gaussian blobs, clipped, quantised
+    to 8 bits and mapped linearly to [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(amp[0], amp[1], blobs)
+    s = rng.uniform(sigma[0], sigma[1], blobs)
+    cx = rng.uniform(0, w, blobs)
+    cy = rng.uniform(0, h, blobs)
+    y = np.arange(h, dtype=np.float64)[:, None]
+    x = np.arange(w, dtype=np.float64)[None, :]
+    lum = np.repeat(y / max(h - 1, 1), w, axis=1)
+    for k in range(blobs):
+        gx = np.exp(-((x - cx[k]) ** 2) / (2 * s[k] ** 2))
+        gy = np.exp(-((y - cy[k]) ** 2) / (2 * s[k] ** 2))
+        lum += a[k] * (gy * gx)
+    lum = np.clip(lum, 0.0, 1.0)
+    l8 = np.round(lum * 255.0)
+    return (lo + (hi - lo) * l8 / 255.0).astype(np.float32)
+
+
+def workload_exposure(cfg: dict) -> np.ndarray:
+    fr, x = cfg["frame"], cfg["exposure"]
+    if x["kind"] == "ramp_x":
+        return ramp_exposure(fr["width"], fr["height"], x["lo"], x["hi"])
+    if x["kind"] == "blobs":
+        return blob_exposure(fr["width"], fr["height"], cfg["seed"], x["blobs"], x["sigma"],
+                             x["amplitude"], x["lo"], x["hi"])
+    raise ValueError(x["kind"])
+
+
+class Frame:
+    """A device-resident frame of tiles (fg_frame_*): per-tile emulsions synthesised on
+    the host once, then rendered any number of times on the GPU."""
+
+    PLAIN = 0
+    BATCHED = 1
+
+    def __init__(self, engine: GpuEngine, cfg: dict, exposure: np.ndarray | None = None,
+                 density_mult: np.ndarray | None = None, threads: int = 0,
+                 frame: tuple | None = None):
+        fr = cfg["frame"]
+        w, h = (frame or (fr["width"], fr["height"]))
+        self.width, self.height, self.tile = w, h, fr["tile"]
+        self.exposure = np.ascontiguousarray(
+            exposure if exposure is not None else workload_exposure(cfg), np.float32)
+        if self.exposure.shape != (h, w):
+            raise ValueError("exposure map shape must be (height, width)")
+        self.engine = engine
+        d = L.FrameDescC()
+        d.width, d.height, d.tile, d.threads = w, h, fr["tile"], threads
+        d.pixel_size = 1.0
+        d.emulsion = emulsion_config(cfg, 0.0, 0.0).c()
+        d.exposure = exposure_config(cfg).c()
+        d.development = development_config(cfg).c()
+        d.scan = scan_config(cfg, 0, 0).c()
+        d.seed = cfg["seed"]
+        d.exposure_map = ptr(self.exposure, L._f32p)
+        self._mult = None if density_mult is None else np.ascontiguousarray(density_mult, np.float32)
+        d.density_mult = ptr(self._mult, L._f32p)
+        self._h = C.c_void_p()
+        check(L.lib().fg_frame_create(engine._h, C.byref(d), C.byref(self._h)))
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and L._lib is not None:
+            L._lib.fg_frame_destroy(h)
+
+    @property
+    def crystals(self) -> int:
+        return int(L.lib().fg_frame_crystal_count(self._h))
+
+    def render(self, mode: int = PLAIN) -> dict:
+        st = L.FrameStatsC()
+        check(L.lib().fg_frame_render(self.engine._h, self._h, mode, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in L.FrameStatsC._fields_}
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width), np.float32)
+        check(L.lib().fg_frame_read(self.engine._h, self._h, ptr(out, L._f32p)))
+        return out
+
+    def counts(self) -> np.ndarray:
+        n = self.crystals
+        out = np.zeros(n, np.uint32)
+        check(L.lib().fg_frame_read_counts(self.engine._h, self._h, n, ptr(out, L._u32p)))
+        return out

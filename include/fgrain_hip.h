@@ -1,0 +1,277 @@
+/*
+ * fgrain_hip.h -- C ABI of the MI355X (gfx950, HIP) photon-transport + scan engine.
+ *
+ * Drop-in for fgrain's GPU boundary (GuyNguyen/fgrain src/gpu.rs) and for the library
+ * stages the `--gpu` path runs (emulsion synthesis -> transport -> development ->
+ * transmittance scan, src/lib.rs:89-111).  Plain C types only: status codes are
+ * returned, outputs are caller-allocated, nothing throws across the ABI.  All lengths
+ * are in one caller-chosen unit (micrometres in the reference's image pipeline,
+ * pixels in the synthetic workloads); coefficients are per that unit.
+ *
+ * Threading: every entry point may be called from any thread; calls on one engine are
+ * serialised by an internal mutex (the reference's dispatch_lock, gpu.rs:169, 503).
+ */
+#ifndef FGRAIN_HIP_H
+#define FGRAIN_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FG_ABI_VERSION 1u
+
+/* GpuError variants (src/error.rs:9-33) plus argument validation. */
+typedef enum fg_status {
+    FG_OK = 0,
+    FG_ERR_ADAPTER_NOT_FOUND = 1, /* GpuError::AdapterNotFound */
+    FG_ERR_DEVICE_REQUEST = 2,    /* GpuError::DeviceRequestFailed */
+    FG_ERR_CHANNEL = 3,           /* GpuError::ChannelError */
+    FG_ERR_BUFFER_MAP = 4,        /* GpuError::BufferMapError */
+    FG_ERR_BUFFER_RANGE = 5,      /* GpuError::BufferRangeError */
+    FG_ERR_PIPELINE = 6,          /* GpuError::PipelineError */
+    FG_ERR_INVALID_ARG = 7
+} fg_status;
+
+typedef struct fg_engine fg_engine;     /* opaque, thread-safe */
+typedef struct fg_emulsion fg_emulsion; /* VirtualEmulsionLayer (emulsion.rs:141-150) */
+typedef struct fg_frame fg_frame;       /* device-resident batch of tiles */
+
+/* GelatinOptics (src/optics.rs:41-60) */
+typedef struct fg_optics {
+    double refractive_index;
+    double absorption_coeff;
+    double scattering_coeff;
+    double anisotropy_g;
+    double antihalation_reflectance;
+    double antihalation_density;
+} fg_optics;
+
+/* EmulsionConfig (src/emulsion.rs:14-38).  min_radius and min_half_thickness are the
+ * reference's fixed clamps 0.15 and 0.05 (emulsion.rs:169, 187), in the length unit. */
+typedef struct fg_emulsion_config {
+    double width, height, depth;
+    double r_min;
+    double mean_crystal_radius;
+    double size_variance;
+    double tabular_aspect_ratio;
+    double base_sensitivity;
+    double sensitivity_sigma;   /* 0.15 in the reference (emulsion.rs:164) */
+    double min_radius;
+    double min_half_thickness;
+    uint32_t morphology;        /* 0 cubic, 1 tabular, 2 octahedral */
+    uint32_t k_samples;         /* Bridson candidates, 30 (emulsion.rs:159) */
+    fg_optics optics;
+} fg_emulsion_config;
+
+/* ExposureConfig (src/transport.rs:15-39).  photons_per_pixel > 0 selects the
+ * stratified sampler (that many packets per exposure-map pixel, positions jittered in
+ * the pixel); 0 selects uniform positions over the patch with
+ * round(width*height*photons_per_sq_unit) packets. */
+typedef struct fg_exposure_config {
+    double photons_per_sq_unit;
+    double illumination_angle_rad;
+    double crystal_absorption_coeff;
+    uint32_t max_bounces;
+    uint32_t photons_per_pixel;
+} fg_exposure_config;
+
+typedef struct fg_expose_stats {
+    uint64_t launched;      /* packets launched (Fresnel-rejected included) */
+    uint64_t bounces;
+    uint64_t nodes_visited;
+    uint64_t prims_tested;
+    uint64_t abs_events;    /* hits that deposited k > 0 latent photons */
+    uint64_t abs_total;     /* sum of deposited latent photons */
+    float kernel_ms;        /* transport kernel time (HIP events) */
+    float pad_;
+} fg_expose_stats;
+
+/* DevelopmentConfig (src/development.rs:12-38) */
+typedef struct fg_development_config {
+    uint32_t threshold_photons;
+    uint32_t pad_;
+    double filament_expansion;
+    double clumping_proximity_factor;
+    double developer_contrast;
+} fg_development_config;
+
+/* DevelopmentReport (src/development.rs:42-55) */
+typedef struct fg_development_report {
+    uint64_t total_crystals;
+    uint64_t developed_crystals;
+    uint64_t fixed_crystals;
+    uint64_t total_clumps;
+    uint64_t max_clump_size;
+    double avg_clump_size;
+} fg_development_report;
+
+#define FG_SCAN_NEGATIVE 0u  /* ScanMode::NegativeTransmittance */
+#define FG_SCAN_POSITIVE 1u  /* ScanMode::PositivePrint */
+#define FG_SCAN_DENSITY 2u   /* ScanMode::OpticalDensity */
+#define FG_SCAN_DNEG 3u      /* D_neg = -log10(clamp(T,1e-4,1)) + 0.05 (image_pipeline.rs:1006-1011) */
+
+/* ScanConfig (src/transmittance.rs:32-60) + aperture supersampling and scanner OTF.
+ * supersample = s splits each output pixel into s x s apertures with
+ * round(sqrt(spp))^2 stratified rays each; otf = 1 applies the separable
+ * [0.035, 0.93, 0.035] filter with clamp-to-edge (image_pipeline.rs:822-855). */
+typedef struct fg_scan_config {
+    uint32_t width_px, height_px;
+    uint32_t samples_per_pixel;
+    uint32_t mode;
+    double silver_opacity;
+    uint32_t supersample;
+    uint32_t otf;
+} fg_scan_config;
+
+/* GpuTileParams (src/gpu.rs:18-37), plus the transport and optics values the
+ * reference's CPU tile path takes from its preset tables (image_pipeline.rs:924-976). */
+typedef struct fg_tile_params {
+    double film_w, film_h, film_d;
+    uint32_t width_px, height_px;
+    uint32_t samples_per_pixel;
+    uint32_t morphology;
+    double silver_opacity;
+    uint64_t seed;
+    double r_min, mean_crystal_radius, size_variance, tabular_aspect_ratio, base_sensitivity;
+    double exposure;
+    uint32_t threshold_photons;
+    uint32_t max_bounces;
+    double filament_expansion, clumping_proximity_factor;
+    fg_optics optics;
+    double photons_per_sq_unit;     /* 6 * max(exposure, 0.05) in the reference */
+    double illumination_angle_rad;  /* 0.12 */
+    double crystal_absorption_coeff;/* 1.3 */
+    double depth;                   /* emulsion depth used for synthesis (= film_d) */
+} fg_tile_params;
+
+typedef struct fg_tile_stats {
+    uint64_t crystals, developed, clumps;
+    uint64_t photons;
+    float ms_transport, ms_develop, ms_scan, ms_total;
+} fg_tile_stats;
+
+/* ---------------------------------------------------------------- engine ------- */
+/* GpuEngine::new (gpu.rs:174).  No HIP device -> FG_ERR_ADAPTER_NOT_FOUND. */
+fg_status fg_engine_create(fg_engine **out);
+void fg_engine_destroy(fg_engine *engine);
+/* GpuEngine::adapter_name (gpu.rs:484-486) */
+const char *fg_engine_adapter_name(const fg_engine *engine);
+/* get_gpu_engine (gpu.rs:9-14): process-wide engine created on first use; a failed
+ * creation is remembered and reported on every later call. */
+fg_status fg_get_engine(fg_engine **out);
+/* Message of the last failed call on this thread (GpuError's payload string). */
+const char *fg_last_error(void);
+uint32_t fg_abi_version(void);
+
+/* ---------------------------------------------------------------- emulsion ----- */
+/* VirtualEmulsionLayer::synthesize (emulsion.rs:155-220): Bridson 3D + per-crystal
+ * draws + BVH::build.  Host work; deterministic in seed. */
+fg_status fg_emulsion_synthesize(const fg_emulsion_config *cfg, uint64_t seed,
+                                 fg_emulsion **out);
+/* An emulsion from given crystals (dims = semi-axes, quat = (w,i,j,k)). */
+fg_status fg_emulsion_from_crystals(const fg_emulsion_config *cfg, int64_t n,
+                                    const double *pos, const double *dims,
+                                    const double *quat, const int32_t *morph,
+                                    const double *sens, fg_emulsion **out);
+void fg_emulsion_destroy(fg_emulsion *em);
+int64_t fg_emulsion_count(const fg_emulsion *em);
+fg_status fg_emulsion_get_crystals(const fg_emulsion *em, double *pos, double *dims,
+                                   double *quat, int32_t *morph, double *sens);
+/* BVH::build layout (bvh.rs:129-151): node boxes (f64), children, leaf ranges,
+ * indices.  Pass NULL pointers to query n_nodes only. */
+fg_status fg_emulsion_get_bvh(const fg_emulsion *em, int64_t *n_nodes, double *bmin,
+                              double *bmax, uint32_t *left, uint32_t *right,
+                              uint32_t *offset, uint32_t *count, uint32_t *indices);
+/* Threaded 32-byte device layout (n_nodes * 32 bytes) and 48-byte leaf-order records. */
+fg_status fg_emulsion_get_device_layout(const fg_emulsion *em, void *nodes, void *records);
+/* Latent counts (HalideCrystal::absorbed_photons), developed flags, filament expansion
+ * and clump ids (-1 = none), per crystal in the caller's crystal order. */
+fg_status fg_emulsion_get_state(const fg_emulsion *em, uint32_t *counts, uint8_t *developed,
+                                double *fe, int64_t *clump_id);
+fg_status fg_emulsion_set_state(fg_emulsion *em, const uint32_t *counts,
+                                const uint8_t *developed, const double *fe);
+
+/* ---------------------------------------------------------------- stages ------- */
+/* expose_emulsion (transport.rs:48-188) on the GPU: adds latent photons to the
+ * emulsion's counts.  exposure_map is a tw x th row-major f32 map looked up by the
+ * tile path's nearest-pixel rule (image_pipeline.rs:949-961); NULL = uniform 1. */
+fg_status fg_expose(fg_engine *engine, fg_emulsion *em, const float *exposure_map,
+                    uint32_t tw, uint32_t th, const fg_exposure_config *cfg, uint64_t seed,
+                    fg_expose_stats *stats);
+/* develop_emulsion (development.rs:59-188): integer threshold (exact), clumping as
+ * connected components, clump boost. */
+fg_status fg_develop(fg_engine *engine, fg_emulsion *em, const fg_development_config *cfg,
+                     fg_development_report *report);
+/* scan_emulsion (transmittance.rs:65-171) over the developed crystals; out has
+ * width_px * height_px values in the configured mode. */
+fg_status fg_scan(fg_engine *engine, const fg_emulsion *em, const fg_scan_config *cfg,
+                  uint64_t seed, float *out);
+/* BVH::intersect_ray_all (bvh.rs:300-347) on the GPU for n_rays float rays: hits in
+ * the reference's DFS callback order (crystal index, t_enter, t_exit). */
+fg_status fg_traverse_all_hits(fg_engine *engine, const fg_emulsion *em, int64_t n_rays,
+                               const float *origins, const float *dirs, int32_t max_hits,
+                               int32_t *n_hits, int32_t *idx, float *t0, float *t1);
+/* First hit = smallest t_enter, ties to the earlier hit in DFS order; idx -1 = none. */
+fg_status fg_traverse_first_hit(fg_engine *engine, const fg_emulsion *em, int64_t n_rays,
+                                const float *origins, const float *dirs, int32_t *idx,
+                                float *t0, float *t1);
+
+/* ------------------------------------------------------ drop-in tile render ----- */
+/* Fill params the way image_pipeline.rs:455-515 and 924-976 do for a preset
+ * ("tri-x", "tri-x-1600", "t-max", "hp5"), a tile of tw x th pixels at grain size
+ * grain_size_px (pixel_size_um = 1 / grain_size_px), exposure and push. */
+fg_status fg_tile_params_init(fg_tile_params *p, const char *preset, double grain_size_px,
+                              uint32_t tw, uint32_t th, double exposure, double push,
+                              uint64_t tile_seed);
+/* GpuEngine::render_tile_all_gpu (gpu.rs:491-938): synthesis (seed), transport
+ * (seed + 1), development, scan (seed + 2) of one tile; out_dneg receives
+ * width_px * height_px D_neg values.  Zero pixels -> FG_OK with nothing written. */
+fg_status fg_render_tile_all_gpu(fg_engine *engine, const float *tile_img,
+                                 const fg_tile_params *params, float *out_dneg,
+                                 fg_tile_stats *stats);
+
+/* ------------------------------------------------------------- frames ---------- */
+#define FG_FRAME_PLAIN 0u   /* one launch set + synchronisation per tile */
+#define FG_FRAME_BATCHED 1u /* persistent work-queue kernels over all tiles, one graph */
+
+/* A frame of width x height pixels cut into tile x tile tiles (edge tiles partial),
+ * one emulsion per tile (seed per image_pipeline.rs:430-434), pixel units scaled by
+ * pixel_size.  density_mult (per tile, NULL = 1) scales crystal density. */
+typedef struct fg_frame_desc {
+    uint32_t width, height, tile;
+    uint32_t threads;             /* host threads for synthesis; 0 = all */
+    double pixel_size;
+    fg_emulsion_config emulsion;  /* width/height are set per tile */
+    fg_exposure_config exposure;
+    fg_development_config development;
+    fg_scan_config scan;          /* width_px/height_px are set per tile */
+    uint64_t seed;
+    const float *exposure_map;    /* width * height, row-major, host */
+    const float *density_mult;    /* tiles, or NULL */
+} fg_frame_desc;
+
+typedef struct fg_frame_stats {
+    uint64_t tiles, crystals, developed, clumps, photons, rays, pixels;
+    uint64_t bounces, nodes_visited, prims_tested, abs_events; /* transport counters */
+    float ms_total, ms_transport, ms_develop, ms_scan;
+    float ms_kernel_transport;    /* summed transport kernel time (HIP events) */
+    float ms_kernel_scan;
+} fg_frame_stats;
+
+fg_status fg_frame_create(fg_engine *engine, const fg_frame_desc *desc, fg_frame **out);
+void fg_frame_destroy(fg_frame *frame);
+fg_status fg_frame_render(fg_engine *engine, fg_frame *frame, uint32_t mode,
+                          fg_frame_stats *stats);
+/* Frame D_neg (width * height, row-major) of the last render. */
+fg_status fg_frame_read(fg_engine *engine, const fg_frame *frame, float *out);
+/* Per-tile latent counts of the last render, crystals in tile order then the tile's
+ * crystal order; n must equal the frame's crystal count. */
+fg_status fg_frame_read_counts(fg_engine *engine, const fg_frame *frame, int64_t n,
+                               uint32_t *counts);
+int64_t fg_frame_crystal_count(const fg_frame *frame);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FGRAIN_HIP_H */

@@ -1,0 +1,119 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * A CPU restatement, in plain C, of the reference's photon-transport + development +
+ * transmittance-scan path (GuyNguyen/fgrain, Rust).  Only tests/, the smoke check and
+ * the CPU-baseline leg of bench.py may load it, and only as the checker or the timed
+ * CPU baseline; the product library never links or calls it.
+ *
+ * Every function names the reference file:line it restates.  FP64 functions follow the
+ * Rust code (f64); the *_f32 twins evaluate the same formulas in float with the
+ * operation order the GPU kernels use, for bit-exact comparisons on identical rays.
+ *
+ * Random streams: the reference draws from rand 0.8 StdRng (ChaCha12), which is not
+ * vendored and cannot be built here, so stream-level parity with the reference is
+ * unpinned.  The oracle instead uses Philox4x32-10 (Salmon et al., SC'11) with the
+ * stream layouts below, which the GPU kernels share; distributions are pinned by
+ * statistical and closed-form tests.
+ */
+#ifndef FGRAIN_ORACLE_H
+#define FGRAIN_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Crystal set, FP64 structure-of-arrays (HalideCrystal, crystal.rs:21-51). */
+typedef struct {
+    int64_t n;
+    const double *pos;        /* n*3 centroid */
+    const double *dims;       /* n*3 semi-axes */
+    const double *quat;       /* n*4 unit quaternion (w, i, j, k) */
+    const int32_t *morph;     /* n: 0 cubic, 1 tabular, 2 octahedral */
+    const double *sens;       /* n sensitivity */
+    const uint8_t *developed; /* n or NULL (= none developed) */
+    const double *fe;         /* n filament expansion or NULL (= 1.0) */
+} orc_crystals;
+
+/* Flattened BVH exactly as bvh.rs:129-295 builds it (caller allocates 2n nodes). */
+typedef struct {
+    int64_t n_nodes;
+    double *bmin, *bmax;      /* n_nodes*3 */
+    uint32_t *left, *right, *offset, *count;
+    uint32_t *indices;        /* n */
+} orc_bvh_out;
+
+typedef struct {
+    double W, H, D;                 /* slab extents */
+    double mu_a, mu_s, g, n_refr;   /* GelatinOptics (optics.rs:41-60) */
+    double r_ahu, od_ahu;
+    double mu_abs;                  /* ExposureConfig::crystal_absorption_coeff */
+    double theta_max;               /* ExposureConfig::illumination_angle_rad */
+    double photons_per_area;        /* sampler 0: photons per unit area at E=1 */
+    int32_t max_bounces;
+    int32_t sampler;                /* 0: reference chunked uniform; 1: photon-keyed (GPU) */
+    int64_t n_photons;              /* sampler 1 with ppp == 0 */
+    int32_t ppp;                    /* sampler 1: photons per pixel, stratified */
+    int32_t threads;                /* worker threads; 0 = all online cpus */
+    uint64_t seed;
+    int32_t tw, th;                 /* exposure map size */
+    int32_t pad_;
+} orc_expose_cfg;
+
+typedef struct {
+    int64_t launched, fresnel_rejected, zero_weight, bounces;
+    int64_t nodes_tested, prims_tested, hits, abs_events;
+    int64_t abs_total, chunks;
+} orc_expose_stats;
+
+typedef struct {
+    int64_t total, developed, fixed, clumps, max_clump;
+    double avg_clump;
+} orc_dev_report;
+
+typedef struct {
+    double W, H, D, r_min, mean_radius, size_sigma, aspect, base_sens, sens_sigma;
+    double min_radius, min_half_thickness;
+    int32_t morph, k_samples;
+} orc_synth_cfg;
+
+/* --- random streams ------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* --- geometry (crystal.rs, bvh.rs) ---------------------------------------------- */
+int orc_crystal_intersect_f64(const orc_crystals *cs, int64_t i, const double o[3],
+                              const double d[3], double t[2]);
+int orc_crystal_intersect_f32(const orc_crystals *cs, int64_t i, const float o[3],
+                              const float d[3], float t[2]);
+int64_t orc_bvh_build(const orc_crystals *cs, orc_bvh_out *out);
+/* All hits in the reference's DFS callback order for n_rays rays; hits of ray r are
+ * written at [r*max_hits, ...) and their number to nhits[r].  f32 twin uses
+ * outward-rounded float boxes and float crystal records. */
+void orc_all_hits_f64(const orc_crystals *cs, int64_t n_rays, const double *o,
+                      const double *d, int32_t max_hits, int32_t *nhits, int32_t *idx,
+                      double *t0, double *t1);
+void orc_all_hits_f32(const orc_crystals *cs, int64_t n_rays, const float *o,
+                      const float *d, int32_t max_hits, int32_t *nhits, int32_t *idx,
+                      float *t0, float *t1);
+int64_t orc_query_radius(const orc_crystals *cs, const double c[3], double r,
+                         int32_t *out, int64_t cap);
+
+/* --- stages ------------------------------------------------------------------------ */
+int orc_expose(const orc_crystals *cs, const float *exposure_map, const orc_expose_cfg *cfg,
+               uint32_t *counts, orc_expose_stats *st);
+int orc_develop(const orc_crystals *cs, const uint32_t *counts, uint32_t threshold,
+                double fe, double pf, uint8_t *developed, double *fe_out, int64_t *clump_id,
+                orc_dev_report *rep);
+int orc_scan(const orc_crystals *cs, double W, double H, double D, int32_t w_px,
+             int32_t h_px, int32_t spp, int32_t ss, double sigma, uint64_t seed,
+             int32_t threads, float *T, int64_t *rays_nodes);
+void orc_dneg(const float *T, int64_t n, float *D);
+void orc_otf3(const float *in, int32_t w, int32_t h, float *out);
+int64_t orc_synthesize(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, double *pos,
+                       double *dims, double *quat, int32_t *morph, double *sens);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,288 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes front-end of ``liboracle.so``, the plain-C CPU restatement of the reference's
+photon-transport + development + scan path (GuyNguyen/fgrain).  Only ``tests/``,
+``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` import this module,
+and only to check or to time the CPU baseline; the product never does.
+
+Each wrapper names the reference function it restates; the C source carries the
+file:line citations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u32p = C.POINTER(C.c_uint32)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+
+
+class OrcCrystals(C.Structure):
+    _fields_ = [("n", C.c_int64), ("pos", _f64p), ("dims", _f64p), ("quat", _f64p),
+                ("morph", _i32p), ("sens", _f64p), ("developed", _u8p), ("fe", _f64p)]
+
+
+class OrcBvhOut(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("bmin", _f64p), ("bmax", _f64p), ("left", _u32p),
+                ("right", _u32p), ("offset", _u32p), ("count", _u32p), ("indices", _u32p)]
+
+
+class OrcExposeCfg(C.Structure):
+    _fields_ = [("W", C.c_double), ("H", C.c_double), ("D", C.c_double),
+                ("mu_a", C.c_double), ("mu_s", C.c_double), ("g", C.c_double),
+                ("n_refr", C.c_double), ("r_ahu", C.c_double), ("od_ahu", C.c_double),
+                ("mu_abs", C.c_double), ("theta_max", C.c_double),
+                ("photons_per_area", C.c_double), ("max_bounces", C.c_int32),
+                ("sampler", C.c_int32), ("n_photons", C.c_int64), ("ppp", C.c_int32),
+                ("threads", C.c_int32), ("seed", C.c_uint64), ("tw", C.c_int32),
+                ("th", C.c_int32), ("pad_", C.c_int32)]
+
+
+class OrcExposeStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "launched", "fresnel_rejected", "zero_weight", "bounces", "nodes_tested",
+        "prims_tested", "hits", "abs_events", "abs_total", "chunks")]
+
+
+class OrcDevReport(C.Structure):
+    _fields_ = [("total", C.c_int64), ("developed", C.c_int64), ("fixed", C.c_int64),
+                ("clumps", C.c_int64), ("max_clump", C.c_int64), ("avg_clump", C.c_double)]
+
+
+class OrcSynthCfg(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "W", "H", "D", "r_min", "mean_radius", "size_sigma", "aspect", "base_sens",
+        "sens_sigma", "min_radius", "min_half_thickness")] + [
+        ("morph", C.c_int32), ("k_samples", C.c_int32)]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_crystal_intersect_f64.argtypes = [C.POINTER(OrcCrystals), C.c_int64, _f64p, _f64p, _f64p]
+        L.orc_crystal_intersect_f32.argtypes = [C.POINTER(OrcCrystals), C.c_int64, _f32p, _f32p, _f32p]
+        L.orc_bvh_build.argtypes = [C.POINTER(OrcCrystals), C.POINTER(OrcBvhOut)]
+        L.orc_bvh_build.restype = C.c_int64
+        L.orc_all_hits_f64.argtypes = [C.POINTER(OrcCrystals), C.c_int64, _f64p, _f64p, C.c_int32,
+                                       _i32p, _i32p, _f64p, _f64p]
+        L.orc_all_hits_f32.argtypes = [C.POINTER(OrcCrystals), C.c_int64, _f32p, _f32p, C.c_int32,
+                                       _i32p, _i32p, _f32p, _f32p]
+        L.orc_query_radius.argtypes = [C.POINTER(OrcCrystals), _f64p, C.c_double, _i32p, C.c_int64]
+        L.orc_query_radius.restype = C.c_int64
+        L.orc_expose.argtypes = [C.POINTER(OrcCrystals), _f32p, C.POINTER(OrcExposeCfg), _u32p,
+                                 C.POINTER(OrcExposeStats)]
+        L.orc_develop.argtypes = [C.POINTER(OrcCrystals), _u32p, C.c_uint32, C.c_double, C.c_double,
+                                  _u8p, _f64p, _i64p, C.POINTER(OrcDevReport)]
+        L.orc_scan.argtypes = [C.POINTER(OrcCrystals), C.c_double, C.c_double, C.c_double,
+                               C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                               C.c_uint64, C.c_int32, _f32p, _i64p]
+        L.orc_dneg.argtypes = [_f32p, C.c_int64, _f32p]
+        L.orc_otf3.argtypes = [_f32p, C.c_int32, C.c_int32, _f32p]
+        L.orc_synthesize.argtypes = [C.POINTER(OrcSynthCfg), C.c_uint64, C.c_int64, _f64p, _f64p,
+                                     _f64p, _i32p, _f64p]
+        L.orc_synthesize.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+@dataclass
+class Crystals:
+    """FP64 crystal set (HalideCrystal fields that the path reads)."""
+    pos: np.ndarray          # (n,3) f64
+    dims: np.ndarray         # (n,3) f64 semi-axes
+    quat: np.ndarray         # (n,4) f64 (w,i,j,k)
+    morph: np.ndarray        # (n,) i32
+    sens: np.ndarray         # (n,) f64
+    developed: np.ndarray | None = None   # (n,) u8
+    fe: np.ndarray | None = None          # (n,) f64
+
+    def __post_init__(self):
+        self.pos = np.ascontiguousarray(self.pos, np.float64).reshape(-1, 3)
+        self.dims = np.ascontiguousarray(self.dims, np.float64).reshape(-1, 3)
+        self.quat = np.ascontiguousarray(self.quat, np.float64).reshape(-1, 4)
+        self.morph = np.ascontiguousarray(self.morph, np.int32).reshape(-1)
+        self.sens = np.ascontiguousarray(self.sens, np.float64).reshape(-1)
+        if self.developed is not None:
+            self.developed = np.ascontiguousarray(self.developed, np.uint8).reshape(-1)
+        if self.fe is not None:
+            self.fe = np.ascontiguousarray(self.fe, np.float64).reshape(-1)
+
+    @property
+    def n(self):
+        return self.pos.shape[0]
+
+    def c(self) -> OrcCrystals:
+        return OrcCrystals(self.n, _p(self.pos, _f64p), _p(self.dims, _f64p), _p(self.quat, _f64p),
+                           _p(self.morph, _i32p), _p(self.sens, _f64p),
+                           _p(self.developed, _u8p), _p(self.fe, _f64p))
+
+
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c, _u32p), _p(k, _u32p), _p(out, _u32p))
+    return out
+
+
+def crystal_intersect(cs: Crystals, i: int, o, d, fp32=False):
+    """HalideCrystal::intersect_ray; returns (t0, t1) or None."""
+    cc = cs.c()
+    if fp32:
+        o_, d_ = np.asarray(o, np.float32), np.asarray(d, np.float32)
+        t = np.zeros(2, np.float32)
+        hit = lib().orc_crystal_intersect_f32(C.byref(cc), i, _p(o_, _f32p), _p(d_, _f32p), _p(t, _f32p))
+    else:
+        o_, d_ = np.asarray(o, np.float64), np.asarray(d, np.float64)
+        t = np.zeros(2, np.float64)
+        hit = lib().orc_crystal_intersect_f64(C.byref(cc), i, _p(o_, _f64p), _p(d_, _f64p), _p(t, _f64p))
+    return (t[0], t[1]) if hit else None
+
+
+def bvh_build(cs: Crystals):
+    """BVH::build -> dict of node arrays in the reference's pre-order layout."""
+    n = cs.n
+    cap = max(2 * n, 1)
+    bmin = np.zeros((cap, 3)); bmax = np.zeros((cap, 3))
+    left = np.zeros(cap, np.uint32); right = np.zeros(cap, np.uint32)
+    off = np.zeros(cap, np.uint32); cnt = np.zeros(cap, np.uint32)
+    idx = np.zeros(max(n, 1), np.uint32)
+    out = OrcBvhOut(0, _p(bmin, _f64p), _p(bmax, _f64p), _p(left, _u32p), _p(right, _u32p),
+                    _p(off, _u32p), _p(cnt, _u32p), _p(idx, _u32p))
+    cc = cs.c()
+    nn = lib().orc_bvh_build(C.byref(cc), C.byref(out))
+    return dict(bmin=bmin[:nn], bmax=bmax[:nn], left=left[:nn], right=right[:nn],
+                offset=off[:nn], count=cnt[:nn], indices=idx[:n])
+
+
+def all_hits(cs: Crystals, o, d, max_hits=256, fp32=False):
+    """BVH::intersect_ray_all over rays (n,3); returns list of (idx[], t0[], t1[])."""
+    ft = np.float32 if fp32 else np.float64
+    pt = _f32p if fp32 else _f64p
+    o = np.ascontiguousarray(o, ft).reshape(-1, 3)
+    d = np.ascontiguousarray(d, ft).reshape(-1, 3)
+    nr = o.shape[0]
+    nh = np.zeros(nr, np.int32)
+    idx = np.zeros((nr, max_hits), np.int32)
+    t0 = np.zeros((nr, max_hits), ft)
+    t1 = np.zeros((nr, max_hits), ft)
+    cc = cs.c()
+    fn = lib().orc_all_hits_f32 if fp32 else lib().orc_all_hits_f64
+    fn(C.byref(cc), nr, _p(o, pt), _p(d, pt), max_hits, _p(nh, _i32p), _p(idx, _i32p),
+       _p(t0, pt), _p(t1, pt))
+    if (nh > max_hits).any():
+        raise ValueError("max_hits too small")
+    return nh, idx, t0, t1
+
+
+def query_radius(cs: Crystals, center, r, cap=4096):
+    c = np.asarray(center, np.float64)
+    out = np.zeros(cap, np.int32)
+    cc = cs.c()
+    n = lib().orc_query_radius(C.byref(cc), _p(c, _f64p), r, _p(out, _i32p), cap)
+    if n > cap:
+        return query_radius(cs, center, r, cap=int(n))
+    return out[:n].copy()
+
+
+def expose(cs: Crystals, exposure_map: np.ndarray | None, *, W, H, D, mu_a, mu_s, g, n_refr,
+           r_ahu, od_ahu, mu_abs, theta_max, max_bounces, seed, sampler=1, ppp=0,
+           n_photons=0, photons_per_area=0.0, threads=0):
+    """expose_emulsion -> (per-crystal counts u32, stats dict)."""
+    if exposure_map is not None:
+        em = np.ascontiguousarray(exposure_map, np.float32)
+        th, tw = em.shape
+    else:
+        em, th, tw = None, 1, 1
+    cfg = OrcExposeCfg(W, H, D, mu_a, mu_s, g, n_refr, r_ahu, od_ahu, mu_abs, theta_max,
+                       photons_per_area, max_bounces, sampler, n_photons, ppp, threads, seed,
+                       tw, th, 0)
+    counts = np.zeros(max(cs.n, 1), np.uint32)
+    st = OrcExposeStats()
+    cc = cs.c()
+    lib().orc_expose(C.byref(cc), _p(em, _f32p), C.byref(cfg), _p(counts, _u32p), C.byref(st))
+    stats = {k: getattr(st, k) for k, _ in OrcExposeStats._fields_}
+    return counts[:cs.n], stats
+
+
+def develop(cs: Crystals, counts, threshold, fe, pf):
+    """develop_emulsion -> (developed u8, fe f64, clump_id i64, report dict)."""
+    n = cs.n
+    counts = np.ascontiguousarray(counts, np.uint32)
+    dev = np.zeros(max(n, 1), np.uint8)
+    feo = np.zeros(max(n, 1), np.float64)
+    cid = np.zeros(max(n, 1), np.int64)
+    rep = OrcDevReport()
+    cc = cs.c()
+    lib().orc_develop(C.byref(cc), _p(counts, _u32p), threshold, fe, pf, _p(dev, _u8p),
+                      _p(feo, _f64p), _p(cid, _i64p), C.byref(rep))
+    return dev[:n], feo[:n], cid[:n], {k: getattr(rep, k) for k, _ in OrcDevReport._fields_}
+
+
+def scan(cs: Crystals, W, H, D, w_px, h_px, spp=4, ss=1, sigma=0.25, seed=0, threads=0):
+    """scan_emulsion (NegativeTransmittance) -> T (h_px, w_px) f32."""
+    T = np.zeros((h_px, w_px), np.float32)
+    nodes = C.c_int64(0)
+    cc = cs.c()
+    lib().orc_scan(C.byref(cc), W, H, D, w_px, h_px, spp, ss, sigma, seed, threads,
+                   _p(T, _f32p), C.byref(nodes))
+    return T
+
+
+def dneg(T):
+    T = np.ascontiguousarray(T, np.float32)
+    out = np.zeros_like(T)
+    lib().orc_dneg(_p(T, _f32p), T.size, _p(out, _f32p))
+    return out
+
+
+def otf3(img):
+    img = np.ascontiguousarray(img, np.float32)
+    h, w = img.shape
+    out = np.zeros_like(img)
+    lib().orc_otf3(_p(img, _f32p), w, h, _p(out, _f32p))
+    return out
+
+
+def synthesize(*, W, H, D, r_min, mean_radius, size_sigma, morph, aspect=1.0, base_sens=1.2,
+               sens_sigma=0.15, min_radius=0.15, min_half_thickness=0.05, k_samples=30, seed=0):
+    """VirtualEmulsionLayer::synthesize -> Crystals."""
+    cfg = OrcSynthCfg(W, H, D, r_min, mean_radius, size_sigma, aspect, base_sens, sens_sigma,
+                      min_radius, min_half_thickness, morph, k_samples)
+    cap = int(W * H * D / (r_min ** 3) * 1.2) + 64
+    while True:
+        pos = np.zeros((cap, 3)); dims = np.zeros((cap, 3)); quat = np.zeros((cap, 4))
+        mo = np.zeros(cap, np.int32); sens = np.zeros(cap)
+        n = lib().orc_synthesize(C.byref(cfg), seed, cap, _p(pos, _f64p), _p(dims, _f64p),
+                                 _p(quat, _f64p), _p(mo, _i32p), _p(sens, _f64p))
+        if n >= 0:
+            return Crystals(pos[:n], dims[:n], quat[:n], mo[:n], sens[:n])
+        cap = -n

@@ -1,0 +1,241 @@
+"""GPU parity of the hot path against the oracle (CPU restatement of the reference).
+
+Every test calls through the C ABI (libfgrain_hip.so) and compares with the oracle on
+the same seeded inputs:
+* traversal: all-hit lists and first hits bit-exact against the FP32 oracle;
+* transport: same Philox streams replayed by the FP64 oracle (near-identical counts),
+  and an independent-stream statistical comparison with the reference's chunked
+  uniform sampler;
+* development: exact (integer threshold, clump partition, expansion);
+* scan: |T_gpu - T_oracle| <= 1e-5 on identical rays and developed sets.
+"""
+import numpy as np
+import pytest
+from scipy import stats
+
+import fgrain_amd as F
+from helpers import expose_kwargs, oracle_crystals, random_rays, trix_config
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_em():
+    cfg = trix_config(96, 96)
+    return cfg, F.VirtualEmulsionLayer.synthesize(cfg, 7)
+
+
+@pytest.mark.parametrize("morph", [0, 1, 2])
+def test_all_hits_bit_exact_vs_fp32_oracle(engine, morph):
+    cfg = trix_config(64, 64, morph=morph, aspect=8.0 if morph == 1 else 1.0)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 11 + morph)
+    o, d = random_rays(4096, 64, 64, 24, seed=morph)
+    nh, idx, t0, t1 = engine.traverse_all_hits(em, o, d, max_hits=128)
+    cs = oracle_crystals(em)
+    onh, oidx, ot0, ot1 = O.all_hits(cs, o, d, max_hits=128, fp32=True)
+    assert (nh == onh).all()
+    assert nh.sum() > 1000
+    for r in range(len(nh)):
+        k = nh[r]
+        assert (idx[r, :k] == oidx[r, :k]).all()
+        assert (t0[r, :k].view(np.uint32) == ot0[r, :k].view(np.uint32)).all()
+        assert (t1[r, :k].view(np.uint32) == ot1[r, :k].view(np.uint32)).all()
+
+
+def test_first_hit_bit_exact_and_fp64_mismatch_rate(engine, small_em):
+    cfg, em = small_em
+    o, d = random_rays(1 << 16, 96, 96, 24, seed=3)
+    idx, t0, t1 = engine.first_hit(em, o, d)
+    cs = oracle_crystals(em)
+    onh, oidx, ot0, ot1 = O.all_hits(cs, o, d, max_hits=256, fp32=True)
+    exp = np.full(len(o), -1, np.int32)
+    e0 = np.zeros(len(o), np.float32)
+    for r in np.nonzero(onh)[0]:
+        k = onh[r]
+        j = int(np.argmin(ot0[r, :k]))  # argmin returns the first minimum: DFS tie order
+        exp[r], e0[r] = oidx[r, j], ot0[r, j]
+    assert (idx == exp).all()
+    hit = idx >= 0
+    assert (t0[hit].view(np.uint32) == e0[hit].view(np.uint32)).all()
+    # informational bound: the float path agrees with the double reference on
+    # nearly every ray
+    dnh, didx, dt0, _ = O.all_hits(cs, o.astype(np.float64), d.astype(np.float64), max_hits=256)
+    d_exp = np.full(len(o), -1, np.int32)
+    for r in np.nonzero(dnh)[0]:
+        d_exp[r] = didx[r, int(np.argmin(dt0[r, :dnh[r]]))]
+    assert np.mean(d_exp != idx) < 1e-3
+
+
+def _expose_gpu(engine, em, cfg, emap, ppp, seed, **kw):
+    x = F.ExposureConfig(0.0, kw.get("theta", 0.12), kw.get("mu_abs", 0.9),
+                         kw.get("max_bounces", 48), ppp)
+    st = F.expose_emulsion(em, emap, x, seed, engine)
+    return em.absorbed_photons().astype(np.int64), st
+
+
+def test_transport_replay_matches_fp64_oracle(engine):
+    """Same photon-keyed Philox streams, f32 GPU vs f64 oracle: paths only diverge
+    where rounding flips a comparison, so counts agree almost everywhere."""
+    cfg = trix_config(64, 64)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 5)
+    emap = np.linspace(0.01, 4.0, 64, dtype=np.float32)[None, :].repeat(64, 0)
+    g, st = _expose_gpu(engine, em, cfg, emap, ppp=16, seed=77)
+    cs = oracle_crystals(em)
+    oc, ost = O.expose(cs, emap, **expose_kwargs(cfg), seed=77, sampler=1, ppp=16)
+    oc = oc.astype(np.int64)
+    assert st["launched"] == ost["launched"] == 16 * 64 * 64
+    assert ost["abs_total"] > 0
+    assert abs(g.sum() - oc.sum()) <= 1e-3 * oc.sum()
+    assert np.mean(g == oc) >= 0.98
+    assert st["bounces"] == pytest.approx(ost["bounces"], rel=1e-3)
+    assert st["nodes_visited"] == pytest.approx(ost["nodes_tested"], rel=1e-3)
+
+
+def test_transport_replay_with_halation(engine):
+    cfg = trix_config(48, 48, depth=16.0, r_ahu=0.9)
+    cfg.optics.antihalation_density = 0.05
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 9)
+    emap = np.full((48, 48), 1.5, np.float32)
+    g, st = _expose_gpu(engine, em, cfg, emap, ppp=32, seed=5)
+    cs = oracle_crystals(em)
+    oc, ost = O.expose(cs, emap, **expose_kwargs(cfg), seed=5, sampler=1, ppp=32)
+    assert abs(g.sum() - oc.astype(np.int64).sum()) <= 1e-3 * oc.sum()
+    assert np.mean(g == oc) >= 0.97
+
+
+def test_transport_statistics_vs_reference_sampler(engine):
+    """Independent streams: GPU (stratified photon-keyed Philox) against the reference
+    algorithm's chunked uniform sampler (transport.rs:65-74).  Per-crystal means agree
+    in aggregate and in bins, and the per-crystal distributions pass a two-sample KS."""
+    cfg = trix_config(96, 96)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 21)
+    emap = np.linspace(0.01, 4.0, 96, dtype=np.float32)[None, :].repeat(96, 0)
+    ppp = 64
+    cum = []
+    for s in range(4):
+        c, _ = _expose_gpu(engine, em, cfg, emap, ppp=ppp, seed=1000 + s)
+        cum.append(c)
+    # absorbed_photons accumulates over calls (record_absorption adds): difference them
+    cum = np.stack(cum)
+    per = np.diff(np.concatenate([np.zeros((1, cum.shape[1]), np.int64), cum]), axis=0)
+    gmean = per.mean(0)
+    cs = oracle_crystals(em)
+    n = ppp * 96 * 96
+    oc, ost = O.expose(cs, emap, **expose_kwargs(cfg), seed=3, sampler=0,
+                       photons_per_area=ppp, threads=8)
+    # the reference launches chunks * ceil(base / chunks) >= base packets
+    oc = oc.astype(np.float64) * (n / ost["launched"])
+    tot_g, tot_o = gmean.sum(), oc.sum()
+    sd = per.sum(1).std(ddof=1)
+    assert abs(tot_g - tot_o) <= 4 * sd * np.sqrt(1 + 1 / 4) + 1e-3 * tot_o
+    assert abs(tot_g - tot_o) / tot_o < 5e-3
+    pos = cs.pos
+    bins = np.minimum((pos[:, 0] / 96 * 8).astype(int), 7)
+    for b in range(8):
+        m = bins == b
+        assert abs(gmean[m].sum() - oc[m].sum()) / oc[m].sum() < 0.02
+    assert stats.ks_2samp(per[0], oc).pvalue > 0.01
+
+
+def test_development_exact(engine):
+    cfg = trix_config(96, 96)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 4)
+    rng = np.random.default_rng(0)
+    counts = rng.poisson(3.0, len(em)).astype(np.uint32)
+    em.set_state(counts=counts)
+    rep = F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
+    st = em.state()
+    cs = oracle_crystals(em)
+    dev, fe, cid, orep = O.develop(cs, counts, 4, 1.2, 1.25)
+    assert (st["developed"] == dev).all()
+    assert (st["clump_id"] == cid).all()
+    assert (st["fe"] == fe).all()
+    assert rep.developed_crystals == orep["developed"]
+    assert rep.total_clumps == orep["clumps"]
+    assert rep.max_clump_size == orep["max_clump"]
+    assert rep.avg_clump_size == pytest.approx(orep["avg_clump"], rel=1e-12)
+
+
+@pytest.mark.parametrize("ss", [1, 2])
+def test_scan_matches_fp64_oracle(engine, ss):
+    cfg = trix_config(64, 48)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 8)
+    rng = np.random.default_rng(1)
+    counts = rng.poisson(3.2, len(em)).astype(np.uint32)
+    em.set_state(counts=counts)
+    F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
+    st = em.state()
+    sc = F.ScanConfig(64, 48, 4, F.ScanMode.NegativeTransmittance, 0.104166667, ss)
+    T, _ = F.scan_emulsion(em, sc, 2, engine)
+    cs = oracle_crystals(em, st["developed"], st["fe"])
+    To = O.scan(cs, 64, 48, 24, 64, 48, spp=4, ss=ss, sigma=0.104166667, seed=2)
+    assert T.min() < 0.9
+    assert np.abs(T - To).max() <= 1e-5
+    sc.mode = F.ScanMode.NegativeDensity
+    D, _ = F.scan_emulsion(em, sc, 2, engine)
+    assert np.abs(D - O.dneg(To)).max() <= 1e-4
+
+
+def test_scan_otf_matches_oracle(engine):
+    cfg = trix_config(40, 40)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 3)
+    em.set_state(counts=np.full(len(em), 9, np.uint32))
+    F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
+    sc = F.ScanConfig(40, 40, 4, F.ScanMode.NegativeTransmittance, 0.104166667, 1, False)
+    T, _ = F.scan_emulsion(em, sc, 2, engine)
+    sc.otf = True
+    To, _ = F.scan_emulsion(em, sc, 2, engine)
+    ref = O.otf3(T)
+    assert (To.view(np.uint32) == ref.view(np.uint32)).all()
+
+
+def test_reference_exposure_latent_image_formation(engine):
+    """transport.rs:196-237: Tri-X 20x20 um, left half lit -> left absorbs > 5x right."""
+    cfg = F.EmulsionConfig.tri_x_400(20.0, 20.0)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 100)
+    emap = np.zeros((64, 64), np.float32)
+    emap[:, :32] = 1.0  # nearest-pixel map of the closure x <= 10
+    F.expose_emulsion(em, emap, F.ExposureConfig(100.0, 0.1, 1.5, 32), 42, engine)
+    c = em.absorbed_photons()
+    x = em.crystals()["pos"][:, 0]
+    left, right = c[x <= 9.0].sum(), c[x >= 11.0].sum()
+    assert left > 0 and left > 5 * right
+
+
+def test_reference_transmittance_evaluation(engine):
+    """transmittance.rs:181-220: one developed 2x2x1 um cube at (5,5,4): centre pixel
+    darker than the corner, corner above 0.95."""
+    cfg = F.EmulsionConfig.t_max_100(10.0, 10.0)
+    em = F.VirtualEmulsionLayer.from_crystals(cfg, [[5, 5, 4]], [[2, 2, 1]], [[1, 0, 0, 0]], 0, 1.0)
+    em.set_state(developed=np.ones(1, np.uint8), fe=np.ones(1))
+    T, _ = F.scan_emulsion(em, F.ScanConfig(16, 16, 4, F.ScanMode.NegativeTransmittance, 20.0), 42,
+                           engine)
+    assert T[8, 8] < T[0, 0]
+    assert T[0, 0] > 0.95
+    # closed form: inside the footprint T = exp(-sigma * 2 * 1.0)
+    assert T[8, 8] == pytest.approx(np.exp(-20.0 * 2.0), abs=1e-6)
+
+
+@pytest.mark.parametrize("exp_val", [0.0, 0.05, 0.18, 0.5, 1.0])
+def test_reference_render_tile_all_gpu(engine, exp_val):
+    """gpu.rs:1280-1317 on the drop-in: a 16x16 tile renders 256 finite D_neg values."""
+    p = F.GpuTileParams.for_preset("tri-x", 16 / 15.0, 16, 16, exposure=1.0, tile_seed=42)
+    out = engine.render_tile_all_gpu(np.full(256, exp_val, np.float32), p)
+    assert out.shape == (256,)
+    assert np.isfinite(out).all()
+    assert (out >= 0.05 - 1e-6).all()
+
+
+def test_render_tile_matches_stage_pipeline(engine):
+    """render_tile_all_gpu = synthesize(seed) -> expose(seed+1) -> develop -> scan(seed+2)
+    (image_pipeline.rs:924-1014) through the stage API, bit for bit."""
+    p = F.GpuTileParams.for_preset("tri-x", 2.4, 40, 32, exposure=1.0, tile_seed=1234)
+    img = np.linspace(0.0, 1.0, 40 * 32, dtype=np.float32)
+    D = engine.render_tile_all_gpu(img, p)
+    cfg = F.EmulsionConfig.tri_x_400(p.c.film_w, p.c.film_h)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 1234)
+    F.expose_emulsion(em, img.reshape(32, 40), F.ExposureConfig(6.0, 0.12, 1.3, 48), 1235, engine)
+    F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
+    D2, _ = F.scan_emulsion(em, F.ScanConfig(40, 32, 4, F.ScanMode.NegativeDensity, 0.25), 1236, engine)
+    assert (D.reshape(32, 40).view(np.uint32) == D2.view(np.uint32)).all()
