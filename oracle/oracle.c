@@ -761,6 +761,10 @@ static void run_photon(const texp *X, orng *rng, double x, double y, orc_expose_
         if (X->b->nn > 0) DFS_TRAVERSE(X->b, tr_box, tr_prim, &h, &cnt);
         st->nodes_tested += cnt.nodes;
         st->prims_tested += cnt.prims;
+        if (w > 0.0) {
+            st->bounces_live++;
+            st->nodes_live += cnt.nodes;
+        }
         w = h.w;
         const double q[3] = {p[0] + d[0] * step, p[1] + d[1] * step, p[2] + d[2] * step};
         if (q[0] < 0.0 || q[0] >= c->W || q[1] < 0.0 || q[1] >= c->H) return;
@@ -894,6 +898,8 @@ int orc_expose(const orc_crystals *cs, const float *exposure_map, const orc_expo
         st->hits += ts[t].st.hits;
         st->abs_events += ts[t].st.abs_events;
         st->abs_total += ts[t].st.abs_total;
+        st->bounces_live += ts[t].st.bounces_live;
+        st->nodes_live += ts[t].st.nodes_live;
     }
     free(args);
     free(ts);
@@ -1326,4 +1332,63 @@ int64_t orc_synthesize(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, dou
     }
     free(pts);
     return np;
+}
+
+/* ------------------------------------------------- optics helpers for tests -- */
+/* GelatinOptics::sample_free_path with a given uniform (optics.rs:100-107) */
+double orc_free_path(double mu_t, double xi)
+{
+    if (mu_t <= 0.0) return 1e6;
+    return -log(fmax(1.0 - xi, DBL_MIN)) / mu_t;
+}
+
+/* GelatinOptics::sample_henyey_greenstein with given uniforms (optics.rs:113-143) */
+void orc_hg(const double d[3], double g, double xi1, double xi2, double out[3])
+{
+    double ct;
+    if (fabs(g) < 1e-4) {
+        ct = 2.0 * xi1 - 1.0;
+    } else {
+        const double term = (1.0 - g * g) / (1.0 - g + 2.0 * g * xi1);
+        ct = (1.0 + g * g - term * term) / (2.0 * g);
+    }
+    ct = fmin(fmax(ct, -1.0), 1.0);
+    const double stt = sqrt(fmax(1.0 - ct * ct, 0.0));
+    const double phi = TAU * xi2;
+    double w[3] = {d[0], d[1], d[2]};
+    normalize3(w);
+    double u[3];
+    if (fabs(w[0]) > 0.1) {
+        u[0] = w[2]; u[1] = 0.0; u[2] = -w[0];
+    } else {
+        u[0] = 0.0; u[1] = -w[2]; u[2] = w[1];
+    }
+    normalize3(u);
+    const double v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2],
+                         w[0] * u[1] - w[1] * u[0]};
+    const double a = stt * cos(phi), bb = stt * sin(phi);
+    for (int k = 0; k < 3; k++) out[k] = (u[k] * a + v[k] * bb) + w[k] * ct;
+    normalize3(out);
+}
+
+/* GelatinOptics::fresnel_air_gelatin (optics.rs:147-154) */
+double orc_fresnel(double n, double cos_i)
+{
+    texp X;
+    memset(&X, 0, sizeof X);
+    const double r = (1.0 - n) / (1.0 + n);
+    X.r0 = r * r;
+    return fresnel(&X, cos_i);
+}
+
+/* handle_antihalation_reflection (optics.rs:160-196) with a sequential stream seeded
+ * by seed; returns alive and updates energy and direction. */
+int orc_antihalation(double r_ahu, double od, uint64_t seed, double *energy, double d[3])
+{
+    texp X;
+    memset(&X, 0, sizeof X);
+    X.halation_eff = r_ahu * pow(10.0, -2.0 * od);
+    orng rng;
+    rng_seq(&rng, seed);
+    return antihalation(&X, energy, d, &rng);
 }

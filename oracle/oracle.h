@@ -65,6 +65,7 @@ typedef struct {
     int64_t launched, fresnel_rejected, zero_weight, bounces;
     int64_t nodes_tested, prims_tested, hits, abs_events;
     int64_t abs_total, chunks;
+    int64_t bounces_live, nodes_live;  /* bounces / nodes of packets with non-zero weight */
 } orc_expose_stats;
 
 typedef struct {
@@ -110,6 +111,10 @@ int orc_scan(const orc_crystals *cs, double W, double H, double D, int32_t w_px,
              int32_t threads, float *T, int64_t *rays_nodes);
 void orc_dneg(const float *T, int64_t n, float *D);
 void orc_otf3(const float *in, int32_t w, int32_t h, float *out);
+double orc_free_path(double mu_t, double xi);
+void orc_hg(const double d[3], double g, double xi1, double xi2, double out[3]);
+double orc_fresnel(double n, double cos_i);
+int orc_antihalation(double r_ahu, double od, uint64_t seed, double *energy, double d[3]);
 int64_t orc_synthesize(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, double *pos,
                        double *dims, double *quat, int32_t *morph, double *sens);
 

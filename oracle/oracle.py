@@ -52,7 +52,8 @@ class OrcExposeCfg(C.Structure):
 class OrcExposeStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "launched", "fresnel_rejected", "zero_weight", "bounces", "nodes_tested",
-        "prims_tested", "hits", "abs_events", "abs_total", "chunks")]
+        "prims_tested", "hits", "abs_events", "abs_total", "chunks", "bounces_live",
+        "nodes_live")]
 
 
 class OrcDevReport(C.Structure):
@@ -105,6 +106,12 @@ def lib():
         L.orc_synthesize.argtypes = [C.POINTER(OrcSynthCfg), C.c_uint64, C.c_int64, _f64p, _f64p,
                                      _f64p, _i32p, _f64p]
         L.orc_synthesize.restype = C.c_int64
+        L.orc_free_path.argtypes = [C.c_double, C.c_double]
+        L.orc_free_path.restype = C.c_double
+        L.orc_hg.argtypes = [_f64p, C.c_double, C.c_double, C.c_double, _f64p]
+        L.orc_fresnel.argtypes = [C.c_double, C.c_double]
+        L.orc_fresnel.restype = C.c_double
+        L.orc_antihalation.argtypes = [C.c_double, C.c_double, C.c_uint64, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -286,3 +293,27 @@ def synthesize(*, W, H, D, r_min, mean_radius, size_sigma, morph, aspect=1.0, ba
         if n >= 0:
             return Crystals(pos[:n], dims[:n], quat[:n], mo[:n], sens[:n])
         cap = -n
+
+
+def free_path(mu_t, xi):
+    """GelatinOptics::sample_free_path for a given uniform."""
+    return lib().orc_free_path(mu_t, xi)
+
+
+def hg(d, g, xi1, xi2):
+    """GelatinOptics::sample_henyey_greenstein for given uniforms."""
+    d_ = np.asarray(d, np.float64)
+    out = np.zeros(3)
+    lib().orc_hg(_p(d_, _f64p), g, xi1, xi2, _p(out, _f64p))
+    return out
+
+
+def fresnel(n, cos_i):
+    return lib().orc_fresnel(n, cos_i)
+
+
+def antihalation(r_ahu, od, seed, energy, d):
+    e = C.c_double(energy)
+    d_ = np.ascontiguousarray(d, np.float64).copy()
+    alive = lib().orc_antihalation(r_ahu, od, seed, C.byref(e), _p(d_, _f64p))
+    return bool(alive), e.value, d_

@@ -88,8 +88,10 @@ def test_transport_replay_matches_fp64_oracle(engine):
     assert ost["abs_total"] > 0
     assert abs(g.sum() - oc.sum()) <= 1e-3 * oc.sum()
     assert np.mean(g == oc) >= 0.98
-    assert st["bounces"] == pytest.approx(ost["bounces"], rel=1e-3)
-    assert st["nodes_visited"] == pytest.approx(ost["nodes_tested"], rel=1e-3)
+    # the kernel stops a packet once its weight is exactly zero (halation off zeroes it;
+    # it can deposit nothing afterwards): compare the work of packets with weight
+    assert st["bounces"] == pytest.approx(ost["bounces_live"], rel=1e-3)
+    assert st["nodes_visited"] == pytest.approx(ost["nodes_live"], rel=1e-3)
 
 
 def test_transport_replay_with_halation(engine):
