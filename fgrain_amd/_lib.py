@@ -142,6 +142,7 @@ SIGNATURES = {
     "fg_frame_render": (_st, [_vp, _vp, C.c_uint32, C.POINTER(FrameStatsC)]),
     "fg_frame_read": (_st, [_vp, _vp, _f32p]),
     "fg_frame_read_counts": (_st, [_vp, _vp, C.c_int64, _u32p]),
+    "fg_frame_read_development": (_st, [_vp, _vp, C.c_int64, _u8p, _f64p, _i64p]),
     "fg_frame_crystal_count": (C.c_int64, [_vp]),
 }
 

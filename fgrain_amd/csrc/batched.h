@@ -1,0 +1,70 @@
+// Batched frame pipeline: every tile of a frame in one pass of device kernels
+// (transport -> development -> developed-crystal BVH -> scan), replayed as a hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "kernels.h"
+
+namespace fg {
+
+// Per-tile descriptor shared by host and device.
+struct TileDev {
+    uint32_t x0, y0, tw, th;        // frame placement (pixels)
+    float W, H, pw, ph;             // extents and pixel size (length units)
+    uint32_t key0, key1;            // transport seed (tile seed + 1)
+    uint32_t skey0, skey1;          // scan seed (tile seed + 2)
+    uint32_t node_base, n_nodes;    // transport BVH nodes
+    uint32_t crystal_base, n;       // crystal slots (leaf order)
+    uint64_t item_base;             // first 64-packet work item
+    uint64_t n_photons;
+    uint32_t block_base, blocks_x;  // scan blocks of 16 x 16 pixels
+    uint32_t blocks, pad0;
+    double maxb;                    // largest semi-axis among the tile's crystals
+    // results of the last render
+    uint32_t n_dev, clumps, max_clump, dev_nodes;
+};
+
+struct DevelopParams {
+    uint32_t threshold;
+    uint32_t pad;
+    double fe, pf;
+};
+
+struct ScanCommon {
+    uint32_t g, ss, mode, pad;
+    float z0, sigma;
+};
+
+struct BatchedFrame;
+
+struct BatchedInput {
+    const Node *nodes;              // transport nodes of all tiles
+    const Crystal *recs;            // transport records of all tiles
+    uint32_t *counts;               // latent counts per slot
+    const float *emap;              // frame exposure map
+    float *dneg;                    // frame output
+    unsigned long long *stats;      // transport counters (5)
+    uint32_t frame_w, frame_h;
+    int64_t n_crystals;
+    std::vector<TileDev> tiles;
+    // host arrays in slot order (per tile: slot k holds crystal bvh.idx[k])
+    std::vector<double> pos64, dims64;
+    std::vector<uint32_t> orig;
+    TransportArgs transport;        // optics and shared fields; per-tile fields overwritten
+    DevelopParams develop;
+    ScanCommon scan;
+};
+
+hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **out);
+// Render the frame; use_graph replays a captured hipGraph (captured on first use).
+hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph);
+hipError_t batched_read_tiles(BatchedFrame *b, hipStream_t s, std::vector<TileDev> &tiles,
+                              uint32_t *err_flags);
+hipError_t batched_read_develop(BatchedFrame *b, hipStream_t s, std::vector<uint8_t> &dev,
+                                std::vector<double> &fe, std::vector<int32_t> &clump);
+void batched_destroy(BatchedFrame *b);
+
+}  // namespace fg

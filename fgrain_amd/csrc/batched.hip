@@ -1,0 +1,753 @@
+// Batched frame pipeline on gfx950: all tiles of a frame go through transport,
+// development (threshold + clumping), the developed-crystal BVH build and the scan in
+// one sequence of device kernels, captured once as a hipGraph and replayed.
+//
+// * transport: a persistent kernel whose waves pull 64-packet work items from one
+//   device counter, tile after tile (work items are numbered tile-major, so the waves
+//   in flight share one tile's nodes and records in L2);
+// * development (development.rs:59-188): integer threshold; clumps as connected
+//   components of the symmetric touch relation, by lock-free union-find with the root
+//   at the smallest original crystal index, numbered in that order like the BFS;
+// * developed-crystal BVH (transmittance.rs:82-89 -> bvh.rs:157-295): level-synchronous
+//   median split.  The split sets depend only on the centroid comparison (with a
+//   centroid extent above 1e-7 both sides are non-empty), so the tree, its leaf sets and
+//   its boxes equal the reference's; only the order inside a leaf can differ;
+// * scan: the pixels of every tile in one launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "batched.h"
+#include "device_common.h"
+
+namespace fg {
+
+namespace {
+
+using namespace dev;
+
+constexpr int kMaxLevels = 64;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+struct Seg {
+    uint32_t tile, start, count, node, depth;
+};
+
+struct BNode {
+    float lo[3], hi[3];
+    uint32_t left, right, parent;
+    uint32_t start, count, depth;
+};
+
+struct Bufs {
+    // per slot
+    double *pos64, *dims64, *mdim, *fe, *cen;
+    uint32_t *orig, *tile_of, *devflag, *parent, *csize, *root_flag, *prefix, *devpos;
+    int32_t *clump;
+    uint32_t *elem[2], *final_elem;
+    float *elo, *ehi;
+    // per node capacity (2 per slot)
+    BNode *bn;
+    uint32_t *leafcnt;
+    Node *dnodes;
+    Crystal *drecs;
+    Seg *seg[2];
+    uint32_t *seg_count;   // 3 rotating counters
+    uint32_t *node_count;  // per tile
+    uint32_t *work;        // transport work counter
+    uint32_t *err;         // [0] depth, [1] leaf size, [2] unfinished build
+    TileDev *tiles;
+};
+
+// ---------------------------------------------------------------- helpers -------
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// find with path halving; parents only move towards roots, so the plain halving
+// stores cannot detach a node from its tree
+__device__ uint32_t uf_find(uint32_t *parent, uint32_t x)
+{
+    for (;;) {
+        const uint32_t p = ld_relaxed(&parent[x]);
+        if (p == x) return x;
+        const uint32_t gp = ld_relaxed(&parent[p]);
+        if (gp == p) return p;
+        st_relaxed(&parent[x], gp);
+        x = gp;
+    }
+}
+
+// link the root with the larger original index under the smaller one
+__device__ void uf_unite(uint32_t *parent, const uint32_t *orig, uint32_t a, uint32_t b)
+{
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (orig[a] < orig[b]) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&parent[a], a, b) == a) return;
+    }
+}
+
+__device__ __forceinline__ uint32_t tile_of_item(const TileDev *tiles, uint32_t n_tiles, uint64_t item)
+{
+    uint32_t lo = 0, hi = n_tiles;  // last tile with item_base <= item
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tiles[mid].item_base <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t tile_of_block(const TileDev *tiles, uint32_t n_tiles, uint32_t b)
+{
+    uint32_t lo = 0, hi = n_tiles;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tiles[mid].block_base <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------ kernels -----
+__global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigned long long *stats,
+                                               uint64_t n, uint32_t n_tiles)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += stride) {
+        B.leafcnt[i] = 0u;
+        if (i < n) {
+            counts[i] = 0u;
+            B.csize[i] = 0u;
+            B.root_flag[i] = 0u;
+        }
+        if (i < n_tiles) {
+            B.node_count[i] = 0u;
+            B.tiles[i].n_dev = B.tiles[i].clumps = B.tiles[i].max_clump = B.tiles[i].dev_nodes = 0u;
+        }
+        if (i < 8) stats[i] = 0ull;
+        if (i < 4) {
+            B.err[i] = 0u;
+            if (i < 3) B.seg_count[i] = 0u;
+            if (i == 3) *B.work = 0u;
+        }
+    }
+}
+
+// persistent transport over every tile's packets
+__global__ void __launch_bounds__(256) k_transport(TransportArgs C, Bufs B, const Node *nodes,
+                                                   const float4 *recs, uint32_t *counts,
+                                                   const float *emap, uint32_t n_tiles,
+                                                   uint64_t total_items)
+{
+    PhotonCounters cnt;
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(B.work, 1u);
+        item = __shfl(item, 0, 64);
+        if (item >= total_items) break;
+        const uint32_t t = tile_of_item(B.tiles, n_tiles, item);
+        const TileDev T = B.tiles[t];
+        const uint64_t p = (item - T.item_base) * 64 + lane;
+        if (p < T.n_photons) {
+            TransportArgs A = C;
+            A.nodes = nodes + T.node_base;
+            A.recs = recs + 3 * (size_t)T.crystal_base;
+            A.n_nodes = T.n_nodes;
+            A.emap = emap + (size_t)T.y0 * C.estride + T.x0;
+            A.tw = T.tw;
+            A.th = T.th;
+            A.counts = counts + T.crystal_base;
+            A.W = T.W;
+            A.H = T.H;
+            A.pw = T.pw;
+            A.ph = T.ph;
+            A.key0 = T.key0;
+            A.key1 = T.key1;
+            A.n_photons = T.n_photons;
+            run_photon(A, p, cnt);
+        }
+    }
+    flush_counters(cnt, C.stats);
+}
+
+__global__ void __launch_bounds__(256) k_threshold(Bufs B, const uint32_t *counts, uint64_t n,
+                                                   DevelopParams P)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t d = counts[s] >= P.threshold ? 1u : 0u;
+    B.devflag[s] = d;
+    B.fe[s] = d ? P.fe : 0.0;
+    B.parent[s] = (uint32_t)s;
+    B.clump[s] = -1;
+}
+
+// touching developed pairs within a tile, found by a box query on the tile's
+// transport BVH (inclusive overlap as bvh.rs:373-379), decided in double precision
+__global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, uint64_t n, DevelopParams P)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || !B.devflag[s]) return;
+    const TileDev T = B.tiles[B.tile_of[s]];
+    const double px = B.pos64[3 * s], py = B.pos64[3 * s + 1], pz = B.pos64[3 * s + 2];
+    const double ri = B.mdim[s] * P.fe;
+    const double R = (ri + T.maxb * P.fe) * P.pf;
+    const float qlo[3] = {__double2float_rd(px - R), __double2float_rd(py - R), __double2float_rd(pz - R)};
+    const float qhi[3] = {__double2float_ru(px + R), __double2float_ru(py + R), __double2float_ru(pz + R)};
+    const Node *nd0 = nodes + T.node_base;
+    uint32_t i = 0;
+    while (i < T.n_nodes) {
+        const NodeView nd = load_node(nd0, i);
+        const bool ov = nd.lo[0] <= qhi[0] && nd.hi[0] >= qlo[0] && nd.lo[1] <= qhi[1] &&
+                        nd.hi[1] >= qlo[1] && nd.lo[2] <= qhi[2] && nd.hi[2] >= qlo[2];
+        if (ov && nd.prim != kInterior) {
+            const uint32_t first = nd.prim >> 4, cnt = nd.prim & 15u;
+            for (uint32_t k = first; k < first + cnt; k++) {
+                const uint64_t j = (uint64_t)T.crystal_base + k;
+                if (j <= s || !B.devflag[j]) continue;
+                const double rj = B.mdim[j] * P.fe;
+                const double touch = (ri + rj) * P.pf;
+                const double dx = B.pos64[3 * j] - px, dy = B.pos64[3 * j + 1] - py,
+                             dz = B.pos64[3 * j + 2] - pz;
+                if (dx * dx + dy * dy + dz * dz <= touch * touch)
+                    uf_unite(B.parent, B.orig, (uint32_t)s, (uint32_t)j);
+            }
+        }
+        i = (ov && nd.prim == kInterior) ? i + 1 : nd.skip;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_roots(Bufs B, uint64_t n)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || !B.devflag[s]) return;
+    const uint32_t r = uf_find(B.parent, (uint32_t)s);
+    B.parent[s] = r;
+    atomicAdd(&B.csize[r], 1u);
+    if (r == s) B.root_flag[B.tiles[B.tile_of[s]].crystal_base + B.orig[s]] = 1u;
+}
+
+// exclusive prefix of 0/1 flags over [base, base + n) by one block per tile
+__device__ uint32_t block_scan_flags(const uint32_t *flags, uint32_t *out, uint32_t base, uint32_t n)
+{
+    __shared__ uint32_t wcnt[4];
+    __shared__ uint32_t total;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t running = 0;
+    for (uint32_t c = 0; c < n; c += 256) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t f = (i < n) ? flags[base + i] : 0u;
+        const unsigned long long m = __ballot(f != 0u);
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t wb = 0, tot = 0;
+        for (uint32_t w = 0; w < 4; w++) {
+            if (w < wave) wb += wcnt[w];
+            tot += wcnt[w];
+        }
+        if (i < n) out[base + i] = running + wb + rank;
+        running += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) total = running;
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(256) k_tile_scans(Bufs B)
+{
+    TileDev &T = B.tiles[blockIdx.x];
+    const uint32_t clumps = block_scan_flags(B.root_flag, B.prefix, T.crystal_base, T.n);
+    const uint32_t ndev = block_scan_flags(B.devflag, B.devpos, T.crystal_base, T.n);
+    if (threadIdx.x == 0) {
+        T.clumps = clumps;
+        T.n_dev = ndev;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_boost(Bufs B, uint64_t n, DevelopParams P)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || !B.devflag[s]) return;
+    const uint32_t t = B.tile_of[s];
+    TileDev &T = B.tiles[t];
+    const uint32_t r = B.parent[s];
+    const uint32_t size = B.csize[r];
+    double fe = P.fe;
+    if (size > 1u) fe *= 1.15 + 0.18 * (double)(min(size, 6u) - 1u);
+    B.fe[s] = fe;
+    B.clump[s] = (int32_t)B.prefix[T.crystal_base + B.orig[r]];
+    if (r == s) atomicMax(&T.max_clump, size);
+    // developed-crystal box (crystal.rs:106-133 with the boosted expansion) and centroid
+    const uint32_t e = T.crystal_base + B.devpos[s];
+    B.elem[0][e] = (uint32_t)s;
+    const double m = B.mdim[s] * fe;
+    for (int a = 0; a < 3; a++) {
+        const double lo = B.pos64[3 * s + a] - m, hi = B.pos64[3 * s + a] + m;
+        B.elo[3 * s + a] = __double2float_rd(lo);
+        B.ehi[3 * s + a] = __double2float_ru(hi);
+        B.cen[3 * s + a] = 0.5 * (lo + hi);
+    }
+}
+
+__global__ void k_root_segments(Bufs B, uint32_t n_tiles)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    const TileDev T = B.tiles[t];
+    if (T.n_dev == 0) return;
+    const uint32_t k = atomicAdd(&B.seg_count[0], 1u);
+    B.seg[0][k] = Seg{t, T.crystal_base, T.n_dev, 2 * T.crystal_base, 0u};
+    B.node_count[t] = 1u;
+    B.bn[2 * T.crystal_base].parent = kNone;
+}
+
+template <class V, class Op>
+__device__ __forceinline__ V block_reduce(V v, V *lds, Op op)
+{
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    V r = lds[0];
+    for (int w = 1; w < 4; w++) r = op(r, lds[w]);
+    return r;
+}
+
+struct MinF { __device__ float operator()(float a, float b) const { return fminf(a, b); } };
+struct MaxF { __device__ float operator()(float a, float b) const { return fmaxf(a, b); } };
+struct MinD { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
+struct MaxD { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
+struct SumU { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; } };
+
+// One level of the median-split build (bvh.rs:166-295), one block per segment.
+__global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
+{
+    __shared__ float lf[4];
+    __shared__ double ld[4];
+    __shared__ uint32_t lu[4];
+    __shared__ uint32_t wl[4], wr[4];
+    const int in = level % 3, outc = (level + 1) % 3, zc = (level + 2) % 3;
+    const Seg *segs = B.seg[level & 1];
+    Seg *out = B.seg[(level + 1) & 1];
+    const uint32_t *src = B.elem[level & 1];
+    uint32_t *dst = B.elem[(level + 1) & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.seg_count[zc] = 0u;
+    const uint32_t nseg = B.seg_count[in];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t si = blockIdx.x; si < nseg; si += gridDim.x) {
+        const Seg sg = segs[si];
+        float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256) {
+            const uint32_t s = src[e];
+            for (int a = 0; a < 3; a++) {
+                blo[a] = fminf(blo[a], B.elo[3 * s + a]);
+                bhi[a] = fmaxf(bhi[a], B.ehi[3 * s + a]);
+                cmn[a] = fmin(cmn[a], B.cen[3 * s + a]);
+                cmx[a] = fmax(cmx[a], B.cen[3 * s + a]);
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            blo[a] = block_reduce(blo[a], lf, MinF());
+            bhi[a] = block_reduce(bhi[a], lf, MaxF());
+            cmn[a] = block_reduce(cmn[a], ld, MinD());
+            cmx[a] = block_reduce(cmx[a], ld, MaxD());
+        }
+        BNode &nd = B.bn[sg.node];
+        const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
+        const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+        const double ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
+        const bool leaf = sg.count <= 4u || ext <= 1e-7;
+        if (threadIdx.x == 0) {
+            for (int a = 0; a < 3; a++) {
+                nd.lo[a] = blo[a];
+                nd.hi[a] = bhi[a];
+            }
+            nd.depth = sg.depth;
+            nd.start = sg.start;
+            nd.count = sg.count;
+        }
+        if (leaf) {
+            for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256)
+                B.final_elem[e] = src[e];
+            if (threadIdx.x == 0) {
+                nd.left = nd.right = kNone;
+                if (sg.count > kMaxLeafCount) atomicOr(&B.err[1], 1u);
+            }
+            __syncthreads();
+            continue;
+        }
+        const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
+                                  (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
+        uint32_t nl = 0;
+        for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256)
+            nl += (B.cen[3 * src[e] + axis] < mid) ? 1u : 0u;
+        nl = block_reduce(nl, lu, SumU());
+        if (nl == 0u || nl == sg.count) {
+            // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
+            for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256) dst[e] = src[e];
+            nl = sg.count / 2;
+        } else {
+            uint32_t base_l = 0, base_r = 0;
+            for (uint32_t c = 0; c < sg.count; c += 256) {
+                const uint32_t e = sg.start + c + threadIdx.x;
+                const bool valid = c + threadIdx.x < sg.count;
+                const uint32_t s = valid ? src[e] : 0u;
+                const bool left = valid && B.cen[3 * s + axis] < mid;
+                const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
+                const unsigned long long below = (1ull << lane) - 1ull;
+                if (lane == 0) {
+                    wl[wave] = (uint32_t)__popcll(ml);
+                    wr[wave] = (uint32_t)__popcll(mr);
+                }
+                __syncthreads();
+                uint32_t ol = 0, orr = 0, tl = 0, tr = 0;
+                for (uint32_t w = 0; w < 4; w++) {
+                    if (w < wave) {
+                        ol += wl[w];
+                        orr += wr[w];
+                    }
+                    tl += wl[w];
+                    tr += wr[w];
+                }
+                if (valid) {
+                    if (left) dst[sg.start + base_l + ol + (uint32_t)__popcll(ml & below)] = s;
+                    else dst[sg.start + nl + base_r + orr + (uint32_t)__popcll(mr & below)] = s;
+                }
+                base_l += tl;
+                base_r += tr;
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) {
+            const TileDev &T = B.tiles[sg.tile];
+            const uint32_t id = atomicAdd(&B.node_count[sg.tile], 2u);
+            const uint32_t l = 2 * T.crystal_base + id, r = l + 1;
+            nd.left = l;
+            nd.right = r;
+            B.bn[l].parent = sg.node;
+            B.bn[r].parent = sg.node;
+            if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
+            const uint32_t k = atomicAdd(&B.seg_count[outc], 2u);
+            out[k] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
+            out[k + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_build_check(Bufs B)
+{
+    if (threadIdx.x == 0 && B.seg_count[kMaxLevels % 3] != 0u) B.err[2] = 1u;
+}
+
+__device__ __forceinline__ bool node_valid(const Bufs &B, uint64_t g, uint32_t &t)
+{
+    t = B.tile_of[g >> 1];
+    const TileDev &T = B.tiles[t];
+    return g - 2ull * T.crystal_base < B.node_count[t];
+}
+
+__global__ void __launch_bounds__(256) k_leafcount(Bufs B, uint64_t cap)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t t;
+    if (g >= cap || !node_valid(B, g, t)) return;
+    if (B.bn[g].left != kNone) return;
+    uint32_t x = (uint32_t)g;
+    while (x != kNone) {
+        atomicAdd(&B.leafcnt[x], 1u);
+        x = B.bn[x].parent;
+    }
+}
+
+// threaded position and skip of every node: a right child sits right after its
+// parent, a left child after its right sibling's subtree (2 * leaves - 1 nodes)
+__global__ void __launch_bounds__(256) k_layout(Bufs B, uint64_t cap)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t t;
+    if (g >= cap || !node_valid(B, g, t)) return;
+    TileDev &T = B.tiles[t];
+    const uint32_t nn = B.node_count[t];
+    uint32_t off = 0, rel = 0, ysize = 0;
+    bool found = false;
+    uint32_t x = (uint32_t)g;
+    for (;;) {
+        const uint32_t p = B.bn[x].parent;
+        if (p == kNone) break;
+        const uint32_t r = B.bn[p].right;
+        if (r == x) {
+            if (!found) {
+                found = true;
+                rel = off;
+                ysize = 2 * B.leafcnt[x] - 1;
+            }
+            off += 1;
+        } else {
+            off += 1 + (2 * B.leafcnt[r] - 1);
+        }
+        x = p;
+    }
+    const BNode &nd = B.bn[g];
+    Node o;
+    for (int a = 0; a < 3; a++) {
+        o.lo[a] = nd.lo[a];
+        o.hi[a] = nd.hi[a];
+    }
+    o.skip = found ? (off - rel) + ysize : nn;
+    o.prim = (nd.left == kNone) ? ((nd.start - T.crystal_base) << 4) | min(nd.count, kMaxLeafCount) : kInterior;
+    B.dnodes[2ull * T.crystal_base + off] = o;
+    if (g == 2ull * T.crystal_base) T.dev_nodes = nn;  // root: one writer per tile
+}
+
+__global__ void __launch_bounds__(256) k_dev_records(Bufs B, const Crystal *recs, uint64_t n)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const TileDev &T = B.tiles[B.tile_of[e]];
+    if (e - T.crystal_base >= T.n_dev) return;
+    const uint32_t s = B.final_elem[e];
+    Crystal r = recs[s];
+    const double fe = B.fe[s];
+    for (int a = 0; a < 3; a++) r.h[a] = (float)(B.dims64[3 * (size_t)s + a] * fe);
+    B.drecs[e] = r;
+}
+
+__global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, uint32_t frame_w,
+                                              uint32_t n_tiles)
+{
+    const uint32_t t = tile_of_block(B.tiles, n_tiles, blockIdx.x);
+    const TileDev &T = B.tiles[t];
+    const uint32_t lb = blockIdx.x - T.block_base;
+    const uint32_t px = (lb % T.blocks_x) * 16 + (threadIdx.x & 15);
+    const uint32_t py = (lb / T.blocks_x) * 16 + (threadIdx.x >> 4);
+    if (px >= T.tw || py >= T.th) return;
+    const Node *nodes = B.dnodes + 2ull * T.crystal_base;
+    const float4 *recs = reinterpret_cast<const float4 *>(B.drecs + T.crystal_base);
+    const uint32_t nn = B.node_count[t];
+    const uint32_t pix = py * T.tw + px;
+    float acc = 0.0f;
+    for (uint32_t ay = 0; ay < S.ss; ay++)
+        for (uint32_t ax = 0; ax < S.ss; ax++)
+            for (uint32_t sy = 0; sy < S.g; sy++)
+                for (uint32_t sx = 0; sx < S.g; sx++) {
+                    float ox, oy;
+                    scan_origin(pix, px, py, ax, ay, sx, sy, S.g, S.ss, T.pw, T.ph, T.skey0, T.skey1, ox, oy);
+                    acc += expf(-S.sigma * silver_path(nodes, nn, recs, ox, oy, S.z0));
+                }
+    const float Tm = acc / (float)(S.g * S.g * S.ss * S.ss);
+    out[(size_t)(T.y0 + py) * frame_w + T.x0 + px] = scan_value(Tm, S.mode);
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- host side -------
+struct BatchedFrame {
+    Bufs B{};
+    std::vector<void *> allocs;
+    BatchedInput in;
+    uint32_t n_tiles = 0;
+    uint64_t total_items = 0;
+    uint32_t total_blocks = 0;
+    int transport_grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+template <class T>
+static hipError_t alloc(BatchedFrame *b, T *&p, size_t count)
+{
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    b->allocs.push_back(v);
+    p = reinterpret_cast<T *>(v);
+    return hipSuccess;
+}
+
+#define TRY(x)                       \
+    do {                             \
+        hipError_t e_ = (x);         \
+        if (e_ != hipSuccess) return e_; \
+    } while (0)
+
+hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **out)
+{
+    BatchedFrame *b = new BatchedFrame();
+    *out = b;
+    b->in = in;
+    b->n_tiles = (uint32_t)in.tiles.size();
+    const size_t n = (size_t)in.n_crystals;
+    Bufs &B = b->B;
+    TRY(alloc(b, B.pos64, 3 * n));
+    TRY(alloc(b, B.dims64, 3 * n));
+    TRY(alloc(b, B.mdim, n));
+    TRY(alloc(b, B.fe, n));
+    TRY(alloc(b, B.cen, 3 * n));
+    TRY(alloc(b, B.orig, n));
+    TRY(alloc(b, B.tile_of, n));
+    TRY(alloc(b, B.devflag, n));
+    TRY(alloc(b, B.parent, n));
+    TRY(alloc(b, B.csize, n));
+    TRY(alloc(b, B.root_flag, n));
+    TRY(alloc(b, B.prefix, n));
+    TRY(alloc(b, B.devpos, n));
+    TRY(alloc(b, B.clump, n));
+    TRY(alloc(b, B.elem[0], n));
+    TRY(alloc(b, B.elem[1], n));
+    TRY(alloc(b, B.final_elem, n));
+    TRY(alloc(b, B.elo, 3 * n));
+    TRY(alloc(b, B.ehi, 3 * n));
+    TRY(alloc(b, B.bn, 2 * n));
+    TRY(alloc(b, B.leafcnt, 2 * n));
+    TRY(alloc(b, B.dnodes, 2 * n));
+    TRY(alloc(b, B.drecs, n));
+    TRY(alloc(b, B.seg[0], n));
+    TRY(alloc(b, B.seg[1], n));
+    TRY(alloc(b, B.seg_count, 3));
+    TRY(alloc(b, B.node_count, b->n_tiles));
+    TRY(alloc(b, B.work, 1));
+    TRY(alloc(b, B.err, 4));
+    TRY(alloc(b, B.tiles, b->n_tiles));
+    // host-side per-slot arrays
+    std::vector<double> mdim(n);
+    std::vector<uint32_t> tile_of(n);
+    std::vector<TileDev> tiles = in.tiles;
+    uint64_t items = 0;
+    uint32_t blocks = 0;
+    for (uint32_t t = 0; t < b->n_tiles; t++) {
+        TileDev &T = tiles[t];
+        T.item_base = items;
+        items += (T.n_photons + 63) / 64;
+        T.blocks_x = (T.tw + 15) / 16;
+        T.blocks = T.blocks_x * ((T.th + 15) / 16);
+        T.block_base = blocks;
+        blocks += T.blocks;
+        double mx = 0.0;
+        for (uint32_t k = 0; k < T.n; k++) {
+            const size_t sl = (size_t)T.crystal_base + k;
+            const double *d = &in.dims64[3 * sl];
+            mdim[sl] = fmax(fmax(d[0], d[1]), d[2]);
+            mx = fmax(mx, mdim[sl]);
+            tile_of[sl] = t;
+        }
+        T.maxb = mx;
+    }
+    b->total_items = items;
+    b->total_blocks = blocks;
+    b->in.tiles = tiles;
+    TRY(hipMemcpyAsync(B.pos64, in.pos64.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    TRY(hipMemcpyAsync(B.dims64, in.dims64.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    TRY(hipMemcpyAsync(B.mdim, mdim.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    TRY(hipMemcpyAsync(B.orig, in.orig.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    TRY(hipMemcpyAsync(B.tile_of, tile_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    TRY(hipMemcpyAsync(B.tiles, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, s));
+    TRY(hipStreamSynchronize(s));
+    int dev = 0, cus = 0, per_cu = 0;
+    TRY(hipGetDevice(&dev));
+    TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_transport), 256, 0));
+    b->transport_grid = std::max(1, cus * std::max(per_cu, 1));
+    return hipSuccess;
+}
+
+static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
+{
+    const BatchedInput &in = b->in;
+    Bufs &B = b->B;
+    const uint64_t n = (uint64_t)in.n_crystals;
+    const uint32_t nb = (uint32_t)((n + 255) / 256), nb2 = (uint32_t)((2 * n + 255) / 256);
+    hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>(nb2, 8192)), dim3(256), 0, s, B, in.counts,
+                       in.stats, n, b->n_tiles);
+    TransportArgs C = in.transport;
+    C.stats = in.stats;
+    C.estride = in.frame_w;
+    hipLaunchKernelGGL(k_transport, dim3(b->transport_grid), dim3(256), 0, s, C, B, in.nodes,
+                       reinterpret_cast<const float4 *>(in.recs), in.counts, in.emap, b->n_tiles,
+                       b->total_items);
+    hipLaunchKernelGGL(k_threshold, dim3(nb), dim3(256), 0, s, B, in.counts, n, in.develop);
+    hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, n, in.develop);
+    hipLaunchKernelGGL(k_roots, dim3(nb), dim3(256), 0, s, B, n);
+    hipLaunchKernelGGL(k_tile_scans, dim3(b->n_tiles), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(k_boost, dim3(nb), dim3(256), 0, s, B, n, in.develop);
+    hipLaunchKernelGGL(k_root_segments, dim3((b->n_tiles + 255) / 256), dim3(256), 0, s, B, b->n_tiles);
+    for (int level = 0; level < kMaxLevels; level++)
+        hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(256), 0, s, B, level);
+    hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B);
+    hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, 2 * n);
+    hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, 2 * n);
+    hipLaunchKernelGGL(k_dev_records, dim3(nb), dim3(256), 0, s, B, in.recs, n);
+    hipLaunchKernelGGL(k_scan, dim3(b->total_blocks), dim3(256), 0, s, B, in.scan, in.dneg, in.frame_w,
+                       b->n_tiles);
+    return hipGetLastError();
+}
+
+hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph)
+{
+    if (!use_graph) return launch_sequence(b, s);
+    if (!b->exec) {
+        TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = launch_sequence(b, s);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return e;
+        if (e2 != hipSuccess) return e2;
+        b->graph = g;
+        TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    return hipGraphLaunch(b->exec, s);
+}
+
+hipError_t batched_read_tiles(BatchedFrame *b, hipStream_t s, std::vector<TileDev> &tiles, uint32_t *err)
+{
+    tiles.resize(b->n_tiles);
+    TRY(hipMemcpyAsync(tiles.data(), b->B.tiles, tiles.size() * sizeof(TileDev), hipMemcpyDeviceToHost, s));
+    TRY(hipMemcpyAsync(err, b->B.err, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
+}
+
+hipError_t batched_read_develop(BatchedFrame *b, hipStream_t s, std::vector<uint8_t> &dev,
+                                std::vector<double> &fe, std::vector<int32_t> &clump)
+{
+    const size_t n = (size_t)b->in.n_crystals;
+    std::vector<uint32_t> d(n);
+    fe.resize(n);
+    clump.resize(n);
+    TRY(hipMemcpyAsync(d.data(), b->B.devflag, n * 4, hipMemcpyDeviceToHost, s));
+    TRY(hipMemcpyAsync(fe.data(), b->B.fe, n * 8, hipMemcpyDeviceToHost, s));
+    TRY(hipMemcpyAsync(clump.data(), b->B.clump, n * 4, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    dev.resize(n);
+    for (size_t i = 0; i < n; i++) dev[i] = (uint8_t)d[i];
+    return hipSuccess;
+}
+
+void batched_destroy(BatchedFrame *b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void *p : b->allocs) (void)hipFree(p);
+    delete b;
+}
+
+}  // namespace fg

@@ -1,0 +1,320 @@
+// Device functions shared by the per-tile kernels (kernels.hip) and the batched
+// pipeline (batched.hip): node and record loads, the photon Philox stream, the
+// transport of one packet and the scan of one pixel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace fg {
+namespace dev {
+
+constexpr float kTwoPi = 6.283185307179586f;
+
+struct NodeView {
+    float lo[3], hi[3];
+    uint32_t skip, prim;
+};
+
+__device__ __forceinline__ NodeView load_node(const Node *nodes, uint32_t i)
+{
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
+    const float4 a = p[0], b = p[1];
+    NodeView v;
+    v.lo[0] = a.x; v.lo[1] = a.y; v.lo[2] = a.z;
+    v.hi[0] = a.w; v.hi[1] = b.x; v.hi[2] = b.y;
+    v.skip = __float_as_uint(b.z);
+    v.prim = __float_as_uint(b.w);
+    return v;
+}
+
+struct Rec {
+    float c[3], sens, q[4], h[3];
+    uint32_t morph;
+};
+
+__device__ __forceinline__ Rec load_rec(const float4 *recs, uint32_t k)
+{
+    const float4 a = recs[3 * k], b = recs[3 * k + 1], c = recs[3 * k + 2];
+    Rec r;
+    r.c[0] = a.x; r.c[1] = a.y; r.c[2] = a.z; r.sens = a.w;
+    r.q[0] = b.x; r.q[1] = b.y; r.q[2] = b.z; r.q[3] = b.w;
+    r.h[0] = c.x; r.h[1] = c.y; r.h[2] = c.z;
+    r.morph = __float_as_uint(c.w);
+    return r;
+}
+
+// Sequential photon stream: draw k is word k & 3 of Philox block k >> 2.
+struct PhotonRng {
+    uint32_t p0, p1, k0, k1, blk, rem;
+    uint32_t b0, b1, b2, b3;
+    __device__ __forceinline__ void init(uint64_t photon, uint32_t key0, uint32_t key1)
+    {
+        p0 = (uint32_t)photon;
+        p1 = (uint32_t)(photon >> 32);
+        k0 = key0;
+        k1 = key1;
+        blk = 0;
+        rem = 0;
+    }
+    __device__ __forceinline__ uint32_t next()
+    {
+        if (rem == 0) {
+            uint32_t c[4] = {p0, p1, blk, kTagPhoton};
+            philox(c, k0, k1);
+            b0 = c[0]; b1 = c[1]; b2 = c[2]; b3 = c[3];
+            blk++;
+            rem = 4;
+        }
+        const uint32_t w = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        rem--;
+        return w;
+    }
+    __device__ __forceinline__ float u() { return u24(next()); }
+};
+
+__device__ __forceinline__ void normalize3(float &x, float &y, float &z)
+{
+    const float n = sqrtf(x * x + y * y + z * z);
+    x = x / n;
+    y = y / n;
+    z = z / n;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+struct PhotonCounters {
+    unsigned long long bounces = 0, nodes = 0, prims = 0, abs = 0, tot = 0;
+};
+
+__device__ __forceinline__ void flush_counters(PhotonCounters &c, unsigned long long *stats)
+{
+    const unsigned long long b = wave_sum(c.bounces), n = wave_sum(c.nodes),
+                             pr = wave_sum(c.prims), ab = wave_sum(c.abs), t = wave_sum(c.tot);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], b);
+        atomicAdd(&stats[1], n);
+        atomicAdd(&stats[2], pr);
+        atomicAdd(&stats[3], ab);
+        atomicAdd(&stats[4], t);
+    }
+}
+
+// One launched packet, the step order of transport.rs:73-185 in f32 with the
+// photon-keyed Philox stream.  Draw order: x, y jitter, theta, phi, Fresnel; per bounce
+// the free path, one Bernoulli per absorbing hit, then the HG pair or the halation
+// draws, then Russian roulette.  Latent photons go to counts[leaf slot].
+__device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, PhotonCounters &cnt)
+{
+    PhotonRng rng;
+    rng.init(p, A.key0, A.key1);
+    float x, y;
+    if (A.ppp > 0) {
+        const uint64_t pix = p / A.ppp;
+        const uint32_t px = (uint32_t)(pix % A.tw), py = (uint32_t)(pix / A.tw);
+        const float ux = rng.u(), uy = rng.u();
+        x = ((float)px + ux) * A.pw;
+        y = ((float)py + uy) * A.ph;
+    } else {
+        const float ux = rng.u(), uy = rng.u();
+        x = ux * A.W;
+        y = uy * A.H;
+    }
+    float w = 1.0f;
+    if (A.emap) {
+        // nearest-pixel lookup of the tile path (image_pipeline.rs:949-961)
+        const float uu = fminf(fmaxf(x / A.W, 0.0f), 1.0f);
+        const float vv = fminf(fmaxf(y / A.H, 0.0f), 1.0f);
+        const uint32_t ix = min((uint32_t)roundf(uu * (float)(A.tw - 1)), A.tw - 1);
+        const uint32_t iy = min((uint32_t)roundf(vv * (float)(A.th - 1)), A.th - 1);
+        w = A.emap[(size_t)iy * A.estride + ix];
+    }
+    w = fmaxf(w, 0.0f);
+    if (!(w > 0.0f)) return;
+    float dx, dy, dz;
+    {
+        const float th = rng.u() * A.theta_max;
+        const float ph = rng.u() * kTwoPi;
+        const float st = __sinf(th);
+        dx = st * __cosf(ph);
+        dy = st * __sinf(ph);
+        dz = __cosf(th);
+        normalize3(dx, dy, dz);
+        // Schlick Fresnel rejection (optics.rs:147-154)
+        const float cz = fminf(fmaxf(fabsf(dz), 0.0f), 1.0f);
+        const float m = 1.0f - cz;
+        const float fr = A.r0 + (1.0f - A.r0) * (m * m * m * m * m);
+        if (rng.u() < fr) return;
+    }
+    float ox = x, oy = y, oz = 0.001f;
+    for (uint32_t b = 0; b < A.max_bounces; b++) {
+        cnt.bounces++;
+        float s;
+        if (A.mu_t <= 0.0f) s = 1e6f;
+        else s = -__logf(1.0f - rng.u()) / A.mu_t;
+        const float step = fminf(s, A.step_cap);
+        const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+        uint32_t i = 0;
+        while (i < A.n_nodes) {
+            const NodeView nd = load_node(A.nodes, i);
+            cnt.nodes++;
+            const bool hit = box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
+            if (hit && nd.prim != kInterior) {
+                const uint32_t first = nd.prim >> 4, nprim = nd.prim & 15u;
+                for (uint32_t k = first; k < first + nprim; k++) {
+                    cnt.prims++;
+                    const Rec r = load_rec(A.recs, k);
+                    float t0, t1;
+                    if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
+                    if (!(t0 < step)) continue;
+                    const float L = fmaxf(fminf(t1, step) - t0, 0.0f);
+                    if (!(L > 0.0f)) continue;
+                    // Beer-Lambert + stochastic rounding (transport.rs:117-134)
+                    const float pa = 1.0f - __expf(-A.mu_abs * L);
+                    const float lam = w * pa * r.sens * 5.0f;
+                    const float fl = fminf(floorf(lam), 4294967040.0f);
+                    const uint32_t whole = fl > 0.0f ? (uint32_t)fl : 0u;
+                    const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
+                    const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
+                    if (k_lat > 0u) {
+                        atomicAdd(&A.counts[k], k_lat);
+                        cnt.abs++;
+                        cnt.tot += k_lat;
+                    }
+                    w *= fmaxf(1.0f - pa * 0.9f, 0.05f);
+                }
+            }
+            i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
+        }
+        const float qx = ox + dx * step, qy = oy + dy * step, qz = oz + dz * step;
+        if (qx < 0.0f || qx >= A.W || qy < 0.0f || qy >= A.H) return;
+        if (qz <= 0.0f) return;
+        if (qz >= A.D) {
+            ox = qx;
+            oy = qy;
+            oz = A.D - 0.001f;
+            // anti-halation backing (optics.rs:160-196)
+            w *= A.hal_eff;
+            if (w < 1e-4f) {
+                if (rng.u() < 0.1f) w /= 0.1f;
+                else return;
+            }
+            if (rng.u() < 0.4f) {
+                const float u1 = rng.u(), u2 = rng.u();
+                const float rr = sqrtf(u1), phi = kTwoPi * u2;
+                dx = rr * __cosf(phi);
+                dy = rr * __sinf(phi);
+                dz = -sqrtf(fmaxf(1.0f - u1, 0.0f));
+                normalize3(dx, dy, dz);
+            } else {
+                dz = -fabsf(dz);
+                normalize3(dx, dy, dz);
+            }
+            // a packet of zero weight deposits nothing from here on
+            if (!(w > 0.0f)) return;
+            continue;
+        }
+        ox = qx;
+        oy = qy;
+        oz = qz;
+        // Henyey-Greenstein (optics.rs:113-143)
+        {
+            const float xi1 = rng.u(), xi2 = rng.u();
+            float ct;
+            if (fabsf(A.g) < 1e-4f) {
+                ct = 2.0f * xi1 - 1.0f;
+            } else {
+                const float term = (1.0f - A.g * A.g) / (1.0f - A.g + 2.0f * A.g * xi1);
+                ct = (1.0f + A.g * A.g - term * term) / (2.0f * A.g);
+            }
+            ct = fminf(fmaxf(ct, -1.0f), 1.0f);
+            const float stt = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+            const float phi = kTwoPi * xi2;
+            float wx = dx, wy = dy, wz = dz;
+            normalize3(wx, wy, wz);
+            float ux, uy, uz;
+            if (fabsf(wx) > 0.1f) { ux = wz; uy = 0.0f; uz = -wx; }
+            else { ux = 0.0f; uy = -wz; uz = wy; }
+            normalize3(ux, uy, uz);
+            const float vx = wy * uz - wz * uy, vy = wz * ux - wx * uz, vz = wx * uy - wy * ux;
+            const float a = stt * __cosf(phi), bb = stt * __sinf(phi);
+            dx = (ux * a + vx * bb) + wx * ct;
+            dy = (uy * a + vy * bb) + wy * ct;
+            dz = (uz * a + vz * bb) + wz * ct;
+            normalize3(dx, dy, dz);
+        }
+        w *= A.albedo;
+        if (w < 1e-3f) {
+            if (rng.u() < 0.1f) w /= 0.1f;
+            else return;
+        }
+    }
+}
+
+// Path length through developed silver of the ray at (ox, oy) along -z: for a ray
+// along -z the reference's slab test (+inf inverse components, NaN-ignoring min/max)
+// is exactly "strictly inside the node's x-y rectangle", every box reaching below the
+// origin; with outward-rounded float boxes and float origins the comparison is
+// exact, so the same crystals are tested as in the double-precision reference.
+__device__ __forceinline__ float silver_path(const Node *nodes, uint32_t n_nodes, const float4 *recs,
+                                             float ox, float oy, float oz)
+{
+    float P = 0.0f;
+    uint32_t i = 0;
+    while (i < n_nodes) {
+        const NodeView nd = load_node(nodes, i);
+        const bool hit = nd.lo[0] < ox && ox < nd.hi[0] && nd.lo[1] < oy && oy < nd.hi[1];
+        if (hit && nd.prim != kInterior) {
+            const uint32_t first = nd.prim >> 4, cnt = nd.prim & 15u;
+            for (uint32_t k = first; k < first + cnt; k++) {
+                const Rec r = load_rec(recs, k);
+                float t0, t1;
+                if (crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, 0.0f, 0.0f, -1.0f, t0, t1))
+                    P += fmaxf(t1 - t0, 0.0f);
+            }
+        }
+        i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
+    }
+    return P;
+}
+
+// Origin of ray (ax, ay, sx, sy) of pixel (px, py): stratified jitter of
+// transmittance.rs:110-120 inside an ss x ss aperture.
+__device__ __forceinline__ void scan_origin(uint32_t pix, uint32_t px, uint32_t py, uint32_t ax,
+                                            uint32_t ay, uint32_t sx, uint32_t sy, uint32_t g,
+                                            uint32_t ss, float pw, float ph, uint32_t key0,
+                                            uint32_t key1, float &ox, float &oy)
+{
+    const uint32_t ray = ((ay * ss + ax) * g + sy) * g + sx;
+    uint32_t c[4] = {pix, ray >> 1, kTagScan, 0u};
+    philox(c, key0, key1);
+    const uint32_t wx = (ray & 1u) ? c[2] : c[0];
+    const uint32_t wy = (ray & 1u) ? c[3] : c[1];
+    const float subx = ((float)sx + u24(wx)) / (float)g;
+    const float suby = ((float)sy + u24(wy)) / (float)g;
+    const float apx = ((float)ax + subx) / (float)ss;
+    const float apy = ((float)ay + suby) / (float)ss;
+    ox = ((float)px + apx) * pw;
+    oy = ((float)py + apy) * ph;
+}
+
+// ScanMode mapping (transmittance.rs:144-152) and D_neg (image_pipeline.rs:1006-1011)
+__device__ __forceinline__ float scan_value(float T, uint32_t mode)
+{
+    if (mode == 0u) return T;
+    if (mode == 1u) return 1.0f - T;
+    if (mode == 2u) {
+        const float t = fmaxf(T, 1e-4f);
+        return fminf(fmaxf(-log10f(t) / 3.0f, 0.0f), 1.0f);
+    }
+    const double t = fmin(fmax((double)T, 1e-4), 1.0);
+    return (float)(-log10(t) + 0.05);
+}
+
+}  // namespace dev
+}  // namespace fg

@@ -12,6 +12,7 @@
 #include <thread>
 #include <vector>
 
+#include "batched.h"
 #include "internal.h"
 #include "kernels.h"
 
@@ -724,6 +725,9 @@ struct fg_frame {
     DevBuf d_nodes, d_recs, d_counts, d_emap, d_dneg, d_dneg2, d_stats, d_scan_nodes, d_scan_recs;
     bool rendered = false;
     bool otf_applied = false;
+    uint32_t last_mode = FG_FRAME_PLAIN;
+    BatchedFrame *batched = nullptr;
+    bool graph_failed = false;
 };
 
 extern "C" int64_t fg_frame_crystal_count(const fg_frame *F) { return F ? F->n_crystals : 0; }
@@ -830,6 +834,7 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
 extern "C" void fg_frame_destroy(fg_frame *F)
 {
     if (!F) return;
+    if (F->batched) batched_destroy(F->batched);
     for (DevBuf *b : {&F->d_nodes, &F->d_recs, &F->d_counts, &F->d_emap, &F->d_dneg, &F->d_dneg2,
                       &F->d_stats, &F->d_scan_nodes, &F->d_scan_recs})
         b->release();
@@ -935,6 +940,106 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
     return FG_OK;
 }
 
+// Batched path: one device pass over all tiles (batched.hip), replayed as a hipGraph.
+static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats *st)
+{
+    const fg_frame_desc &D = F->desc;
+    if (!F->batched) {
+        BatchedInput in;
+        in.nodes = F->d_nodes.as<Node>();
+        in.recs = F->d_recs.as<Crystal>();
+        in.counts = F->d_counts.as<uint32_t>();
+        in.emap = F->d_emap.as<float>();
+        in.dneg = F->d_dneg.as<float>();
+        in.stats = F->d_stats.as<unsigned long long>();
+        in.frame_w = D.width;
+        in.frame_h = D.height;
+        in.n_crystals = F->n_crystals;
+        in.pos64.resize((size_t)F->n_crystals * 3);
+        in.dims64.resize((size_t)F->n_crystals * 3);
+        in.orig.resize((size_t)F->n_crystals);
+        for (size_t k = 0; k < F->tiles.size(); k++) {
+            const TileRec &t = F->tiles[k];
+            const Emulsion &em = F->ems[k];
+            for (int64_t j = 0; j < t.n; j++) {
+                const size_t sl = (size_t)(t.crystal_base + j);
+                const uint32_t ci = em.bvh.idx[(size_t)j];
+                for (int a = 0; a < 3; a++) {
+                    in.pos64[3 * sl + a] = em.pos[3 * (size_t)ci + a];
+                    in.dims64[3 * sl + a] = em.dims[3 * (size_t)ci + a];
+                }
+                in.orig[sl] = ci;
+            }
+            TransportArgs a{};
+            fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1);
+            if (k == 0) in.transport = a;
+            TileDev td{};
+            td.x0 = t.x0;
+            td.y0 = t.y0;
+            td.tw = t.tw;
+            td.th = t.th;
+            td.W = a.W;
+            td.H = a.H;
+            td.pw = a.pw;
+            td.ph = a.ph;
+            td.key0 = a.key0;
+            td.key1 = a.key1;
+            td.skey0 = (uint32_t)(t.seed + 2);
+            td.skey1 = (uint32_t)((t.seed + 2) >> 32);
+            td.node_base = (uint32_t)t.node_base;
+            td.n_nodes = t.n_nodes;
+            td.crystal_base = (uint32_t)t.crystal_base;
+            td.n = (uint32_t)t.n;
+            td.n_photons = a.n_photons;
+            in.tiles.push_back(td);
+        }
+        in.develop = DevelopParams{D.development.threshold_photons, 0u, D.development.filament_expansion,
+                                   D.development.clumping_proximity_factor};
+        const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
+        in.scan.g = std::max((uint32_t)llround(sqrt((double)spp)), 1u);
+        in.scan.ss = std::max(D.scan.supersample, 1u);
+        in.scan.mode = D.scan.mode;
+        in.scan.z0 = (float)(D.emulsion.depth + 0.1);
+        in.scan.sigma = (float)D.scan.silver_opacity;
+        if ((uint64_t)F->n_crystals * 2 >= 0xFFFFFFFFull)
+            return fail(FG_ERR_INVALID_ARG, "batched frame: too many crystals for 32-bit node ids");
+        HIP_TRY(batched_create(in, E->stream, &F->batched), FG_ERR_BUFFER_RANGE);
+    }
+    HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
+    hipError_t e = batched_render(F->batched, E->stream, !F->graph_failed);
+    if (e != hipSuccess && !F->graph_failed) {
+        // graph capture unavailable: launch the same sequence directly from now on
+        F->graph_failed = true;
+        (void)hipGetLastError();
+        e = batched_render(F->batched, E->stream, false);
+    }
+    HIP_TRY(e, FG_ERR_PIPELINE);
+    HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
+    std::vector<TileDev> tiles;
+    uint32_t err[4];
+    HIP_TRY(batched_read_tiles(F->batched, E->stream, tiles, err), FG_ERR_BUFFER_MAP);
+    if (err[0] || err[1] || err[2])
+        return fail(FG_ERR_PIPELINE, "batched frame: developed-crystal BVH exceeds the reference stack depth or leaf size");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
+    unsigned long long cnt[8];
+    HIP_TRY(hipMemcpy(cnt, F->d_stats.p, 64, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+    st->bounces = cnt[0];
+    st->nodes_visited = cnt[1];
+    st->prims_tested = cnt[2];
+    st->abs_events = cnt[3];
+    const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
+    const uint64_t g = std::max((uint32_t)llround(sqrt((double)spp)), 1u), ss = std::max(D.scan.supersample, 1u);
+    for (const TileDev &t : tiles) {
+        st->photons += t.n_photons;
+        st->developed += t.n_dev;
+        st->clumps += t.clumps;
+        st->rays += (uint64_t)t.tw * t.th * g * g * ss * ss;
+    }
+    st->ms_transport = ms;  // whole device pass (graph)
+    return FG_OK;
+}
+
 extern "C" fg_status fg_frame_render(fg_engine *E, fg_frame *F, uint32_t mode, fg_frame_stats *stats)
 {
     if (!E || !F) return fail(FG_ERR_INVALID_ARG, "fg_frame_render: null argument");
@@ -949,7 +1054,7 @@ extern "C" fg_status fg_frame_render(fg_engine *E, fg_frame *F, uint32_t mode, f
     const auto t0 = Clock::now();
     fg_status s;
     if (mode == FG_FRAME_PLAIN) s = frame_render_plain(E, F, &st);
-    else return fail(FG_ERR_PIPELINE, "fg_frame_render: batched mode not available in this build");
+    else s = frame_render_batched(E, F, &st);
     if (s != FG_OK) return s;
     F->otf_applied = false;
     if (F->desc.scan.otf) {
@@ -960,6 +1065,7 @@ extern "C" fg_status fg_frame_render(fg_engine *E, fg_frame *F, uint32_t mode, f
     HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
     st.ms_total = ms_since(t0);
     F->rendered = true;
+    F->last_mode = mode;
     if (stats) *stats = st;
     return FG_OK;
 }
@@ -987,6 +1093,44 @@ extern "C" fg_status fg_frame_read_counts(fg_engine *E, const fg_frame *F, int64
         const TileRec &t = F->tiles[k];
         const Emulsion &em = F->ems[k];
         for (int64_t j = 0; j < t.n; j++) counts[t.crystal_base + em.bvh.idx[(size_t)j]] = c[(size_t)(t.crystal_base + j)];
+    }
+    return FG_OK;
+}
+
+extern "C" fg_status fg_frame_read_development(fg_engine *E, const fg_frame *F, int64_t n, uint8_t *developed,
+                                               double *fe, int64_t *clump_id)
+{
+    if (!E || !F || n != F->n_crystals) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_development: bad argument");
+    if (!F->rendered) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_development: frame not rendered");
+    std::lock_guard<std::mutex> lk(E->mu);
+    if (F->last_mode == FG_FRAME_PLAIN) {
+        for (size_t k = 0; k < F->tiles.size(); k++) {
+            const TileRec &t = F->tiles[k];
+            const Emulsion &em = F->ems[k];
+            for (int64_t j = 0; j < t.n; j++) {
+                const size_t o = (size_t)(t.crystal_base + j);
+                if (developed) developed[o] = em.developed[(size_t)j];
+                if (fe) fe[o] = em.fe[(size_t)j];
+                if (clump_id) clump_id[o] = em.clump[(size_t)j];
+            }
+        }
+        return FG_OK;
+    }
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    std::vector<uint8_t> d;
+    std::vector<double> f;
+    std::vector<int32_t> c;
+    HIP_TRY(batched_read_develop(F->batched, E->stream, d, f, c), FG_ERR_BUFFER_MAP);
+    for (size_t k = 0; k < F->tiles.size(); k++) {
+        const TileRec &t = F->tiles[k];
+        const Emulsion &em = F->ems[k];
+        for (int64_t j = 0; j < t.n; j++) {
+            const size_t sl = (size_t)(t.crystal_base + j);
+            const size_t o = (size_t)t.crystal_base + em.bvh.idx[(size_t)j];
+            if (developed) developed[o] = d[sl];
+            if (fe) fe[o] = f[sl];
+            if (clump_id) clump_id[o] = c[sl];
+        }
     }
     return FG_OK;
 }

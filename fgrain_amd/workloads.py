@@ -106,6 +106,7 @@ class Frame:
         fr = cfg["frame"]
         w, h = (frame or (fr["width"], fr["height"]))
         self.width, self.height, self.tile = w, h, fr["tile"]
+        self.seed = cfg["seed"]
         self.exposure = np.ascontiguousarray(
             exposure if exposure is not None else workload_exposure(cfg), np.float32)
         if self.exposure.shape != (h, w):
@@ -142,6 +143,27 @@ class Frame:
     def read(self) -> np.ndarray:
         out = np.zeros((self.height, self.width), np.float32)
         check(L.lib().fg_frame_read(self.engine._h, self._h, ptr(out, L._f32p)))
+        return out
+
+    def development(self):
+        """(developed u8, filament expansion f64, clump id i64) of the last render."""
+        n = self.crystals
+        d, f, c = np.zeros(n, np.uint8), np.zeros(n), np.zeros(n, np.int64)
+        check(L.lib().fg_frame_read_development(self.engine._h, self._h, n, ptr(d, L._u8p),
+                                                ptr(f, L._f64p), ptr(c, L._i64p)))
+        return d, f, c
+
+    def tile_ranges(self):
+        """(x0, y0, tw, th, seed) per tile, in frame order."""
+        out = []
+        nx = -(-self.width // self.tile)
+        ny = -(-self.height // self.tile)
+        for ty in range(ny):
+            for tx in range(nx):
+                idx = ty * nx + tx
+                seed = (self.seed + idx * 73856093 + tx * 19349663 + ty * 83492791) % (1 << 64)
+                x0, y0 = tx * self.tile, ty * self.tile
+                out.append((x0, y0, min(self.tile, self.width - x0), min(self.tile, self.height - y0), seed))
         return out
 
     def counts(self) -> np.ndarray:

@@ -269,6 +269,10 @@ fg_status fg_frame_read(fg_engine *engine, const fg_frame *frame, float *out);
  * crystal order; n must equal the frame's crystal count. */
 fg_status fg_frame_read_counts(fg_engine *engine, const fg_frame *frame, int64_t n,
                                uint32_t *counts);
+/* Development of the last render (developed flag, filament expansion, clump id or -1),
+ * crystals in tile order then the tile's crystal order. */
+fg_status fg_frame_read_development(fg_engine *engine, const fg_frame *frame, int64_t n,
+                                    uint8_t *developed, double *fe, int64_t *clump_id);
 int64_t fg_frame_crystal_count(const fg_frame *frame);
 
 #ifdef __cplusplus

@@ -1,0 +1,120 @@
+"""Frames on the GPU: the plain per-tile path equals the stage pipeline tile by tile,
+the batched device path (persistent transport, device development and
+developed-crystal BVH, one hipGraph) equals the plain path, graph replays are
+deterministic, and device development equals the oracle."""
+import copy
+
+import numpy as np
+import pytest
+
+import fgrain_amd as F
+from fgrain_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(ppp=16, otf=0, ss=2, seed=5):
+    cfg = copy.deepcopy(W.load_config("frame24mp_trix_cubic"))
+    cfg["exposure"]["photons_per_pixel"] = ppp
+    cfg["scan"]["otf"] = otf
+    cfg["scan"]["supersample"] = ss
+    cfg["seed"] = seed
+    return cfg
+
+
+@pytest.fixture(scope="module")
+def frame(engine):
+    cfg = _cfg()
+    w, h = 680, 420  # 3 x 2 tiles of 320 px, partial edge tiles 40 and 100 px
+    exp = W.blob_exposure(w, h, 5, sigma=(40, 160))
+    return cfg, W.Frame(engine, cfg, exp, frame=(w, h))
+
+
+def _tile_stage(engine, cfg, exp, x0, y0, tw, th, seed):
+    em = F.VirtualEmulsionLayer.synthesize(W.emulsion_config(cfg, float(tw), float(th)), seed)
+    F.expose_emulsion(em, np.ascontiguousarray(exp[y0:y0 + th, x0:x0 + tw]), W.exposure_config(cfg),
+                      seed + 1, engine)
+    F.develop_emulsion(em, W.development_config(cfg), engine)
+    D, _ = F.scan_emulsion(em, W.scan_config(cfg, tw, th), seed + 2, engine)
+    return em, D
+
+
+def test_plain_frame_equals_stage_pipeline_per_tile(engine, frame):
+    cfg, fr = frame
+    st = fr.render(W.Frame.PLAIN)
+    assert st["tiles"] == 6 and st["photons"] == 16 * 680 * 420
+    D = fr.read()
+    counts = fr.counts()
+    base = 0
+    for (x0, y0, tw, th, seed) in fr.tile_ranges():
+        em, Dt = _tile_stage(engine, cfg, fr.exposure, x0, y0, tw, th, seed)
+        n = len(em)
+        assert np.array_equal(counts[base:base + n], em.absorbed_photons())
+        assert (D[y0:y0 + th, x0:x0 + tw].view(np.uint32) == Dt.view(np.uint32)).all()
+        base += n
+    assert base == fr.crystals
+
+
+def test_batched_frame_equals_plain(engine, frame):
+    cfg, fr = frame
+    sp = fr.render(W.Frame.PLAIN)
+    Dp, cp, (dp, fp, kp) = fr.read(), fr.counts(), fr.development()
+    sb = fr.render(W.Frame.BATCHED)
+    Db, cb, (db, fb, kb) = fr.read(), fr.counts(), fr.development()
+    assert np.array_equal(cp, cb)
+    assert np.array_equal(dp, db)
+    assert np.array_equal(kp, kb)
+    assert np.array_equal(fp, fb)
+    assert sp["developed"] == sb["developed"] and sp["clumps"] == sb["clumps"]
+    assert sp["nodes_visited"] == sb["nodes_visited"] and sp["prims_tested"] == sb["prims_tested"]
+    # same developed-crystal BVH leaf sets and boxes; only the summation order of the
+    # chords inside a leaf may differ
+    assert np.abs(Dp - Db).max() <= 1e-5
+
+
+def test_batched_graph_replay_is_deterministic(engine, frame):
+    cfg, fr = frame
+    outs = []
+    for _ in range(3):
+        fr.render(W.Frame.BATCHED)
+        outs.append((fr.read(), fr.counts()))
+    for D, c in outs[1:]:
+        assert (D.view(np.uint32) == outs[0][0].view(np.uint32)).all()
+        assert np.array_equal(c, outs[0][1])
+
+
+def test_batched_development_matches_oracle(engine):
+    cfg = _cfg(ppp=2, seed=11)
+    w, h = 400, 200
+    exp = W.ramp_exposure(w, h, 0.01, 1.5)
+    fr = W.Frame(engine, cfg, exp, frame=(w, h))
+    fr.render(W.Frame.BATCHED)
+    counts = fr.counts()
+    dev, fe, clump = fr.development()
+    d = cfg["development"]
+    base = 0
+    for (x0, y0, tw, th, seed) in fr.tile_ranges():
+        em = F.VirtualEmulsionLayer.synthesize(W.emulsion_config(cfg, float(tw), float(th)), seed)
+        n = len(em)
+        c = em.crystals()
+        cs = O.Crystals(c["pos"], c["dims"], c["quat"], c["morph"], c["sens"])
+        od, ofe, ocid, rep = O.develop(cs, counts[base:base + n], d["threshold_photons"],
+                                       d["filament_expansion"], d["clumping_proximity_factor"])
+        assert 0 < rep["developed"] < n and rep["clumps"] > 1
+        assert np.array_equal(dev[base:base + n], od)
+        assert np.array_equal(clump[base:base + n], ocid)
+        assert np.array_equal(fe[base:base + n], ofe)
+        base += n
+
+
+def test_batched_frame_with_otf_matches_oracle_filter(engine):
+    cfg = _cfg(ppp=4, otf=1, ss=1, seed=3)
+    w, h = 352, 90
+    fr = W.Frame(engine, cfg, W.blob_exposure(w, h, 3, sigma=(20, 80)), frame=(w, h))
+    fr.render(W.Frame.BATCHED)
+    D_otf = fr.read()
+    cfg2 = _cfg(ppp=4, otf=0, ss=1, seed=3)
+    fr2 = W.Frame(engine, cfg2, fr.exposure, frame=(w, h))
+    fr2.render(W.Frame.BATCHED)
+    assert (D_otf.view(np.uint32) == O.otf3(fr2.read()).view(np.uint32)).all()
