@@ -159,11 +159,21 @@ __device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, P
         else s = -__logf(1.0f - rng.u()) / A.mu_t;
         const float step = fminf(s, A.step_cap);
         const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+        // bounding box of the step segment grown by the protrusion reach and a margin
+        // far above float rounding of the coordinates
+        const bool prune = A.reach >= 0.0f;
+        const float ex = ox + dx * step, ey = oy + dy * step, ez = oz + dz * step;
+        const float mg = A.reach + 1e-3f + 1e-5f * (fabsf(ox) + fabsf(oy) + fabsf(oz) + step);
+        const float slo0 = fminf(ox, ex) - mg, shi0 = fmaxf(ox, ex) + mg;
+        const float slo1 = fminf(oy, ey) - mg, shi1 = fmaxf(oy, ey) + mg;
+        const float slo2 = fminf(oz, ez) - mg, shi2 = fmaxf(oz, ez) + mg;
         uint32_t i = 0;
         while (i < A.n_nodes) {
             const NodeView nd = load_node(A.nodes, i);
             cnt.nodes++;
-            const bool hit = box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
+            const bool near = !prune || (nd.lo[0] <= shi0 && nd.hi[0] >= slo0 && nd.lo[1] <= shi1 &&
+                                         nd.hi[1] >= slo1 && nd.lo[2] <= shi2 && nd.hi[2] >= slo2);
+            const bool hit = near && box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
             if (hit && nd.prim != kInterior) {
                 const uint32_t first = nd.prim >> 4, nprim = nd.prim & 15u;
                 for (uint32_t k = first; k < first + nprim; k++) {

@@ -306,9 +306,12 @@ static fg_status ensure_device(fg_engine *E, Emulsion &em)
     return FG_OK;
 }
 
+static bool g_full_traversal = getenv("FG_TRANSPORT_FULL_TRAVERSAL") != nullptr;
+
 static void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const fg_exposure_config &x,
-                           uint32_t tw, uint32_t th, uint64_t seed)
+                           uint32_t tw, uint32_t th, uint64_t seed, double reach)
 {
+    a.reach = g_full_traversal ? -1.0f : (float)reach * 1.0001f;
     const fg_optics &o = ec.optics;
     const double mu_t = o.absorption_coeff + o.scattering_coeff;
     const double r = (1.0 - o.refractive_index) / (1.0 + o.refractive_index);
@@ -348,7 +351,7 @@ static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, ui
     a.nodes = d.nodes.as<Node>();
     a.recs = d.recs.as<float4>();
     a.n_nodes = d.n_nodes;
-    fill_transport(a, em.cfg, cfg, emap ? tw : 1, emap ? th : 1, seed);
+    fill_transport(a, em.cfg, cfg, emap ? tw : 1, emap ? th : 1, seed, em.reach);
     if (emap) {
         HIP_TRY(E->emap.ensure((size_t)tw * th * sizeof(float)), FG_ERR_BUFFER_RANGE);
         HIP_TRY(hipMemcpyAsync(E->emap.p, emap, (size_t)tw * th * sizeof(float), hipMemcpyHostToDevice,
@@ -859,7 +862,7 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
         a.nodes = F->d_nodes.as<Node>() + t.node_base;
         a.recs = reinterpret_cast<const float4 *>(F->d_recs.as<Crystal>() + t.crystal_base);
         a.n_nodes = t.n_nodes;
-        fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1);
+        fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach);
         a.emap = F->d_emap.as<float>() + (size_t)t.y0 * D.width + t.x0;
         a.estride = D.width;
         a.counts = F->d_counts.as<uint32_t>() + t.crystal_base;
@@ -971,7 +974,7 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
                 in.orig[sl] = ci;
             }
             TransportArgs a{};
-            fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1);
+            fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach);
             if (k == 0) in.transport = a;
             TileDev td{};
             td.x0 = t.x0;

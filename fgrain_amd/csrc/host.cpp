@@ -352,6 +352,12 @@ bool finalize_emulsion(Emulsion &em, std::string &msg)
     if (!flatten_bvh(em.bvh, em.nodes, msg)) return false;
     make_records(em, em.bvh, nullptr, nullptr, em.recs);
     em.dev_state_dirty = true;
+    double m = 0.0;
+    for (int64_t i = 0; i < em.n; i++)
+        if (em.morph[(size_t)i] == 0)
+            m = std::max(m, fmax(fmax(em.dims[(size_t)(3 * i)], em.dims[(size_t)(3 * i + 1)]),
+                                 em.dims[(size_t)(3 * i + 2)]));
+    em.reach = (sqrt(3.0) - 1.0) * m;
     return true;
 }
 

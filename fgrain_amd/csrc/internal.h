@@ -37,6 +37,7 @@ struct Emulsion {
     std::vector<Crystal> recs;    // records in leaf order (bvh.idx order)
     DeviceEmulsion *dev = nullptr;
     bool dev_state_dirty = true;  // counts changed on the host since the last upload
+    double reach = 0.0;           // (sqrt(3) - 1) * largest semi-axis of a cubic crystal
 };
 
 // Philox sequential stream with the layout documented in fg_layout.h.

@@ -24,6 +24,13 @@ struct TransportArgs {
     uint32_t max_bounces;
     uint32_t key0, key1;
     uint64_t n_photons;
+    // Exact pruning: a crystal point on the step segment lies within `reach` of the
+    // crystal's AABB (corners of a rotated box stick out by at most (sqrt(3) - 1) times
+    // its largest semi-axis; ellipsoids stay inside).  Nodes whose box, grown by reach,
+    // misses the segment's bounding box hold no crystal that can absorb in this step.
+    // reach < 0 disables the test (full infinite-ray traversal as the reference).
+    float reach;
+    float pad1;
 };
 
 struct ScanArgs {

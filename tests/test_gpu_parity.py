@@ -91,7 +91,9 @@ def test_transport_replay_matches_fp64_oracle(engine):
     # the kernel stops a packet once its weight is exactly zero (halation off zeroes it;
     # it can deposit nothing afterwards): compare the work of packets with weight
     assert st["bounces"] == pytest.approx(ost["bounces_live"], rel=1e-3)
-    assert st["nodes_visited"] == pytest.approx(ost["nodes_live"], rel=1e-3)
+    # the kernel skips subtrees that cannot hold an absorbing crystal (exact pruning);
+    # the reference walks the full infinite ray
+    assert 0.2 * ost["nodes_live"] < st["nodes_visited"] <= ost["nodes_live"] * 1.001
 
 
 def test_transport_replay_with_halation(engine):
