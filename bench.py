@@ -79,7 +79,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["plain", "batched"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile comparison step")
     ap.add_argument("--workload", default="frame24mp_trix_cubic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -117,15 +118,21 @@ def main():
     ms_step = elapsed / args.steps * 1e3
     photons = agg["photons"]
     pixels = agg["pixels"]
-    # dominant kernel: transport; algorithmic bytes per launch from the device counters
-    # of the same reference-order traversal (4 B exposure + 32 B per node + 48 B per
-    # crystal record + 8 B per absorbing hit)
-    alg_bytes = 4 * photons + 32 * agg["nodes_visited"] + 48 * agg["prims_tested"] + 8 * agg["abs_events"]
+    # dominant kernel: transport.  Algorithmic bytes per launch = bytes per photon path of
+    # the reference algorithm (oracle count over sampled tiles of this workload: 4 B
+    # exposure + 32 B per BVH node + 48 B per crystal record + 8 B per absorbing hit on
+    # the full infinite-ray traversal) x the paths one launch processes.
+    per_path = json.load(open(os.path.join(ROOT, "tests", "golden", "roofline_bytes.json")))[
+        args.workload]["transport_per_path"]
+    alg_bytes = per_path["bytes"] * photons
+    dev_bytes = 4 * photons + 32 * agg["nodes_visited"] + 48 * agg["prims_tested"] + 8 * agg["abs_events"]
     roof = None
     if agg["ms_kernel_transport"] > 0:
         ach = alg_bytes * args.steps / (agg["ms_kernel_transport"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
-                "frac": ach / MI355X_HBM_GBPS, "traffic": None}
+                "frac": ach / MI355X_HBM_GBPS, "traffic": None,
+                "bytes_per_path": per_path["bytes"],
+                "launches_per_step": 1 if args.mode == "batched" else agg["tiles"]}
     out = {
         "metric": "Mphoton-paths/s + scanned Mpix/s, 24MP Tri-X, 1 GPU; adjoint/forward time ratio",
         "value": photons / (ms_step * 1e-3) / 1e6,
@@ -149,9 +156,18 @@ def main():
         "ms_breakdown": {k: agg[k] / args.steps for k in ("ms_transport", "ms_develop", "ms_scan",
                                                           "ms_kernel_transport", "ms_kernel_scan")},
         "roofline": roof,
-        "transport_per_path": {"bytes": alg_bytes / photons, "nodes": agg["nodes_visited"] / photons,
-                               "prims": agg["prims_tested"] / photons, "bounces": agg["bounces"] / photons},
+        "transport_per_path_executed": {
+            "bytes": dev_bytes / photons, "nodes": agg["nodes_visited"] / photons,
+            "prims": agg["prims_tested"] / photons, "bounces": agg["bounces"] / photons},
+        "transport_per_path_reference": per_path,
     }
+    if args.mode == "batched" and not args.no_plain:
+        log("plain per-tile comparison step ...")
+        t1 = time.perf_counter()
+        frame.render(W.Frame.PLAIN)
+        plain_ms = (time.perf_counter() - t1) * 1e3
+        out["plain_ms_per_step"] = plain_ms
+        out["batched_speedup_vs_plain"] = plain_ms / ms_step
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, host cores) ...")
         out["cpu_baseline"] = cpu_baseline(cfg, exposure)

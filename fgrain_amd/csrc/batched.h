@@ -65,6 +65,9 @@ hipError_t batched_read_tiles(BatchedFrame *b, hipStream_t s, std::vector<TileDe
                               uint32_t *err_flags);
 hipError_t batched_read_develop(BatchedFrame *b, hipStream_t s, std::vector<uint8_t> &dev,
                                 std::vector<double> &fe, std::vector<int32_t> &clump);
+// Kernel times of the last render (HIP events around transport, development + BVH,
+// and scan, recorded on the render stream; inside the graph as event-record nodes).
+hipError_t batched_kernel_ms(BatchedFrame *b, float *transport, float *develop, float *scan);
 void batched_destroy(BatchedFrame *b);
 
 }  // namespace fg

@@ -571,6 +571,8 @@ struct BatchedFrame {
     int transport_grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // transport and scan brackets
+    bool capturing = false;
 };
 
 template <class T>
@@ -667,7 +669,13 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_transport), 256, 0));
     b->transport_grid = std::max(1, cus * std::max(per_cu, 1));
+    for (auto &e : b->ev) TRY(hipEventCreate(&e));
     return hipSuccess;
+}
+
+static hipError_t mark(BatchedFrame *b, int k, hipStream_t s)
+{
+    return hipEventRecordWithFlags(b->ev[k], s, b->capturing ? hipEventRecordExternal : 0u);
 }
 
 static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
@@ -681,9 +689,11 @@ static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
     TransportArgs C = in.transport;
     C.stats = in.stats;
     C.estride = in.frame_w;
+    TRY(mark(b, 0, s));
     hipLaunchKernelGGL(k_transport, dim3(b->transport_grid), dim3(256), 0, s, C, B, in.nodes,
                        reinterpret_cast<const float4 *>(in.recs), in.counts, in.emap, b->n_tiles,
                        b->total_items);
+    TRY(mark(b, 1, s));
     hipLaunchKernelGGL(k_threshold, dim3(nb), dim3(256), 0, s, B, in.counts, n, in.develop);
     hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, n, in.develop);
     hipLaunchKernelGGL(k_roots, dim3(nb), dim3(256), 0, s, B, n);
@@ -696,8 +706,10 @@ static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
     hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, 2 * n);
     hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, 2 * n);
     hipLaunchKernelGGL(k_dev_records, dim3(nb), dim3(256), 0, s, B, in.recs, n);
+    TRY(mark(b, 2, s));
     hipLaunchKernelGGL(k_scan, dim3(b->total_blocks), dim3(256), 0, s, B, in.scan, in.dneg, in.frame_w,
                        b->n_tiles);
+    TRY(mark(b, 3, s));
     return hipGetLastError();
 }
 
@@ -706,7 +718,9 @@ hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph)
     if (!use_graph) return launch_sequence(b, s);
     if (!b->exec) {
         TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        b->capturing = true;
         hipError_t e = launch_sequence(b, s);
+        b->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (e != hipSuccess) return e;
@@ -741,9 +755,19 @@ hipError_t batched_read_develop(BatchedFrame *b, hipStream_t s, std::vector<uint
     return hipSuccess;
 }
 
+hipError_t batched_kernel_ms(BatchedFrame *b, float *transport, float *develop, float *scan)
+{
+    TRY(hipEventSynchronize(b->ev[3]));
+    TRY(hipEventElapsedTime(transport, b->ev[0], b->ev[1]));
+    TRY(hipEventElapsedTime(develop, b->ev[1], b->ev[2]));
+    return hipEventElapsedTime(scan, b->ev[2], b->ev[3]);
+}
+
 void batched_destroy(BatchedFrame *b)
 {
     if (!b) return;
+    for (auto &e : b->ev)
+        if (e) (void)hipEventDestroy(e);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : b->allocs) (void)hipFree(p);

@@ -1039,7 +1039,14 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
         st->clumps += t.clumps;
         st->rays += (uint64_t)t.tw * t.th * g * g * ss * ss;
     }
-    st->ms_transport = ms;  // whole device pass (graph)
+    float tk = 0, dk = 0, sk = 0;
+    HIP_TRY(batched_kernel_ms(F->batched, &tk, &dk, &sk), FG_ERR_CHANNEL);
+    st->ms_kernel_transport = tk;
+    st->ms_kernel_scan = sk;
+    st->ms_transport = tk;
+    st->ms_develop = dk;
+    st->ms_scan = sk;
+    (void)ms;
     return FG_OK;
 }
 
