@@ -83,11 +83,23 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile comparison step")
     ap.add_argument("--workload", default="frame24mp_trix_cubic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 4 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(tiles would shard across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20, "copies": 4}))
+        return
 
     import fgrain_amd as F
     from fgrain_amd import workloads as W
