@@ -69,6 +69,11 @@ class ScanConfigC(C.Structure):
                 ("silver_opacity", C.c_double), ("supersample", C.c_uint32), ("otf", C.c_uint32)]
 
 
+class TuningC(C.Structure):
+    _fields_ = [("transport_kernel", C.c_uint32), ("transport_chunk", C.c_uint32),
+                ("transport_wait_pct", C.c_uint32), ("scan_rays_per_thread", C.c_uint32)]
+
+
 class TileParamsC(C.Structure):
     _fields_ = [("film_w", C.c_double), ("film_h", C.c_double), ("film_d", C.c_double),
                 ("width_px", C.c_uint32), ("height_px", C.c_uint32),
@@ -116,6 +121,8 @@ SIGNATURES = {
     "fg_engine_destroy": (None, [_vp]),
     "fg_engine_adapter_name": (C.c_char_p, [_vp]),
     "fg_get_engine": (_st, [C.POINTER(_vp)]),
+    "fg_engine_set_tuning": (_st, [_vp, C.POINTER(TuningC)]),
+    "fg_engine_get_tuning": (_st, [_vp, C.POINTER(TuningC)]),
     "fg_emulsion_synthesize": (_st, [C.POINTER(EmulsionConfigC), C.c_uint64, C.POINTER(_vp)]),
     "fg_emulsion_from_crystals": (_st, [C.POINTER(EmulsionConfigC), C.c_int64, _f64p, _f64p,
                                         _f64p, _i32p, _f64p, C.POINTER(_vp)]),

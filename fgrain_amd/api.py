@@ -281,6 +281,17 @@ class GpuEngine:
     def adapter_name(self) -> str:
         return L.lib().fg_engine_adapter_name(self._h).decode()
 
+    def set_tuning(self, transport_kernel=0, transport_chunk=64, transport_wait_pct=80,
+                   scan_rays_per_thread=0):
+        """Launch parameters (speed only; results do not depend on them)."""
+        t = L.TuningC(transport_kernel, transport_chunk, transport_wait_pct, scan_rays_per_thread)
+        check(L.lib().fg_engine_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = L.TuningC()
+        check(L.lib().fg_engine_get_tuning(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in L.TuningC._fields_}
+
     def render_tile_all_gpu(self, tile_img, params: "GpuTileParams") -> np.ndarray:
         """GpuEngine::render_tile_all_gpu (gpu.rs:491-938): D_neg of one tile."""
         p = params.c

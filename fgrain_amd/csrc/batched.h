@@ -10,23 +10,6 @@
 
 namespace fg {
 
-// Per-tile descriptor shared by host and device.
-struct TileDev {
-    uint32_t x0, y0, tw, th;        // frame placement (pixels)
-    float W, H, pw, ph;             // extents and pixel size (length units)
-    uint32_t key0, key1;            // transport seed (tile seed + 1)
-    uint32_t skey0, skey1;          // scan seed (tile seed + 2)
-    uint32_t node_base, n_nodes;    // transport BVH nodes
-    uint32_t crystal_base, n;       // crystal slots (leaf order)
-    uint64_t item_base;             // first 64-packet work item
-    uint64_t n_photons;
-    uint32_t block_base, blocks_x;  // scan blocks of 16 x 16 pixels
-    uint32_t blocks, pad0;
-    double maxb;                    // largest semi-axis among the tile's crystals
-    // results of the last render
-    uint32_t n_dev, clumps, max_clump, dev_nodes;
-};
-
 struct DevelopParams {
     uint32_t threshold;
     uint32_t pad;

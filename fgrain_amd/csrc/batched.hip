@@ -54,7 +54,7 @@ struct Bufs {
     Seg *seg[2];
     uint32_t *seg_count;   // 3 rotating counters
     uint32_t *node_count;  // per tile
-    uint32_t *work;        // transport work counter
+    unsigned long long *work;  // transport work counter
     uint32_t *err;         // [0] depth, [1] leaf size, [2] unfinished build
     TileDev *tiles;
 };
@@ -100,17 +100,6 @@ __device__ void uf_unite(uint32_t *parent, const uint32_t *orig, uint32_t a, uin
     }
 }
 
-__device__ __forceinline__ uint32_t tile_of_item(const TileDev *tiles, uint32_t n_tiles, uint64_t item)
-{
-    uint32_t lo = 0, hi = n_tiles;  // last tile with item_base <= item
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (tiles[mid].item_base <= item) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ uint32_t tile_of_block(const TileDev *tiles, uint32_t n_tiles, uint32_t b)
 {
     uint32_t lo = 0, hi = n_tiles;
@@ -142,7 +131,7 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
         if (i < 4) {
             B.err[i] = 0u;
             if (i < 3) B.seg_count[i] = 0u;
-            if (i == 3) *B.work = 0u;
+            if (i == 3) *B.work = 0ull;
         }
     }
 }
@@ -151,37 +140,10 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
 __global__ void __launch_bounds__(256) k_transport(TransportArgs C, Bufs B, const Node *nodes,
                                                    const float4 *recs, uint32_t *counts,
                                                    const float *emap, uint32_t n_tiles,
-                                                   uint64_t total_items)
+                                                   uint64_t total_photons)
 {
     PhotonCounters cnt;
-    const uint32_t lane = threadIdx.x & 63;
-    for (;;) {
-        uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(B.work, 1u);
-        item = __shfl(item, 0, 64);
-        if (item >= total_items) break;
-        const uint32_t t = tile_of_item(B.tiles, n_tiles, item);
-        const TileDev T = B.tiles[t];
-        const uint64_t p = (item - T.item_base) * 64 + lane;
-        if (p < T.n_photons) {
-            TransportArgs A = C;
-            A.nodes = nodes + T.node_base;
-            A.recs = recs + 3 * (size_t)T.crystal_base;
-            A.n_nodes = T.n_nodes;
-            A.emap = emap + (size_t)T.y0 * C.estride + T.x0;
-            A.tw = T.tw;
-            A.th = T.th;
-            A.counts = counts + T.crystal_base;
-            A.W = T.W;
-            A.H = T.H;
-            A.pw = T.pw;
-            A.ph = T.ph;
-            A.key0 = T.key0;
-            A.key1 = T.key1;
-            A.n_photons = T.n_photons;
-            run_photon(A, p, cnt);
-        }
-    }
+    transport_persistent<true>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work, total_photons, cnt);
     flush_counters(cnt, C.stats);
 }
 
@@ -567,6 +529,7 @@ struct BatchedFrame {
     BatchedInput in;
     uint32_t n_tiles = 0;
     uint64_t total_items = 0;
+    uint64_t total_photons = 0;
     uint32_t total_blocks = 0;
     int transport_grid = 0;
     hipGraph_t graph = nullptr;
@@ -627,7 +590,7 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.seg[1], n));
     TRY(alloc(b, B.seg_count, 3));
     TRY(alloc(b, B.node_count, b->n_tiles));
-    TRY(alloc(b, B.work, 1));
+    TRY(alloc(b, B.work, 1));  // 64-bit packet counter
     TRY(alloc(b, B.err, 4));
     TRY(alloc(b, B.tiles, b->n_tiles));
     // host-side per-slot arrays
@@ -640,6 +603,8 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
         TileDev &T = tiles[t];
         T.item_base = items;
         items += (T.n_photons + 63) / 64;
+        T.photon_base = b->total_photons;
+        b->total_photons += T.n_photons;
         T.blocks_x = (T.tw + 15) / 16;
         T.blocks = T.blocks_x * ((T.th + 15) / 16);
         T.block_base = blocks;
@@ -692,7 +657,7 @@ static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
     TRY(mark(b, 0, s));
     hipLaunchKernelGGL(k_transport, dim3(b->transport_grid), dim3(256), 0, s, C, B, in.nodes,
                        reinterpret_cast<const float4 *>(in.recs), in.counts, in.emap, b->n_tiles,
-                       b->total_items);
+                       b->total_photons);
     TRY(mark(b, 1, s));
     hipLaunchKernelGGL(k_threshold, dim3(nb), dim3(256), 0, s, B, in.counts, n, in.develop);
     hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, n, in.develop);

@@ -266,6 +266,301 @@ __device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, P
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Persistent transport.  Each lane runs a small state machine (wait for a packet,
+// start a bounce, walk the BVH, process a pending leaf, finish the bounce) so that a
+// lane whose packet ends takes the next packet immediately instead of idling until
+// the slowest lane of its wave finishes.  Packets are handed out from a wave-local
+// pool refilled with one atomic per `chunk` packets, consecutive ids to consecutive
+// lanes (the ids of one pixel).  Node steps continue until `wait_pct` percent of the
+// walking lanes hold a hit leaf, then those leaves are processed together.  For every
+// packet the draws, the order of its hits and its deposits are those of run_photon.
+
+struct LaneTile {
+    const Node *nodes;
+    const float4 *recs;
+    uint32_t *counts;
+    const float *emap;
+    uint32_t n_nodes, tw, th, key0, key1;
+    float W, H, pw, ph;
+};
+
+__device__ __forceinline__ void lane_tile_single(const TransportArgs &C, LaneTile &T)
+{
+    T.nodes = C.nodes;
+    T.recs = C.recs;
+    T.counts = C.counts;
+    T.emap = C.emap;
+    T.n_nodes = C.n_nodes;
+    T.tw = C.tw;
+    T.th = C.th;
+    T.key0 = C.key0;
+    T.key1 = C.key1;
+    T.W = C.W;
+    T.H = C.H;
+    T.pw = C.pw;
+    T.ph = C.ph;
+}
+
+// frame packet id g -> tile (last tile with photon_base <= g) and its constants
+__device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32_t n_tiles, uint64_t g,
+                                                    const Node *nodes, const float4 *recs,
+                                                    uint32_t *counts, const float *emap,
+                                                    uint32_t estride, LaneTile &T)
+{
+    uint32_t lo = 0, hi = n_tiles;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tiles[mid].photon_base <= g) lo = mid;
+        else hi = mid;
+    }
+    const TileDev &t = tiles[lo];
+    T.nodes = nodes + t.node_base;
+    T.recs = recs + 3 * (size_t)t.crystal_base;
+    T.counts = counts + t.crystal_base;
+    T.emap = emap + (size_t)t.y0 * estride + t.x0;
+    T.n_nodes = t.n_nodes;
+    T.tw = t.tw;
+    T.th = t.th;
+    T.key0 = t.key0;
+    T.key1 = t.key1;
+    T.W = t.W;
+    T.H = t.H;
+    T.pw = t.pw;
+    T.ph = t.ph;
+    return g - t.photon_base;
+}
+
+enum : uint32_t { kNeed = 0, kStart = 1, kWalk = 2, kDone = 3 };
+
+template <bool kFrame>
+__device__ __forceinline__ void transport_persistent(const TransportArgs &C, const TileDev *tiles,
+                                                     uint32_t n_tiles, const Node *nodes,
+                                                     const float4 *recs, uint32_t *counts,
+                                                     const float *emap, unsigned long long *work,
+                                                     uint64_t total, PhotonCounters &cnt)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t chunk = max(C.chunk, 64u);
+    unsigned long long pool = 0;
+    uint32_t pool_left = 0;
+    uint32_t phase = kNeed;
+    LaneTile T;
+    if (!kFrame) lane_tile_single(C, T);
+    PhotonRng rng;
+    float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f, w = 0.f;
+    float step = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, h0 = 0.f, h1 = 0.f, h2 = 0.f;
+    uint32_t b = 0, i = 0, leaf = 0;
+    const bool prune = C.reach >= 0.0f;
+    for (;;) {
+        // ---- hand out packets to lanes without one
+        const unsigned long long nm = __ballot(phase == kNeed);
+        if (nm) {
+            const uint32_t n = (uint32_t)__popcll(nm), r = (uint32_t)__popcll(nm & below);
+            unsigned long long id = pool + r;
+            if (n > pool_left) {
+                unsigned long long nb = 0;
+                if (lane == 0) nb = atomicAdd(work, (unsigned long long)chunk);
+                nb = __shfl(nb, 0, 64);
+                if (r >= pool_left) id = nb + (r - pool_left);
+                pool = nb + (n - pool_left);
+                pool_left = chunk - (n - pool_left);
+            } else {
+                pool += n;
+                pool_left -= n;
+            }
+            if (phase == kNeed) {
+                if (id >= total) {
+                    phase = kDone;
+                } else {
+                    uint64_t p = id;
+                    if (kFrame) p = lane_tile_frame(tiles, n_tiles, id, nodes, recs, counts, emap, C.estride, T);
+                    rng.init(p, T.key0, T.key1);
+                    float x, y;
+                    if (C.ppp > 0) {
+                        const uint64_t pix = p / C.ppp;
+                        const uint32_t px = (uint32_t)(pix % T.tw), py = (uint32_t)(pix / T.tw);
+                        const float ux = rng.u(), uy = rng.u();
+                        x = ((float)px + ux) * T.pw;
+                        y = ((float)py + uy) * T.ph;
+                    } else {
+                        const float ux = rng.u(), uy = rng.u();
+                        x = ux * T.W;
+                        y = uy * T.H;
+                    }
+                    w = 1.0f;
+                    if (T.emap) {
+                        const float uu = fminf(fmaxf(x / T.W, 0.0f), 1.0f);
+                        const float vv = fminf(fmaxf(y / T.H, 0.0f), 1.0f);
+                        const uint32_t ex = min((uint32_t)roundf(uu * (float)(T.tw - 1)), T.tw - 1);
+                        const uint32_t ey = min((uint32_t)roundf(vv * (float)(T.th - 1)), T.th - 1);
+                        w = T.emap[(size_t)ey * C.estride + ex];
+                    }
+                    w = fmaxf(w, 0.0f);
+                    if (w > 0.0f) {
+                        const float th = rng.u() * C.theta_max;
+                        const float ph = rng.u() * kTwoPi;
+                        const float st = __sinf(th);
+                        dx = st * __cosf(ph);
+                        dy = st * __sinf(ph);
+                        dz = __cosf(th);
+                        normalize3(dx, dy, dz);
+                        const float cz = fminf(fmaxf(fabsf(dz), 0.0f), 1.0f);
+                        const float m = 1.0f - cz;
+                        const float fr = C.r0 + (1.0f - C.r0) * (m * m * m * m * m);
+                        if (!(rng.u() < fr)) {
+                            ox = x;
+                            oy = y;
+                            oz = 0.001f;
+                            b = 0;
+                            phase = kStart;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(phase != kDone) == 0ull) break;
+        // ---- start a bounce: free path, step, segment box
+        if (phase == kStart) {
+            if (b >= C.max_bounces) {
+                phase = kNeed;
+            } else {
+                b++;
+                cnt.bounces++;
+                float s;
+                if (C.mu_t <= 0.0f) s = 1e6f;
+                else s = -__logf(1.0f - rng.u()) / C.mu_t;
+                step = fminf(s, C.step_cap);
+                ix = 1.0f / dx;
+                iy = 1.0f / dy;
+                iz = 1.0f / dz;
+                const float ex = ox + dx * step, ey = oy + dy * step, ez = oz + dz * step;
+                const float mg = C.reach + 1e-3f + 1e-5f * (fabsf(ox) + fabsf(oy) + fabsf(oz) + step);
+                s0 = fminf(ox, ex) - mg;
+                h0 = fmaxf(ox, ex) + mg;
+                s1 = fminf(oy, ey) - mg;
+                h1 = fmaxf(oy, ey) + mg;
+                s2 = fminf(oz, ez) - mg;
+                h2 = fmaxf(oz, ez) + mg;
+                i = 0;
+                leaf = 0;
+                phase = kWalk;
+            }
+        }
+        // ---- node steps (reference DFS order via skip links)
+        {
+            const uint32_t walking = (uint32_t)__popcll(__ballot(phase == kWalk));
+            const uint32_t stop = (walking * (100u - min(C.wait_pct, 100u))) / 100u;
+            for (uint32_t it = 0;; it++) {
+                const bool stepping = phase == kWalk && leaf == 0u && i < T.n_nodes;
+                const uint32_t ns = (uint32_t)__popcll(__ballot(stepping));
+                if (ns == 0u || (it > 0u && ns <= stop)) break;
+                if (stepping) {
+                    const NodeView nd = load_node(T.nodes, i);
+                    cnt.nodes++;
+                    const bool near = !prune || (nd.lo[0] <= h0 && nd.hi[0] >= s0 && nd.lo[1] <= h1 &&
+                                                 nd.hi[1] >= s1 && nd.lo[2] <= h2 && nd.hi[2] >= s2);
+                    const bool hit = near && box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
+                    if (hit && nd.prim != kInterior) leaf = nd.prim;
+                    i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
+                }
+            }
+        }
+        // ---- the pending leaf: Beer-Lambert on each crystal hit before the step end
+        if (phase == kWalk && leaf != 0u) {
+            const uint32_t first = leaf >> 4, nprim = leaf & 15u;
+            for (uint32_t k = first; k < first + nprim; k++) {
+                cnt.prims++;
+                const Rec r = load_rec(T.recs, k);
+                float t0, t1;
+                if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
+                if (!(t0 < step)) continue;
+                const float L = fmaxf(fminf(t1, step) - t0, 0.0f);
+                if (!(L > 0.0f)) continue;
+                const float pa = 1.0f - __expf(-C.mu_abs * L);
+                const float lam = w * pa * r.sens * 5.0f;
+                const float fl = fminf(floorf(lam), 4294967040.0f);
+                const uint32_t whole = fl > 0.0f ? (uint32_t)fl : 0u;
+                const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
+                const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
+                if (k_lat > 0u) {
+                    atomicAdd(&T.counts[k], k_lat);
+                    cnt.abs++;
+                    cnt.tot += k_lat;
+                }
+                w *= fmaxf(1.0f - pa * 0.9f, 0.05f);
+            }
+            leaf = 0u;
+        }
+        // ---- end of the bounce: move, boundaries, halation or scattering, roulette
+        if (phase == kWalk && leaf == 0u && i >= T.n_nodes) {
+            const float qx = ox + dx * step, qy = oy + dy * step, qz = oz + dz * step;
+            if (qx < 0.0f || qx >= T.W || qy < 0.0f || qy >= T.H || qz <= 0.0f) {
+                phase = kNeed;
+            } else if (qz >= C.D) {
+                ox = qx;
+                oy = qy;
+                oz = C.D - 0.001f;
+                phase = kStart;
+                w *= C.hal_eff;
+                if (w < 1e-4f) {
+                    if (rng.u() < 0.1f) w /= 0.1f;
+                    else phase = kNeed;
+                }
+                if (phase == kStart) {
+                    if (rng.u() < 0.4f) {
+                        const float u1 = rng.u(), u2 = rng.u();
+                        const float rr = sqrtf(u1), phi = kTwoPi * u2;
+                        dx = rr * __cosf(phi);
+                        dy = rr * __sinf(phi);
+                        dz = -sqrtf(fmaxf(1.0f - u1, 0.0f));
+                        normalize3(dx, dy, dz);
+                    } else {
+                        dz = -fabsf(dz);
+                        normalize3(dx, dy, dz);
+                    }
+                    if (!(w > 0.0f)) phase = kNeed;
+                }
+            } else {
+                ox = qx;
+                oy = qy;
+                oz = qz;
+                const float xi1 = rng.u(), xi2 = rng.u();
+                float ct;
+                if (fabsf(C.g) < 1e-4f) {
+                    ct = 2.0f * xi1 - 1.0f;
+                } else {
+                    const float term = (1.0f - C.g * C.g) / (1.0f - C.g + 2.0f * C.g * xi1);
+                    ct = (1.0f + C.g * C.g - term * term) / (2.0f * C.g);
+                }
+                ct = fminf(fmaxf(ct, -1.0f), 1.0f);
+                const float stt = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+                const float phi = kTwoPi * xi2;
+                float wx = dx, wy = dy, wz = dz;
+                normalize3(wx, wy, wz);
+                float ux, uy, uz;
+                if (fabsf(wx) > 0.1f) { ux = wz; uy = 0.0f; uz = -wx; }
+                else { ux = 0.0f; uy = -wz; uz = wy; }
+                normalize3(ux, uy, uz);
+                const float vx = wy * uz - wz * uy, vy = wz * ux - wx * uz, vz = wx * uy - wy * ux;
+                const float a = stt * __cosf(phi), bb = stt * __sinf(phi);
+                dx = (ux * a + vx * bb) + wx * ct;
+                dy = (uy * a + vy * bb) + wy * ct;
+                dz = (uz * a + vz * bb) + wz * ct;
+                normalize3(dx, dy, dz);
+                w *= C.albedo;
+                phase = kStart;
+                if (w < 1e-3f) {
+                    if (rng.u() < 0.1f) w /= 0.1f;
+                    else phase = kNeed;
+                }
+            }
+        }
+    }
+}
+
 // Path length through developed silver of the ray at (ox, oy) along -z: for a ray
 // along -z the reference's slab test (+inf inverse components, NaN-ignoring min/max)
 // is exactly "strictly inside the node's x-y rectangle", every box reaching below the

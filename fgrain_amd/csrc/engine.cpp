@@ -71,6 +71,9 @@ struct DeviceEmulsion {
 
 struct fg_engine {
     int device = 0;
+    int cus = 0;
+    int persistent_grid = 0;
+    fg_tuning tuning{0u, 64u, 80u, 0u};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string name;
@@ -111,6 +114,8 @@ extern "C" fg_status fg_engine_create(fg_engine **out)
     }
     E->name = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
               std::to_string(prop.multiProcessorCount) + " CUs)";
+    E->cus = prop.multiProcessorCount;
+    E->persistent_grid = E->cus * transport_persistent_blocks_per_cu();
     *out = E;
     return FG_OK;
 }
@@ -128,6 +133,26 @@ extern "C" void fg_engine_destroy(fg_engine *E)
 }
 
 extern "C" const char *fg_engine_adapter_name(const fg_engine *E) { return E ? E->name.c_str() : ""; }
+
+extern "C" fg_status fg_engine_set_tuning(fg_engine *E, const fg_tuning *t)
+{
+    if (!E || !t) return fail(FG_ERR_INVALID_ARG, "fg_engine_set_tuning: null argument");
+    if (t->transport_kernel > 1 || t->transport_wait_pct > 100)
+        return fail(FG_ERR_INVALID_ARG, "fg_engine_set_tuning: transport_kernel must be 0 or 1, wait_pct <= 100");
+    std::lock_guard<std::mutex> lk(E->mu);
+    E->tuning.transport_kernel = t->transport_kernel;
+    E->tuning.transport_chunk = t->transport_chunk ? std::max(t->transport_chunk, 64u) : 64u;
+    E->tuning.transport_wait_pct = t->transport_wait_pct ? t->transport_wait_pct : 80u;
+    E->tuning.scan_rays_per_thread = t->scan_rays_per_thread;
+    return FG_OK;
+}
+
+extern "C" fg_status fg_engine_get_tuning(const fg_engine *E, fg_tuning *t)
+{
+    if (!E || !t) return fail(FG_ERR_INVALID_ARG, "fg_engine_get_tuning: null argument");
+    *t = E->tuning;
+    return FG_OK;
+}
 
 static std::once_flag g_engine_once;
 static fg_engine *g_engine = nullptr;
@@ -309,8 +334,10 @@ static fg_status ensure_device(fg_engine *E, Emulsion &em)
 static bool g_full_traversal = getenv("FG_TRANSPORT_FULL_TRAVERSAL") != nullptr;
 
 static void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const fg_exposure_config &x,
-                           uint32_t tw, uint32_t th, uint64_t seed, double reach)
+                           uint32_t tw, uint32_t th, uint64_t seed, double reach, const fg_tuning &tn)
 {
+    a.chunk = tn.transport_chunk;
+    a.wait_pct = tn.transport_wait_pct;
     a.reach = g_full_traversal ? -1.0f : (float)reach * 1.0001f;
     const fg_optics &o = ec.optics;
     const double mu_t = o.absorption_coeff + o.scattering_coeff;
@@ -340,6 +367,13 @@ static void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const
         a.n_photons = (uint64_t)llround(ec.width * ec.height * x.photons_per_sq_unit);
 }
 
+// The transport kernel the tuning selects; work is a zeroed 64-bit counter.
+static hipError_t launch_transport_any(fg_engine *E, const TransportArgs &a, unsigned long long *work)
+{
+    if (E->tuning.transport_kernel == 1) return launch_transport(a, E->stream);
+    return launch_transport_persistent(a, work, E->persistent_grid, E->stream);
+}
+
 static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, uint32_t tw,
                                uint32_t th, const fg_exposure_config &cfg, uint64_t seed,
                                fg_expose_stats *stats)
@@ -351,7 +385,7 @@ static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, ui
     a.nodes = d.nodes.as<Node>();
     a.recs = d.recs.as<float4>();
     a.n_nodes = d.n_nodes;
-    fill_transport(a, em.cfg, cfg, emap ? tw : 1, emap ? th : 1, seed, em.reach);
+    fill_transport(a, em.cfg, cfg, emap ? tw : 1, emap ? th : 1, seed, em.reach, E->tuning);
     if (emap) {
         HIP_TRY(E->emap.ensure((size_t)tw * th * sizeof(float)), FG_ERR_BUFFER_RANGE);
         HIP_TRY(hipMemcpyAsync(E->emap.p, emap, (size_t)tw * th * sizeof(float), hipMemcpyHostToDevice,
@@ -361,11 +395,12 @@ static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, ui
     }
     HIP_TRY(E->stats.ensure(8 * sizeof(unsigned long long)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(hipMemsetAsync(E->stats.p, 0, 8 * sizeof(unsigned long long), E->stream), FG_ERR_BUFFER_MAP);
+    a.estride = emap ? tw : 1;
     if (em.n > 0) HIP_TRY(hipMemsetAsync(d.counts.p, 0, (size_t)em.n * 4, E->stream), FG_ERR_BUFFER_MAP);
     a.counts = d.counts.as<uint32_t>();
     a.stats = E->stats.as<unsigned long long>();
     HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
-    HIP_TRY(launch_transport(a, E->stream), FG_ERR_PIPELINE);
+    HIP_TRY(launch_transport_any(E, a, E->stats.as<unsigned long long>() + 6), FG_ERR_PIPELINE);
     HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
     std::vector<uint32_t> c((size_t)em.n);
     unsigned long long st[8];
@@ -731,6 +766,7 @@ struct fg_frame {
     uint32_t last_mode = FG_FRAME_PLAIN;
     BatchedFrame *batched = nullptr;
     bool graph_failed = false;
+    fg_tuning batched_tuning{};
 };
 
 extern "C" int64_t fg_frame_crystal_count(const fg_frame *F) { return F ? F->n_crystals : 0; }
@@ -862,14 +898,15 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
         a.nodes = F->d_nodes.as<Node>() + t.node_base;
         a.recs = reinterpret_cast<const float4 *>(F->d_recs.as<Crystal>() + t.crystal_base);
         a.n_nodes = t.n_nodes;
-        fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach);
+        fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach, E->tuning);
         a.emap = F->d_emap.as<float>() + (size_t)t.y0 * D.width + t.x0;
         a.estride = D.width;
         a.counts = F->d_counts.as<uint32_t>() + t.crystal_base;
         a.stats = F->d_stats.as<unsigned long long>();
         if (t.n) HIP_TRY(hipMemsetAsync(a.counts, 0, (size_t)t.n * 4, E->stream), FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipMemsetAsync(F->d_stats.as<unsigned long long>() + 6, 0, 8, E->stream), FG_ERR_BUFFER_MAP);
         HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
-        HIP_TRY(launch_transport(a, E->stream), FG_ERR_PIPELINE);
+        HIP_TRY(launch_transport_any(E, a, F->d_stats.as<unsigned long long>() + 6), FG_ERR_PIPELINE);
         HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
         c.resize((size_t)t.n);
         if (t.n)
@@ -947,7 +984,12 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
 static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats *st)
 {
     const fg_frame_desc &D = F->desc;
+    if (F->batched && memcmp(&F->batched_tuning, &E->tuning, sizeof(fg_tuning)) != 0) {
+        batched_destroy(F->batched);  // launch parameters are baked into the graph
+        F->batched = nullptr;
+    }
     if (!F->batched) {
+        F->batched_tuning = E->tuning;
         BatchedInput in;
         in.nodes = F->d_nodes.as<Node>();
         in.recs = F->d_recs.as<Crystal>();
@@ -974,7 +1016,7 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
                 in.orig[sl] = ci;
             }
             TransportArgs a{};
-            fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach);
+            fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach, E->tuning);
             if (k == 0) in.transport = a;
             TileDev td{};
             td.x0 = t.x0;

@@ -13,7 +13,7 @@ struct TransportArgs {
     const Node *nodes;
     const float4 *recs;      // 3 float4 per crystal record, leaf order
     uint32_t n_nodes;
-    uint32_t pad0;
+    uint32_t chunk;          // packets a wave takes from the work counter at a time (>= 64)
     const float *emap;       // exposure map (may be null: weight 1)
     uint32_t tw, th, estride;
     uint32_t ppp;            // > 0: stratified, photon p in pixel p / ppp
@@ -30,7 +30,27 @@ struct TransportArgs {
     // misses the segment's bounding box hold no crystal that can absorb in this step.
     // reach < 0 disables the test (full infinite-ray traversal as the reference).
     float reach;
-    float pad1;
+    // persistent transport: node steps continue until this percentage of the lanes
+    // that are walking the BVH hold a pending leaf (100: all of them)
+    uint32_t wait_pct;
+};
+
+// Per-tile descriptor shared by host and device.
+struct TileDev {
+    uint32_t x0, y0, tw, th;        // frame placement (pixels)
+    float W, H, pw, ph;             // extents and pixel size (length units)
+    uint32_t key0, key1;            // transport seed (tile seed + 1)
+    uint32_t skey0, skey1;          // scan seed (tile seed + 2)
+    uint32_t node_base, n_nodes;    // transport BVH nodes
+    uint32_t crystal_base, n;       // crystal slots (leaf order)
+    uint64_t item_base;             // first 64-packet work item
+    uint64_t n_photons;
+    uint64_t photon_base;           // first packet id of the tile in the frame
+    uint32_t block_base, blocks_x;  // scan blocks of 16 x 16 pixels
+    uint32_t blocks, pad0;
+    double maxb;                    // largest semi-axis among the tile's crystals
+    // results of the last render
+    uint32_t n_dev, clumps, max_clump, dev_nodes;
 };
 
 struct ScanArgs {
@@ -48,6 +68,11 @@ struct ScanArgs {
 };
 
 hipError_t launch_transport(const TransportArgs &a, hipStream_t s);
+// Persistent transport of one tile (packets 0 .. a.n_photons - 1); work must point to a
+// zeroed 64-bit counter; grid = resident blocks of the device.
+hipError_t launch_transport_persistent(const TransportArgs &a, unsigned long long *work, int grid,
+                                       hipStream_t s);
+int transport_persistent_blocks_per_cu();
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s);
 hipError_t launch_all_hits(const Node *nodes, uint32_t n_nodes, const float4 *recs,
                            const float *o, const float *d, int64_t n_rays, int32_t max_hits,

@@ -26,6 +26,14 @@ __global__ void __launch_bounds__(256) transport_kernel(TransportArgs A)
     if (A.stats) flush_counters(cnt, A.stats);
 }
 
+// Persistent transport of one tile.
+__global__ void __launch_bounds__(256) transport_persistent_kernel(TransportArgs A, unsigned long long *work)
+{
+    PhotonCounters cnt;
+    transport_persistent<false>(A, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work, A.n_photons, cnt);
+    if (A.stats) flush_counters(cnt, A.stats);
+}
+
 // ------------------------------------------------------------------------ scan ---
 // One thread per output pixel: ss x ss apertures of g x g stratified rays from
 // z = D + 0.1 along -z (transmittance.rs:110-142).  For a ray along -z the
@@ -170,6 +178,23 @@ hipError_t launch_transport(const TransportArgs &a, hipStream_t s)
     const uint64_t blocks = (a.n_photons + 255) / 256;
     hipLaunchKernelGGL(transport_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_transport_persistent(const TransportArgs &a, unsigned long long *work, int grid,
+                                       hipStream_t s)
+{
+    if (a.n_photons == 0) return hipSuccess;
+    hipLaunchKernelGGL(transport_persistent_kernel, dim3(grid), dim3(256), 0, s, a, work);
+    return hipGetLastError();
+}
+
+int transport_persistent_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(transport_persistent_kernel),
+                                                     256, 0) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
 }
 
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s)

@@ -164,6 +164,17 @@ fg_status fg_get_engine(fg_engine **out);
 const char *fg_last_error(void);
 uint32_t fg_abi_version(void);
 
+/* Launch parameters chosen by the autotuner (fgrain_amd/autotune.py); zero fields keep
+ * the defaults.  They change speed only: every setting gives the same results. */
+typedef struct fg_tuning {
+    uint32_t transport_kernel;   /* 0 persistent lanes (default), 1 one thread per packet */
+    uint32_t transport_chunk;    /* packets per work fetch of a wave, >= 64 (default 64) */
+    uint32_t transport_wait_pct; /* BVH steps continue until this % of lanes hold a leaf (80) */
+    uint32_t scan_rays_per_thread; /* reserved (0) */
+} fg_tuning;
+fg_status fg_engine_set_tuning(fg_engine *engine, const fg_tuning *tuning);
+fg_status fg_engine_get_tuning(const fg_engine *engine, fg_tuning *tuning);
+
 /* ---------------------------------------------------------------- emulsion ----- */
 /* VirtualEmulsionLayer::synthesize (emulsion.rs:155-220): Bridson 3D + per-crystal
  * draws + BVH::build.  Host work; deterministic in seed. */

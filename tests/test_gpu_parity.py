@@ -96,6 +96,28 @@ def test_transport_replay_matches_fp64_oracle(engine):
     assert 0.2 * ost["nodes_live"] < st["nodes_visited"] <= ost["nodes_live"] * 1.001
 
 
+@pytest.mark.parametrize("tuning", [dict(transport_kernel=1), dict(transport_wait_pct=100),
+                                    dict(transport_wait_pct=1, transport_chunk=640)])
+def test_transport_kernels_and_tunings_give_identical_counts(engine, tuning):
+    """The persistent lane-regenerating kernel and the one-thread-per-packet kernel, under
+    any tuning, deposit exactly the same latent photons (same draws, same hit order)."""
+    cfg = trix_config(48, 40, r_ahu=0.6)
+    cfg.optics.antihalation_density = 0.1
+    emap = np.linspace(0.01, 4.0, 48, dtype=np.float32)[None, :].repeat(40, 0)
+    ref = F.VirtualEmulsionLayer.synthesize(cfg, 12)
+    engine.set_tuning()
+    _, st0 = _expose_gpu(engine, ref, cfg, emap, ppp=24, seed=3)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 12)
+    try:
+        engine.set_tuning(**tuning)
+        _, st1 = _expose_gpu(engine, em, cfg, emap, ppp=24, seed=3)
+    finally:
+        engine.set_tuning()
+    assert np.array_equal(ref.absorbed_photons(), em.absorbed_photons())
+    assert st0["bounces"] == st1["bounces"] and st0["abs_total"] == st1["abs_total"]
+    assert st0["prims_tested"] == st1["prims_tested"]
+
+
 def test_transport_replay_with_halation(engine):
     cfg = trix_config(48, 48, depth=16.0, r_ahu=0.9)
     cfg.optics.antihalation_density = 0.05
