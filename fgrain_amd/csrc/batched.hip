@@ -499,25 +499,16 @@ __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, 
 {
     const uint32_t t = tile_of_block(B.tiles, n_tiles, blockIdx.x);
     const TileDev &T = B.tiles[t];
-    const uint32_t lb = blockIdx.x - T.block_base;
-    const uint32_t px = (lb % T.blocks_x) * 16 + (threadIdx.x & 15);
-    const uint32_t py = (lb / T.blocks_x) * 16 + (threadIdx.x >> 4);
-    if (px >= T.tw || py >= T.th) return;
+    const uint32_t L = scan_lanes(S.g * S.g * S.ss * S.ss);
+    const uint32_t pix = (blockIdx.x - T.block_base) * (256u / L) + threadIdx.x / L;
+    const bool valid = pix < T.tw * T.th;
+    const uint32_t px = valid ? pix % T.tw : 0u, py = valid ? pix / T.tw : 0u;
     const Node *nodes = B.dnodes + 2ull * T.crystal_base;
     const float4 *recs = reinterpret_cast<const float4 *>(B.drecs + T.crystal_base);
-    const uint32_t nn = B.node_count[t];
-    const uint32_t pix = py * T.tw + px;
-    float acc = 0.0f;
-    for (uint32_t ay = 0; ay < S.ss; ay++)
-        for (uint32_t ax = 0; ax < S.ss; ax++)
-            for (uint32_t sy = 0; sy < S.g; sy++)
-                for (uint32_t sx = 0; sx < S.g; sx++) {
-                    float ox, oy;
-                    scan_origin(pix, px, py, ax, ay, sx, sy, S.g, S.ss, T.pw, T.ph, T.skey0, T.skey1, ox, oy);
-                    acc += expf(-S.sigma * silver_path(nodes, nn, recs, ox, oy, S.z0));
-                }
-    const float Tm = acc / (float)(S.g * S.g * S.ss * S.ss);
-    out[(size_t)(T.y0 + py) * frame_w + T.x0 + px] = scan_value(Tm, S.mode);
+    const float Tm = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, S.g, S.ss, T.pw, T.ph, S.z0,
+                                S.sigma, T.skey0, T.skey1, nodes, B.node_count[t], recs);
+    if (valid && threadIdx.x % L == 0u)
+        out[(size_t)(T.y0 + py) * frame_w + T.x0 + px] = scan_value(Tm, S.mode);
 }
 
 }  // namespace
@@ -605,8 +596,9 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
         items += (T.n_photons + 63) / 64;
         T.photon_base = b->total_photons;
         b->total_photons += T.n_photons;
-        T.blocks_x = (T.tw + 15) / 16;
-        T.blocks = T.blocks_x * ((T.th + 15) / 16);
+        const uint32_t ppb = 256u / scan_lanes(in.scan.g * in.scan.g * in.scan.ss * in.scan.ss);
+        T.blocks_x = 0;
+        T.blocks = (uint32_t)(((uint64_t)T.tw * T.th + ppb - 1) / ppb);
         T.block_base = blocks;
         blocks += T.blocks;
         double mx = 0.0;

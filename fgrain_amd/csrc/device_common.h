@@ -44,6 +44,38 @@ __device__ __forceinline__ Rec load_rec(const float4 *recs, uint32_t k)
     return r;
 }
 
+// Conservative pre-test before crystal_hit: the crystal lies inside the sphere about its
+// centre of radius |h| (box) or max(h) (ellipsoid); when the ray's line passes farther
+// than that radius (plus a margin well above the float rounding of crystal_hit) the
+// exact test would miss too, so skipping it changes no result.  (dx, dy, dz) is a unit
+// vector.
+__device__ __forceinline__ bool sphere_may_hit(const float4 &a, const float4 &c, float ox, float oy,
+                                               float oz, float dx, float dy, float dz)
+{
+    const float vx = a.x - ox, vy = a.y - oy, vz = a.z - oz;
+    const float tc = vx * dx + vy * dy + vz * dz;
+    const float v2 = vx * vx + vy * vy + vz * vz;
+    const float d2 = v2 - tc * tc;
+    const uint32_t morph = __float_as_uint(c.w);
+    const float r2 = morph == 0u ? c.x * c.x + c.y * c.y + c.z * c.z
+                                 : fmaxf(fmaxf(c.x, c.y), c.z) * fmaxf(fmaxf(c.x, c.y), c.z);
+    const float r = sqrtf(r2);
+    const float m = r * 1.001f + 1e-4f * sqrtf(v2) + 1e-4f;
+    return d2 <= m * m;
+}
+
+// Same for a ray along -z through (ox, oy): distance in the x-y plane.
+__device__ __forceinline__ bool sphere_may_hit_z(const float4 &a, const float4 &c, float ox, float oy)
+{
+    const float vx = a.x - ox, vy = a.y - oy;
+    const float d2 = vx * vx + vy * vy;
+    const uint32_t morph = __float_as_uint(c.w);
+    const float r2 = morph == 0u ? c.x * c.x + c.y * c.y + c.z * c.z
+                                 : fmaxf(fmaxf(c.x, c.y), c.z) * fmaxf(fmaxf(c.x, c.y), c.z);
+    const float m = sqrtf(r2) * 1.001f + 1e-4f * (fabsf(vx) + fabsf(vy)) + 1e-4f;
+    return d2 <= m * m;
+}
+
 // Sequential photon stream: draw k is word k & 3 of Philox block k >> 2.
 struct PhotonRng {
     uint32_t p0, p1, k0, k1, blk, rem;
@@ -178,6 +210,7 @@ __device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, P
                 const uint32_t first = nd.prim >> 4, nprim = nd.prim & 15u;
                 for (uint32_t k = first; k < first + nprim; k++) {
                     cnt.prims++;
+                    if (!sphere_may_hit(A.recs[3 * k], A.recs[3 * k + 2], ox, oy, oz, dx, dy, dz)) continue;
                     const Rec r = load_rec(A.recs, k);
                     float t0, t1;
                     if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
@@ -473,6 +506,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             const uint32_t first = leaf >> 4, nprim = leaf & 15u;
             for (uint32_t k = first; k < first + nprim; k++) {
                 cnt.prims++;
+                if (!sphere_may_hit(T.recs[3 * k], T.recs[3 * k + 2], ox, oy, oz, dx, dy, dz)) continue;
                 const Rec r = load_rec(T.recs, k);
                 float t0, t1;
                 if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
@@ -577,6 +611,7 @@ __device__ __forceinline__ float silver_path(const Node *nodes, uint32_t n_nodes
         if (hit && nd.prim != kInterior) {
             const uint32_t first = nd.prim >> 4, cnt = nd.prim & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
+                if (!sphere_may_hit_z(recs[3 * k], recs[3 * k + 2], ox, oy)) continue;
                 const Rec r = load_rec(recs, k);
                 float t0, t1;
                 if (crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, 0.0f, 0.0f, -1.0f, t0, t1))
@@ -606,6 +641,36 @@ __device__ __forceinline__ void scan_origin(uint32_t pix, uint32_t px, uint32_t 
     const float apy = ((float)ay + suby) / (float)ss;
     ox = ((float)px + apx) * pw;
     oy = ((float)py + apy) * ph;
+}
+
+// Lanes per pixel of the scan kernels: all rays of a pixel in adjacent lanes when their
+// number divides the wavefront, else one lane per pixel.
+FG_HD uint32_t scan_lanes(uint32_t rays)
+{
+    return (rays <= 64u && (64u % rays) == 0u) ? rays : 1u;
+}
+
+// Mean transmittance of pixel (px, py) computed by L cooperating lanes (sub = lane index
+// within the pixel's group); the group sum is a fixed xor-tree, so results are
+// reproducible.  Every lane of the wave must call this (invalid lanes pass valid=false).
+__device__ __forceinline__ float scan_pixel(bool valid, uint32_t pix, uint32_t px, uint32_t py,
+                                            uint32_t sub, uint32_t L, uint32_t g, uint32_t ss,
+                                            float pw, float ph, float z0, float sigma, uint32_t key0,
+                                            uint32_t key1, const Node *nodes, uint32_t n_nodes,
+                                            const float4 *recs)
+{
+    const uint32_t R = g * g * ss * ss;
+    float acc = 0.0f;
+    if (valid) {
+        for (uint32_t r = sub; r < R; r += L) {
+            const uint32_t sx = r % g, sy = (r / g) % g, ax = (r / (g * g)) % ss, ay = r / (g * g * ss);
+            float ox, oy;
+            scan_origin(pix, px, py, ax, ay, sx, sy, g, ss, pw, ph, key0, key1, ox, oy);
+            acc += expf(-sigma * silver_path(nodes, n_nodes, recs, ox, oy, z0));
+        }
+    }
+    for (uint32_t off = L >> 1; off >= 1u; off >>= 1) acc += __shfl_xor(acc, (int)off, 64);
+    return acc / (float)R;
 }
 
 // ScanMode mapping (transmittance.rs:144-152) and D_neg (image_pipeline.rs:1006-1011)

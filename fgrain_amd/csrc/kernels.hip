@@ -44,21 +44,13 @@ __global__ void __launch_bounds__(256) transport_persistent_kernel(TransportArgs
 // tested as in the double-precision reference.
 __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
 {
-    const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15);
-    const uint32_t py = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (px >= A.w_px || py >= A.h_px) return;
-    const uint32_t pix = py * A.w_px + px;
-    float acc = 0.0f;
-    for (uint32_t ay = 0; ay < A.ss; ay++)
-        for (uint32_t ax = 0; ax < A.ss; ax++)
-            for (uint32_t sy = 0; sy < A.g; sy++)
-                for (uint32_t sx = 0; sx < A.g; sx++) {
-                    float ox, oy;
-                    scan_origin(pix, px, py, ax, ay, sx, sy, A.g, A.ss, A.pw, A.ph, A.key0, A.key1, ox, oy);
-                    acc += expf(-A.sigma * silver_path(A.nodes, A.n_nodes, A.recs, ox, oy, A.z0));
-                }
-    const float T = acc / (float)(A.g * A.g * A.ss * A.ss);
-    A.out[(size_t)py * A.ostride + px] = scan_value(T, A.mode);
+    const uint32_t L = scan_lanes(A.g * A.g * A.ss * A.ss);
+    const uint32_t pix = blockIdx.x * (256u / L) + threadIdx.x / L;
+    const bool valid = pix < A.w_px * A.h_px;
+    const uint32_t px = valid ? pix % A.w_px : 0u, py = valid ? pix / A.w_px : 0u;
+    const float T = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, A.g, A.ss, A.pw, A.ph, A.z0,
+                               A.sigma, A.key0, A.key1, A.nodes, A.n_nodes, A.recs);
+    if (valid && threadIdx.x % L == 0u) A.out[(size_t)py * A.ostride + px] = scan_value(T, A.mode);
 }
 
 // ------------------------------------------------------- traversal for tests ---
@@ -200,7 +192,9 @@ int transport_persistent_blocks_per_cu()
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s)
 {
     if (a.w_px == 0 || a.h_px == 0) return hipSuccess;
-    hipLaunchKernelGGL(scan_kernel, dim3((a.w_px + 15) / 16, (a.h_px + 15) / 16), dim3(256), 0, s, a);
+    const uint32_t ppb = 256u / scan_lanes(a.g * a.g * a.ss * a.ss);
+    const uint32_t blocks = (uint32_t)(((uint64_t)a.w_px * a.h_px + ppb - 1) / ppb);
+    hipLaunchKernelGGL(scan_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
