@@ -44,27 +44,12 @@ __device__ __forceinline__ Rec load_rec(const float4 *recs, uint32_t k)
     return r;
 }
 
-// Conservative pre-test before crystal_hit: the crystal lies inside the sphere about its
-// centre of radius |h| (box) or max(h) (ellipsoid); when the ray's line passes farther
-// than that radius (plus a margin well above the float rounding of crystal_hit) the
-// exact test would miss too, so skipping it changes no result.  (dx, dy, dz) is a unit
-// vector.
-__device__ __forceinline__ bool sphere_may_hit(const float4 &a, const float4 &c, float ox, float oy,
-                                               float oz, float dx, float dy, float dz)
-{
-    const float vx = a.x - ox, vy = a.y - oy, vz = a.z - oz;
-    const float tc = vx * dx + vy * dy + vz * dz;
-    const float v2 = vx * vx + vy * vy + vz * vz;
-    const float d2 = v2 - tc * tc;
-    const uint32_t morph = __float_as_uint(c.w);
-    const float r2 = morph == 0u ? c.x * c.x + c.y * c.y + c.z * c.z
-                                 : fmaxf(fmaxf(c.x, c.y), c.z) * fmaxf(fmaxf(c.x, c.y), c.z);
-    const float r = sqrtf(r2);
-    const float m = r * 1.001f + 1e-4f * sqrtf(v2) + 1e-4f;
-    return d2 <= m * m;
-}
-
-// Same for a ray along -z through (ox, oy): distance in the x-y plane.
+// Conservative pre-test before crystal_hit for a ray along -z through (ox, oy): the
+// crystal lies inside the sphere about its centre of radius |h| (box) or max(h)
+// (ellipsoid); when the ray passes farther than that radius in the x-y plane (plus a
+// margin well above the float rounding of crystal_hit) the exact test would miss too,
+// so skipping it changes no result.  The rays of one pixel sit in adjacent lanes, so
+// the test mostly rejects for the whole group at once.
 __device__ __forceinline__ bool sphere_may_hit_z(const float4 &a, const float4 &c, float ox, float oy)
 {
     const float vx = a.x - ox, vy = a.y - oy;
@@ -210,7 +195,6 @@ __device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, P
                 const uint32_t first = nd.prim >> 4, nprim = nd.prim & 15u;
                 for (uint32_t k = first; k < first + nprim; k++) {
                     cnt.prims++;
-                    if (!sphere_may_hit(A.recs[3 * k], A.recs[3 * k + 2], ox, oy, oz, dx, dy, dz)) continue;
                     const Rec r = load_rec(A.recs, k);
                     float t0, t1;
                     if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
@@ -506,7 +490,6 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             const uint32_t first = leaf >> 4, nprim = leaf & 15u;
             for (uint32_t k = first; k < first + nprim; k++) {
                 cnt.prims++;
-                if (!sphere_may_hit(T.recs[3 * k], T.recs[3 * k + 2], ox, oy, oz, dx, dy, dz)) continue;
                 const Rec r = load_rec(T.recs, k);
                 float t0, t1;
                 if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
