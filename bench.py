@@ -138,11 +138,17 @@ def main():
         args.workload]["transport_per_path"]
     alg_bytes = per_path["bytes"] * photons
     dev_bytes = 4 * photons + 32 * agg["nodes_visited"] + 48 * agg["prims_tested"] + 8 * agg["abs_events"]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath) and args.mode == "batched":
+        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs,
+        # copy-calibrated (profiles/transport_scan_frame24mp_batched.md)
+        traffic = json.load(open(tpath))["transport_traffic_bytes"]
     roof = None
     if agg["ms_kernel_transport"] > 0:
         ach = alg_bytes * args.steps / (agg["ms_kernel_transport"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
-                "frac": ach / MI355X_HBM_GBPS, "traffic": None,
+                "frac": ach / MI355X_HBM_GBPS, "traffic": traffic,
                 "bytes_per_path": per_path["bytes"],
                 "launches_per_step": 1 if args.mode == "batched" else agg["tiles"]}
     out = {
