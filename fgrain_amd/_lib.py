@@ -42,7 +42,7 @@ class EmulsionConfigC(C.Structure):
 class ExposureConfigC(C.Structure):
     _fields_ = [("photons_per_sq_unit", C.c_double), ("illumination_angle_rad", C.c_double),
                 ("crystal_absorption_coeff", C.c_double), ("max_bounces", C.c_uint32),
-                ("photons_per_pixel", C.c_uint32)]
+                ("photons_per_pixel", C.c_uint32), ("base_thickness", C.c_double)]
 
 
 class ExposeStatsC(C.Structure):

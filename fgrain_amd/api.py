@@ -203,11 +203,12 @@ class ExposureConfig:
     crystal_absorption_coeff: float = 1.2
     max_bounces: int = 64
     photons_per_pixel: int = 0
+    base_thickness: float = 0.0
 
     def c(self):
         return L.ExposureConfigC(self.photons_per_sq_um, self.illumination_angle_rad,
                                  self.crystal_absorption_coeff, self.max_bounces,
-                                 self.photons_per_pixel)
+                                 self.photons_per_pixel, self.base_thickness)
 
 
 @dataclass

@@ -225,6 +225,7 @@ __device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, P
             ox = qx;
             oy = qy;
             oz = A.D - 0.001f;
+            const float inx = dx, iny = dy, inz = dz;
             // anti-halation backing (optics.rs:160-196)
             w *= A.hal_eff;
             if (w < 1e-4f) {
@@ -244,6 +245,12 @@ __device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, P
             }
             // a packet of zero weight deposits nothing from here on
             if (!(w > 0.0f)) return;
+            if (A.base > 0.0f) {
+                if (!(inz > 1e-6f) || !(-dz > 1e-6f)) return;
+                ox += A.base * (inx / inz + dx / (-dz));
+                oy += A.base * (iny / inz + dy / (-dz));
+                if (ox < 0.0f || ox >= A.W || oy < 0.0f || oy >= A.H) return;
+            }
             continue;
         }
         ox = qx;
@@ -520,6 +527,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 ox = qx;
                 oy = qy;
                 oz = C.D - 0.001f;
+                const float inx = dx, iny = dy, inz = dz;
                 phase = kStart;
                 w *= C.hal_eff;
                 if (w < 1e-4f) {
@@ -539,6 +547,15 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                         normalize3(dx, dy, dz);
                     }
                     if (!(w > 0.0f)) phase = kNeed;
+                    if (phase == kStart && C.base > 0.0f) {
+                        if (!(inz > 1e-6f) || !(-dz > 1e-6f)) {
+                            phase = kNeed;
+                        } else {
+                            ox += C.base * (inx / inz + dx / (-dz));
+                            oy += C.base * (iny / inz + dy / (-dz));
+                            if (ox < 0.0f || ox >= T.W || oy < 0.0f || oy >= T.H) phase = kNeed;
+                        }
+                    }
                 }
             } else {
                 ox = qx;

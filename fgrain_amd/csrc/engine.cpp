@@ -338,6 +338,7 @@ static void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const
 {
     a.chunk = tn.transport_chunk;
     a.wait_pct = tn.transport_wait_pct;
+    a.base = (float)x.base_thickness;
     a.reach = g_full_traversal ? -1.0f : (float)reach * 1.0001f;
     const fg_optics &o = ec.optics;
     const double mu_t = o.absorption_coeff + o.scattering_coeff;
@@ -430,6 +431,7 @@ static bool valid_exposure(const fg_exposure_config *c, std::string &msg)
     if (!c) { msg = "null exposure config"; return false; }
     if (!(c->illumination_angle_rad >= 0.0)) { msg = "illumination angle must be >= 0"; return false; }
     if (!(c->photons_per_sq_unit >= 0.0)) { msg = "photon density must be >= 0"; return false; }
+    if (!(c->base_thickness >= 0.0)) { msg = "base thickness must be >= 0"; return false; }
     return true;
 }
 

@@ -33,6 +33,11 @@ struct TransportArgs {
     // persistent transport: node steps continue until this percentage of the lanes
     // that are walking the BVH hold a pending leaf (100: all of them)
     uint32_t wait_pct;
+    // film base below the emulsion (0 = the reference's in-place reflection): a
+    // reflected packet re-enters displaced by base * (tan of the way down + tan of the
+    // way up), and is lost when that leaves the patch
+    float base;
+    float pad2;
 };
 
 // Per-tile descriptor shared by host and device.

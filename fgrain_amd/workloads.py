@@ -40,7 +40,7 @@ def emulsion_config(cfg: dict, width: float, height: float) -> EmulsionConfig:
 def exposure_config(cfg: dict) -> ExposureConfig:
     x = cfg["exposure"]
     return ExposureConfig(0.0, x["illumination_angle_rad"], x["crystal_absorption_coeff"],
-                          x["max_bounces"], x["photons_per_pixel"])
+                          x["max_bounces"], x["photons_per_pixel"], x.get("base_thickness", 0.0))
 
 
 def development_config(cfg: dict) -> DevelopmentConfig:
@@ -93,6 +93,15 @@ def workload_exposure(cfg: dict) -> np.ndarray:
     raise ValueError(x["kind"])
 
 
+def density_multipliers(cfg: dict, n_tiles: int):
+    """Seeded per-tile crystal-density multipliers U(low, high), or None."""
+    dm = cfg.get("density_multiplier")
+    if not dm:
+        return None
+    rng = np.random.default_rng(cfg["seed"] + 1)
+    return rng.uniform(dm["low"], dm["high"], n_tiles).astype(np.float32)
+
+
 class Frame:
     """A device-resident frame of tiles (fg_frame_*): per-tile emulsions synthesised on
     the host once, then rendered any number of times on the GPU."""
@@ -121,6 +130,9 @@ class Frame:
         d.scan = scan_config(cfg, 0, 0).c()
         d.seed = cfg["seed"]
         d.exposure_map = ptr(self.exposure, L._f32p)
+        if density_mult is None:
+            n_tiles = -(-w // fr["tile"]) * -(-h // fr["tile"])
+            density_mult = density_multipliers(cfg, n_tiles)
         self._mult = None if density_mult is None else np.ascontiguousarray(density_mult, np.float32)
         d.density_mult = ptr(self._mult, L._f32p)
         self._h = C.c_void_p()

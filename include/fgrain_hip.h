@@ -74,6 +74,9 @@ typedef struct fg_exposure_config {
     double crystal_absorption_coeff;
     uint32_t max_bounces;
     uint32_t photons_per_pixel;
+    double base_thickness;  /* film base below the emulsion; 0 = reflect in place as the
+                               reference (optics.rs:160-196); > 0: a reflected packet
+                               re-enters displaced by base * (tan in + tan out) */
 } fg_exposure_config;
 
 typedef struct fg_expose_stats {

@@ -773,7 +773,20 @@ static void run_photon(const texp *X, orng *rng, double x, double y, orc_expose_
             p[0] = q[0];
             p[1] = q[1];
             p[2] = c->D - 0.001;
+            const double din[3] = {d[0], d[1], d[2]};
             if (!antihalation(X, &w, d, rng)) return;
+            /* zero weight deposits nothing more (the reference keeps walking it) */
+            if (!(w > 0.0)) {
+                if (c->base_thickness > 0.0) return;
+                continue;
+            }
+            if (c->base_thickness > 0.0) {
+                /* extension: re-entry displaced by the path through the film base */
+                if (!(din[2] > 1e-6) || !(-d[2] > 1e-6)) return;
+                p[0] += c->base_thickness * (din[0] / din[2] + d[0] / (-d[2]));
+                p[1] += c->base_thickness * (din[1] / din[2] + d[1] / (-d[2]));
+                if (p[0] < 0.0 || p[0] >= c->W || p[1] < 0.0 || p[1] >= c->H) return;
+            }
             continue;
         }
         p[0] = q[0];

@@ -59,6 +59,7 @@ typedef struct {
     uint64_t seed;
     int32_t tw, th;                 /* exposure map size */
     int32_t pad_;
+    double base_thickness;          /* extension: film base below the emulsion (0 = reference) */
 } orc_expose_cfg;
 
 typedef struct {

@@ -46,7 +46,7 @@ class OrcExposeCfg(C.Structure):
                 ("photons_per_area", C.c_double), ("max_bounces", C.c_int32),
                 ("sampler", C.c_int32), ("n_photons", C.c_int64), ("ppp", C.c_int32),
                 ("threads", C.c_int32), ("seed", C.c_uint64), ("tw", C.c_int32),
-                ("th", C.c_int32), ("pad_", C.c_int32)]
+                ("th", C.c_int32), ("pad_", C.c_int32), ("base_thickness", C.c_double)]
 
 
 class OrcExposeStats(C.Structure):
@@ -222,7 +222,7 @@ def query_radius(cs: Crystals, center, r, cap=4096):
 
 def expose(cs: Crystals, exposure_map: np.ndarray | None, *, W, H, D, mu_a, mu_s, g, n_refr,
            r_ahu, od_ahu, mu_abs, theta_max, max_bounces, seed, sampler=1, ppp=0,
-           n_photons=0, photons_per_area=0.0, threads=0):
+           n_photons=0, photons_per_area=0.0, threads=0, base_thickness=0.0):
     """expose_emulsion -> (per-crystal counts u32, stats dict)."""
     if exposure_map is not None:
         em = np.ascontiguousarray(exposure_map, np.float32)
@@ -231,7 +231,7 @@ def expose(cs: Crystals, exposure_map: np.ndarray | None, *, W, H, D, mu_a, mu_s
         em, th, tw = None, 1, 1
     cfg = OrcExposeCfg(W, H, D, mu_a, mu_s, g, n_refr, r_ahu, od_ahu, mu_abs, theta_max,
                        photons_per_area, max_bounces, sampler, n_photons, ppp, threads, seed,
-                       tw, th, 0)
+                       tw, th, 0, base_thickness)
     counts = np.zeros(max(cs.n, 1), np.uint32)
     st = OrcExposeStats()
     cc = cs.c()

@@ -130,6 +130,29 @@ def test_transport_replay_with_halation(engine):
     assert np.mean(g == oc) >= 0.97
 
 
+def test_transport_replay_tabular_with_film_base(engine):
+    """Tabular (ellipsoid) crystals, halation on below a film base (the lateral offset
+    extension, also in the oracle): replayed streams agree."""
+    cfg = trix_config(64, 64, depth=19.2, r_ahu=0.6, morph=1, aspect=8.0, mean=0.9, r_min=2.16)
+    cfg.optics.antihalation_density = 0.0
+    cfg.min_half_thickness = 0.05
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 31)
+    emap = np.full((64, 64), 2.0, np.float32)
+    x = F.ExposureConfig(0.0, 0.12, 0.9, 48, 16, base_thickness=10.0)
+    st = F.expose_emulsion(em, emap, x, 8, engine)
+    g = em.absorbed_photons().astype(np.int64)
+    cs = oracle_crystals(em)
+    oc, ost = O.expose(cs, emap, **expose_kwargs(cfg), seed=8, sampler=1, ppp=16, base_thickness=10.0)
+    assert ost["abs_total"] > 0
+    assert abs(g.sum() - oc.astype(np.int64).sum()) <= 1e-3 * oc.sum()
+    assert np.mean(g == oc) >= 0.97
+    assert st["bounces"] == pytest.approx(ost["bounces_live"], rel=2e-3)
+    # the base offset changes the result (it is not silently ignored)
+    em2 = F.VirtualEmulsionLayer.synthesize(cfg, 31)
+    F.expose_emulsion(em2, emap, F.ExposureConfig(0.0, 0.12, 0.9, 48, 16), 8, engine)
+    assert not np.array_equal(em2.absorbed_photons(), em.absorbed_photons())
+
+
 def test_transport_statistics_vs_reference_sampler(engine):
     """Independent streams: GPU (stratified photon-keyed Philox) against the reference
     algorithm's chunked uniform sampler (transport.rs:65-74).  Per-crystal means agree
