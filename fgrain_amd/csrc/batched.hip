@@ -42,7 +42,7 @@ struct BNode {
 struct Bufs {
     // per slot
     double *pos64, *dims64, *mdim, *fe, *cen;
-    uint32_t *orig, *tile_of, *devflag, *parent, *csize, *root_flag, *prefix, *devpos;
+    uint32_t *orig, *tile_of, *devflag, *parent, *root, *csize, *root_flag, *prefix, *devpos;
     int32_t *clump;
     uint32_t *elem[2], *final_elem;
     float *elo, *ehi;
@@ -194,12 +194,15 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, uint64
     }
 }
 
+// Each developed crystal's root goes to its own array: other threads of this kernel
+// still halve paths through `parent`, so a root stored there could be overwritten by
+// an ancestor.
 __global__ void __launch_bounds__(256) k_roots(Bufs B, uint64_t n)
 {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n || !B.devflag[s]) return;
     const uint32_t r = uf_find(B.parent, (uint32_t)s);
-    B.parent[s] = r;
+    B.root[s] = r;
     atomicAdd(&B.csize[r], 1u);
     if (r == s) B.root_flag[B.tiles[B.tile_of[s]].crystal_base + B.orig[s]] = 1u;
 }
@@ -249,7 +252,7 @@ __global__ void __launch_bounds__(256) k_boost(Bufs B, uint64_t n, DevelopParams
     if (s >= n || !B.devflag[s]) return;
     const uint32_t t = B.tile_of[s];
     TileDev &T = B.tiles[t];
-    const uint32_t r = B.parent[s];
+    const uint32_t r = B.root[s];
     const uint32_t size = B.csize[r];
     double fe = P.fe;
     if (size > 1u) fe *= 1.15 + 0.18 * (double)(min(size, 6u) - 1u);
@@ -563,6 +566,7 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.tile_of, n));
     TRY(alloc(b, B.devflag, n));
     TRY(alloc(b, B.parent, n));
+    TRY(alloc(b, B.root, n));
     TRY(alloc(b, B.csize, n));
     TRY(alloc(b, B.root_flag, n));
     TRY(alloc(b, B.prefix, n));

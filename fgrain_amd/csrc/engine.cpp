@@ -453,10 +453,14 @@ extern "C" fg_status fg_expose(fg_engine *E, fg_emulsion *em, const float *emap,
 extern "C" fg_status fg_develop(fg_engine *E, fg_emulsion *em, const fg_development_config *cfg,
                                 fg_development_report *report)
 {
-    if (!E || !em || !cfg) return fail(FG_ERR_INVALID_ARG, "fg_develop: null argument");
-    std::lock_guard<std::mutex> lk(E->mu);
+    if (!em || !cfg) return fail(FG_ERR_INVALID_ARG, "fg_develop: null argument");
     fg_development_report rep;
-    develop_host(em->em, *cfg, rep);
+    if (E) {
+        std::lock_guard<std::mutex> lk(E->mu);
+        develop_host(em->em, *cfg, rep);
+    } else {
+        develop_host(em->em, *cfg, rep);  // host work; no engine needed
+    }
     if (report) *report = rep;
     return FG_OK;
 }
