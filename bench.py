@@ -134,8 +134,11 @@ def main():
     # the reference algorithm (oracle count over sampled tiles of this workload: 4 B
     # exposure + 32 B per BVH node + 48 B per crystal record + 8 B per absorbing hit on
     # the full infinite-ray traversal) x the paths one launch processes.
-    per_path = json.load(open(os.path.join(ROOT, "tests", "golden", "roofline_bytes.json")))[
-        args.workload]["transport_per_path"]
+    per_path = json.load(open(os.path.join(ROOT, "tests", "golden", "roofline_bytes.json"))).get(
+        args.workload, {}).get("transport_per_path")
+    if per_path is None:
+        raise SystemExit(f"no algorithmic byte count for {args.workload}: "
+                         "run tests/golden/make_roofline_bytes.py {args.workload}")
     alg_bytes = per_path["bytes"] * photons
     dev_bytes = 4 * photons + 32 * agg["nodes_visited"] + 48 * agg["prims_tested"] + 8 * agg["abs_events"]
     traffic = None

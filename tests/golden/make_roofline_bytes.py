@@ -25,13 +25,16 @@ def main(name="frame24mp_trix_cubic", tiles=(0, 9, 18, 125, 133, 228, 246), ppp=
     exp = W.workload_exposure(cfg)
     fr = cfg["frame"]
     nx = -(-fr["width"] // fr["tile"])
+    ny = -(-fr["height"] // fr["tile"])
+    mult = W.density_multipliers(cfg, nx * ny)
     tot = dict(photons=0, nodes=0, prims=0, abs=0)
     for idx in tiles:
         tx, ty = idx % nx, idx // nx
         x0, y0 = tx * fr["tile"], ty * fr["tile"]
         tw, th = min(fr["tile"], fr["width"] - x0), min(fr["tile"], fr["height"] - y0)
         seed = (cfg["seed"] + idx * 73856093 + tx * 19349663 + ty * 83492791) % (1 << 64)
-        cs = O.synthesize(W=tw, H=th, D=e["depth"], r_min=e["r_min"], mean_radius=e["mean_crystal_radius"],
+        r_min = e["r_min"] / (float(mult[idx]) ** (1.0 / 3.0) if mult is not None else 1.0)
+        cs = O.synthesize(W=tw, H=th, D=e["depth"], r_min=r_min, mean_radius=e["mean_crystal_radius"],
                           size_sigma=e["size_variance"], morph=e["morphology"], aspect=e["tabular_aspect_ratio"],
                           base_sens=e["base_sensitivity"], sens_sigma=e["sensitivity_sigma"],
                           min_radius=e["min_radius"], min_half_thickness=e["min_half_thickness"], seed=seed)
@@ -40,7 +43,7 @@ def main(name="frame24mp_trix_cubic", tiles=(0, 9, 18, 125, 133, 228, 246), ppp=
                          n_refr=o["refractive_index"], r_ahu=o["antihalation_reflectance"],
                          od_ahu=o["antihalation_density"], mu_abs=x["crystal_absorption_coeff"],
                          theta_max=x["illumination_angle_rad"], max_bounces=x["max_bounces"],
-                         seed=seed + 1, sampler=1, ppp=ppp)
+                         seed=seed + 1, sampler=1, ppp=ppp, base_thickness=x.get("base_thickness", 0.0))
         tot["photons"] += st["launched"]
         tot["nodes"] += st["nodes_tested"]
         tot["prims"] += st["prims_tested"]
