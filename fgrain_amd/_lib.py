@@ -111,6 +111,23 @@ class FrameStatsC(C.Structure):
                                  "ms_kernel_transport", "ms_kernel_scan")]
 
 
+class ExpectedModelC(C.Structure):
+    _fields_ = [("exposure", ExposureConfigC), ("optics", Optics),
+                ("development", DevelopmentConfigC), ("scan", ScanConfigC),
+                ("expose_seed", C.c_uint64), ("scan_seed", C.c_uint64),
+                ("weight_cutoff", C.c_double)]
+
+
+class ParamGradsC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "crystal_absorption_coeff", "anisotropy_g", "antihalation_reflectance", "reserved_")]
+
+
+class AdjointStatsC(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("packets", "bounces", "events")] + [
+        (n, C.c_float) for n in ("ms_replay", "ms_scan", "ms_total", "pad_")]
+
+
 # every entry point of include/fgrain_hip.h: name -> (restype, argtypes)
 _vp = C.c_void_p
 _st = C.c_int
@@ -151,6 +168,11 @@ SIGNATURES = {
     "fg_frame_read_counts": (_st, [_vp, _vp, C.c_int64, _u32p]),
     "fg_frame_read_development": (_st, [_vp, _vp, C.c_int64, _u8p, _f64p, _i64p]),
     "fg_frame_crystal_count": (C.c_int64, [_vp]),
+    "fg_expected_forward": (_st, [_vp, _vp, C.POINTER(ExpectedModelC), _f32p, C.c_uint32,
+                                  C.c_uint32, _f32p, _f32p, _f32p, C.POINTER(AdjointStatsC)]),
+    "fg_vjp": (_st, [_vp, _vp, C.POINTER(ExpectedModelC), _f32p, C.c_uint32, C.c_uint32,
+                     _f32p, _f32p, _f32p, C.POINTER(ParamGradsC), _f32p,
+                     C.POINTER(AdjointStatsC)]),
 }
 
 _lib = None

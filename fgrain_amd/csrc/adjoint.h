@@ -1,0 +1,58 @@
+// Launch interfaces of the expected-model kernels (adjoint.hip): path replay of the
+// photon streams for the expected dose and for its gradients, development
+// probability, expected scan and its adjoint.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace fg {
+
+struct ReplayArgs {
+    TransportArgs t;       // geometry, optics, exposure map, keys (t.counts unused)
+    double *dose;          // dose pass: expected dose accumulators, transport slot order
+    const float *gdose;    // gradient pass: dL/d(dose) per transport slot
+    double *gmap;          // gradient pass: dL/dE accumulators (tw * th)
+    double *partials;      // gradient pass: per block (mu_abs, g, R) sums, 4 doubles each
+    unsigned long long *stats;  // packets, bounces, events
+    float hal_unit;        // 10^(-2 OD): t.hal_eff = R * hal_unit
+    float refl;            // antihalation reflectance R
+    float cut;             // a packet ends once its relative weight and R tangent are below
+    float pad;
+};
+
+// Expected-scan parameters: the BVH holds every crystal with semi-axes scaled by the
+// filament expansion; the sens word of each record holds the crystal's transport slot.
+struct ExpScanArgs {
+    const Node *nodes;
+    const float4 *recs;
+    uint32_t n_nodes;
+    uint32_t w_px, h_px, g, ss, mode;
+    float pw, ph, z0, sigma;
+    uint32_t key0, key1;
+    const float *prob;     // development probability per transport slot
+    float *out;            // forward: w_px * h_px
+    const float *gout;     // adjoint: dL/d(out)
+    double *gprob;         // adjoint: dL/d(prob) accumulators per transport slot
+};
+
+int replay_blocks(uint64_t n_photons);
+hipError_t launch_replay_dose(const ReplayArgs &a, hipStream_t s);
+hipError_t launch_replay_grad(const ReplayArgs &a, hipStream_t s);
+// prob = P[Poisson(dose) >= thr] and dprob/ddose, from the float dose
+hipError_t launch_dev_prob(const double *dose64, float *dose32, float *prob, float *dprob,
+                           uint32_t n, uint32_t thr, hipStream_t s);
+hipError_t launch_dev_prob_from(const float *dose32, float *prob, float *dprob, uint32_t n,
+                                uint32_t thr, hipStream_t s);
+hipError_t launch_expected_scan(const ExpScanArgs &a, hipStream_t s);
+hipError_t launch_expected_scan_adjoint(const ExpScanArgs &a, hipStream_t s);
+// gdose = gprob * dprob (float)
+hipError_t launch_dose_grad(const double *gprob, const float *dprob, float *gdose, uint32_t n,
+                            hipStream_t s);
+// out[j] = sum over blocks of partials[4 * b + j], j < 4, in a fixed order
+hipError_t launch_reduce_partials(const double *partials, uint32_t blocks, double *out,
+                                  hipStream_t s);
+hipError_t launch_f64_to_f32(const double *in, float *out, uint32_t n, hipStream_t s);
+
+}  // namespace fg

@@ -13,78 +13,18 @@
 #include <vector>
 
 #include "batched.h"
-#include "internal.h"
-#include "kernels.h"
+#include "engine_state.h"
 
 using namespace fg;
 
 // ------------------------------------------------------------------ errors -------
 static thread_local std::string g_last_error;
 
-static fg_status fail(fg_status s, const std::string &msg)
+fg_status fail(fg_status s, const std::string &msg)
 {
     g_last_error = msg;
     return s;
 }
-
-#define HIP_TRY(call, status)                                                          \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(status, std::string(#call) + ": " + hipGetErrorString(e_));    \
-    } while (0)
-
-// Grow-only device buffer.
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes)
-    {
-        if (bytes <= cap && p) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            if (e != hipSuccess) return e;
-            p = nullptr;
-            cap = 0;
-        }
-        const size_t b = std::max<size_t>(bytes, 256);
-        hipError_t e = hipMalloc(&p, b);
-        if (e == hipSuccess) cap = b;
-        return e;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
-};
-
-namespace fg {
-struct DeviceEmulsion {
-    DevBuf nodes, recs, counts;
-    uint32_t n_nodes = 0;
-    int64_t n = 0;
-};
-}  // namespace fg
-
-struct fg_engine {
-    int device = 0;
-    int cus = 0;
-    int persistent_grid = 0;
-    fg_tuning tuning{0u, 64u, 80u, 0u};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string name;
-    std::mutex mu;
-    DevBuf emap, stats, scan_nodes, scan_recs, out, out2, rays_o, rays_d, hits_n, hits_i,
-        hits_t0, hits_t1;
-};
-
-struct fg_emulsion {
-    Emulsion em;
-};
 
 // ------------------------------------------------------------------ engine -------
 extern "C" uint32_t fg_abi_version(void) { return FG_ABI_VERSION; }
@@ -108,6 +48,8 @@ extern "C" fg_status fg_engine_create(fg_engine **out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&E->ev0);
     if (e == hipSuccess) e = hipEventCreate(&E->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev2);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev3);
     if (e != hipSuccess) {
         delete E;
         return fail(FG_ERR_DEVICE_REQUEST, std::string("HIP device setup: ") + hipGetErrorString(e));
@@ -124,10 +66,14 @@ extern "C" void fg_engine_destroy(fg_engine *E)
 {
     if (!E) return;
     for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->out, &E->out2,
-                      &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1})
+                      &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
+                      &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
+                      &E->gmap, &E->partials, &E->pgrad, &E->gout})
         b->release();
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
+    if (E->ev2) (void)hipEventDestroy(E->ev2);
+    if (E->ev3) (void)hipEventDestroy(E->ev3);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     delete E;
 }
@@ -234,6 +180,8 @@ static void free_device(Emulsion &em)
         em.dev->nodes.release();
         em.dev->recs.release();
         em.dev->counts.release();
+        em.dev->exp_nodes.release();
+        em.dev->exp_recs.release();
         delete em.dev;
         em.dev = nullptr;
     }
@@ -312,7 +260,7 @@ extern "C" fg_status fg_emulsion_set_state(fg_emulsion *E, const uint32_t *count
 }
 
 // ------------------------------------------------------------- device helpers ---
-static fg_status ensure_device(fg_engine *E, Emulsion &em)
+fg_status ensure_device(fg_engine *E, Emulsion &em)
 {
     if (em.dev) return FG_OK;
     em.dev = new DeviceEmulsion();
@@ -333,8 +281,8 @@ static fg_status ensure_device(fg_engine *E, Emulsion &em)
 
 static bool g_full_traversal = getenv("FG_TRANSPORT_FULL_TRAVERSAL") != nullptr;
 
-static void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const fg_exposure_config &x,
-                           uint32_t tw, uint32_t th, uint64_t seed, double reach, const fg_tuning &tn)
+void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const fg_exposure_config &x,
+                    uint32_t tw, uint32_t th, uint64_t seed, double reach, const fg_tuning &tn)
 {
     a.chunk = tn.transport_chunk;
     a.wait_pct = tn.transport_wait_pct;
@@ -426,7 +374,7 @@ static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, ui
     return FG_OK;
 }
 
-static bool valid_exposure(const fg_exposure_config *c, std::string &msg)
+bool valid_exposure(const fg_exposure_config *c, std::string &msg)
 {
     if (!c) { msg = "null exposure config"; return false; }
     if (!(c->illumination_angle_rad >= 0.0)) { msg = "illumination angle must be >= 0"; return false; }
@@ -483,7 +431,7 @@ static bool build_scan_bvh(const Emulsion &em, std::vector<Node> &nodes, std::ve
     return true;
 }
 
-static bool valid_scan(const fg_scan_config *c, std::string &msg)
+bool valid_scan(const fg_scan_config *c, std::string &msg)
 {
     if (!c) { msg = "null scan config"; return false; }
     if (c->mode > 3) { msg = "scan mode must be 0..3"; return false; }

@@ -1,0 +1,86 @@
+// Engine and emulsion state shared by the C-ABI translation units (engine.cpp,
+// adjoint_host.cpp); not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "internal.h"
+#include "kernels.h"
+
+// Sets the thread-local message fg_last_error returns and passes the status through.
+fg_status fail(fg_status s, const std::string &msg);
+
+#define HIP_TRY(call, status)                                                          \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(status, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        const size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) cap = b;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+namespace fg {
+struct DeviceEmulsion {
+    DevBuf nodes, recs, counts;
+    uint32_t n_nodes = 0;
+    int64_t n = 0;
+    // expected-model scan BVH: every crystal, semi-axes times exp_fe, record sens words
+    // holding transport slots (built on first use for a given expansion)
+    DevBuf exp_nodes, exp_recs;
+    uint32_t exp_n_nodes = 0;
+    double exp_fe = -1.0;
+};
+}  // namespace fg
+
+struct fg_engine {
+    int device = 0;
+    int cus = 0;
+    int persistent_grid = 0;
+    fg_tuning tuning{0u, 64u, 80u, 0u};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    std::string name;
+    std::mutex mu;
+    DevBuf emap, stats, scan_nodes, scan_recs, out, out2, rays_o, rays_d, hits_n, hits_i,
+        hits_t0, hits_t1;
+    // expected model and its adjoint
+    DevBuf dose64, dose32, prob, dprob, gprob, gdose, gmap, partials, pgrad, gout;
+};
+
+struct fg_emulsion {
+    fg::Emulsion em;
+};
+
+fg_status ensure_device(fg_engine *E, fg::Emulsion &em);
+void fill_transport(fg::TransportArgs &a, const fg_emulsion_config &ec, const fg_exposure_config &x,
+                    uint32_t tw, uint32_t th, uint64_t seed, double reach, const fg_tuning &tn);
+bool valid_exposure(const fg_exposure_config *c, std::string &msg);
+bool valid_scan(const fg_scan_config *c, std::string &msg);

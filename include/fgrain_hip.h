@@ -289,6 +289,62 @@ fg_status fg_frame_read_development(fg_engine *engine, const fg_frame *frame, in
                                     uint8_t *developed, double *fe, int64_t *clump_id);
 int64_t fg_frame_crystal_count(const fg_frame *frame);
 
+/* ---- differentiable (expected) model and its vector-Jacobian product ------------
+ * No counterpart in the reference: the reference path is not differentiable (integer
+ * latent counts, a Boolean development mask).  The expected model replaces each by its
+ * expectation over the packet's stochastic rounding and the development threshold:
+ *   dose_i  = sum over replayed packets and their hits on crystal i of
+ *             5 * E(px) * w * (1 - exp(-mu_abs * L)) * sens_i
+ *             (the mean of the latent count of transport.rs:117-134; packets replay
+ *             the photon stream of fg_expose's stratified sampler without the rounding
+ *             and roulette draws, and end once their relative weight falls below
+ *             weight_cutoff);
+ *   prob_i  = P[Poisson(dose_i) >= threshold]   (development.rs:69-84 in expectation);
+ *   T(ray)  = prod over crystals hit, geometry scaled by the unboosted filament
+ *             expansion, of 1 - prob_i * (1 - exp(-sigma * chord_i));
+ *   out(px) = scan mode map of the mean T over the pixel's rays (transmittance.rs:
+ *             137-152, image_pipeline.rs:1006-1011), rays as fg_scan's.
+ * Gradients: exposure map and mu_abs, antihalation reflectance by path replay with
+ * forward tangents of the packet weight; HG anisotropy g by the score function of the
+ * Henyey-Greenstein density (the packet's path is held fixed). */
+typedef struct fg_expected_model {
+    fg_exposure_config exposure;       /* photons_per_pixel > 0; crystal_absorption_coeff */
+    fg_optics optics;                  /* anisotropy_g, antihalation_reflectance */
+    fg_development_config development; /* threshold_photons, filament_expansion */
+    fg_scan_config scan;               /* otf must be 0 */
+    uint64_t expose_seed, scan_seed;
+    double weight_cutoff;              /* 0 = 1e-6 */
+} fg_expected_model;
+
+typedef struct fg_param_grads {
+    double crystal_absorption_coeff;
+    double anisotropy_g;
+    double antihalation_reflectance;
+    double reserved_;
+} fg_param_grads;
+
+typedef struct fg_adjoint_stats {
+    uint64_t packets, bounces, events;
+    float ms_replay;    /* path-replay kernel (dose or gradient pass) */
+    float ms_scan;      /* expected scan or its adjoint */
+    float ms_total;     /* the whole call, host included */
+    float pad_;
+} fg_adjoint_stats;
+
+/* Expected-model forward on one tile: dose and prob per crystal (crystal order, each
+ * optional) and out (scan.width_px * scan.height_px).  dose is what fg_vjp takes back. */
+fg_status fg_expected_forward(fg_engine *engine, const fg_emulsion *em,
+                              const fg_expected_model *model, const float *exposure_map,
+                              uint32_t tw, uint32_t th, float *dose, float *prob, float *out,
+                              fg_adjoint_stats *stats);
+/* Vector-Jacobian product of fg_expected_forward: given grad_out = dL/d(out), writes
+ * dL/d(exposure map) (tw * th), the parameter gradients and, optionally, dL/d(dose) per
+ * crystal.  dose: the forward's output for the same inputs, or NULL to recompute it. */
+fg_status fg_vjp(fg_engine *engine, const fg_emulsion *em, const fg_expected_model *model,
+                 const float *exposure_map, uint32_t tw, uint32_t th, const float *dose,
+                 const float *grad_out, float *grad_exposure, fg_param_grads *grads,
+                 float *grad_dose, fg_adjoint_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

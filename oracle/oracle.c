@@ -656,8 +656,8 @@ static inline void tr_prim(uint32_t id, hitctx *h)
     h->w *= fmax(1.0 - pa * 0.9, 0.05);
 }
 
-/* optics.rs:113-143 */
-static void sample_hg(double d[3], double g, orng *rng)
+/* optics.rs:113-143; returns the sampled cosine */
+static double sample_hg(double d[3], double g, orng *rng)
 {
     const double xi1 = rng_u(rng), xi2 = rng_u(rng);
     double ct;
@@ -688,6 +688,7 @@ static void sample_hg(double d[3], double g, orng *rng)
     const double a = stt * cos(phi), bb = stt * sin(phi);
     for (int k = 0; k < 3; k++) d[k] = (u[k] * a + v[k] * bb) + w[k] * ct;
     normalize3(d);
+    return ct;
 }
 
 /* optics.rs:160-196; returns 0 when the packet dies */
@@ -1404,4 +1405,457 @@ int orc_antihalation(double r_ahu, double od, uint64_t seed, double *energy, dou
     orng rng;
     rng_seq(&rng, seed);
     return antihalation(&X, energy, d, &rng);
+}
+
+/* =====================================================================================
+ * Expected (differentiable) model -- no reference counterpart.  FP64 twin of the
+ * library's path replay: the packets of the photon-keyed sampler (sampler 1, ppp > 0)
+ * without the stochastic-rounding and roulette draws; every hit deposits its mean
+ * latent count 5 * E * w * pa * sens (transport.rs:117-134); a packet ends when it
+ * leaves the slab, after max_bounces, or once its relative weight w and the weight's
+ * reflectance tangent are both below the cutoff.  Gradients: forward tangents of w for
+ * mu_abs and the antihalation reflectance R, the Henyey-Greenstein score for g.
+ * ===================================================================================== */
+typedef struct {
+    const obvh *b;
+    const ocrys *cr;
+    const orc_expected_cfg *e;
+    const float *map;
+    const double *gdose;    /* NULL: dose pass */
+    double mu_t, albedo, r0, hal_unit, hal_eff, cut;
+    int64_t n_total, grain, next;
+    int64_t n;              /* crystals */
+} trep;
+
+typedef struct {
+    double *dose;           /* dose pass: n */
+    double *gmap;           /* gradient pass: tw * th */
+    double gmu, gg, gr;
+    int64_t bounces, events;
+} trep_acc;
+
+typedef struct {
+    const trep *X;
+    trep_acc *A;
+    const double *o, *d;
+    double inv[3];
+    double step, E;
+    double w, w_mu, w_r, score;
+    double a_e, a_mu, a_g, a_r;
+} rephit;
+
+static inline int rp_box(const onode *nd, rephit *h)
+{
+    return box_hit_d(nd->mn, nd->mx, h->o, h->inv);
+}
+
+static inline void rp_prim(uint32_t id, rephit *h)
+{
+    const ocrys *r = &h->X->cr[id];
+    double t0, t1;
+    if (!crystal_intersect_d(r->c, r->q, r->dims, r->morph, h->o, h->d, &t0, &t1)) return;
+    if (!(t0 < h->step)) return;
+    const double L = fmax(fmin(t1, h->step) - t0, 0.0);
+    if (!(L > 0.0)) return;
+    h->A->events++;
+    const double pa = 1.0 - exp(-h->X->e->x.mu_abs * L);
+    if (h->X->gdose) {
+        const double base = 5.0 * r->sens * h->X->gdose[id];
+        const double dep = h->w * pa;
+        h->a_e += base * dep;
+        h->a_mu += base * h->E * (h->w_mu * pa + h->w * L * (1.0 - pa));
+        h->a_r += base * h->E * (h->w_r * pa);
+        h->a_g += base * h->E * dep * h->score;
+    } else {
+        h->A->dose[id] += h->E * h->w * pa * r->sens * 5.0;
+    }
+    const double m = 1.0 - 0.9 * pa, dm = -0.9 * L * (1.0 - pa);
+    h->w_mu = h->w_mu * m + h->w * dm;
+    h->w_r *= m;
+    h->w *= m;
+}
+
+/* d/dg log p_g(mu) of the Henyey-Greenstein density of mu = cos(theta) */
+double orc_hg_score(double g, double mu)
+{
+    return -2.0 * g / (1.0 - g * g) + 3.0 * (mu - g) / (1.0 + g * g - 2.0 * g * mu);
+}
+
+static void replay_packet(const trep *X, trep_acc *A, int64_t p)
+{
+    const orc_expose_cfg *c = &X->e->x;
+    orng rng;
+    rng_photon(&rng, c->seed, (uint64_t)p);
+    const int64_t pix = p / c->ppp;
+    const int64_t px = pix % c->tw, py = pix / c->tw;
+    const float ux = (float)rng_u(&rng), uy = (float)rng_u(&rng);
+    const float pw = (float)(c->W / (double)c->tw), ph = (float)(c->H / (double)c->th);
+    const double x = (double)(((float)px + ux) * pw), y = (double)(((float)py + uy) * ph);
+    /* nearest-pixel lookup (image_pipeline.rs:949-961) */
+    const double u = fmin(fmax(x / c->W, 0.0), 1.0), v = fmin(fmax(y / c->H, 0.0), 1.0);
+    int64_t ex = (int64_t)round(u * (double)(c->tw - 1)), ey = (int64_t)round(v * (double)(c->th - 1));
+    if (ex > c->tw - 1) ex = c->tw - 1;
+    if (ey > c->th - 1) ey = c->th - 1;
+    const int64_t e_idx = ey * c->tw + ex;
+    const double E = fmax((double)X->map[e_idx], 0.0);
+    if (!(E > 0.0)) return;
+    const double th = rng_u(&rng) * c->theta_max;
+    const double phi = rng_u(&rng) * TAU;
+    const double s_t = sin(th);
+    double d[3] = {s_t * cos(phi), s_t * sin(phi), cos(th)};
+    normalize3(d);
+    {
+        const double cz = fmin(fmax(fabs(d[2]), 0.0), 1.0), m = 1.0 - cz;
+        if (rng_u(&rng) < X->r0 + (1.0 - X->r0) * (m * m * m * m * m)) return;
+    }
+    double o[3] = {x, y, 0.001};
+    rephit h;
+    memset(&h, 0, sizeof h);
+    h.X = X;
+    h.A = A;
+    h.o = o;
+    h.d = d;
+    h.E = E;
+    h.w = 1.0;
+    for (int b = 0; b < c->max_bounces; b++) {
+        A->bounces++;
+        const double s = X->mu_t <= 0.0 ? 1e6 : -log(fmax(1.0 - rng_u(&rng), DBL_MIN)) / X->mu_t;
+        h.step = fmin(s, c->D * 1.5);
+        for (int a = 0; a < 3; a++) h.inv[a] = 1.0 / d[a];
+        tcount cnt = {0, 0};
+        if (X->b->nn > 0) DFS_TRAVERSE(X->b, rp_box, rp_prim, &h, &cnt);
+        const double q[3] = {o[0] + d[0] * h.step, o[1] + d[1] * h.step, o[2] + d[2] * h.step};
+        if (q[0] < 0.0 || q[0] >= c->W || q[1] < 0.0 || q[1] >= c->H) break;
+        if (q[2] <= 0.0) break;
+        if (q[2] >= c->D) {
+            o[0] = q[0];
+            o[1] = q[1];
+            o[2] = c->D - 0.001;
+            const double din[3] = {d[0], d[1], d[2]};
+            h.w_r = h.w_r * X->hal_eff + h.w * X->hal_unit;
+            h.w_mu *= X->hal_eff;
+            h.w *= X->hal_eff;
+            if (rng_u(&rng) < 0.4) {
+                const double u1 = rng_u(&rng), u2 = rng_u(&rng);
+                const double r = sqrt(u1), ph2 = TAU * u2;
+                d[0] = r * cos(ph2);
+                d[1] = r * sin(ph2);
+                d[2] = -sqrt(fmax(1.0 - u1, 0.0));
+                normalize3(d);
+            } else {
+                d[2] = -fabs(d[2]);
+                normalize3(d);
+            }
+            if (h.w < X->cut && h.w_r < X->cut) break;
+            if (c->base_thickness > 0.0) {
+                if (!(din[2] > 1e-6) || !(-d[2] > 1e-6)) break;
+                o[0] += c->base_thickness * (din[0] / din[2] + d[0] / (-d[2]));
+                o[1] += c->base_thickness * (din[1] / din[2] + d[1] / (-d[2]));
+                if (o[0] < 0.0 || o[0] >= c->W || o[1] < 0.0 || o[1] >= c->H) break;
+            }
+            continue;
+        }
+        o[0] = q[0];
+        o[1] = q[1];
+        o[2] = q[2];
+        const double ct = sample_hg(d, c->g, &rng);
+        h.score += orc_hg_score(c->g, ct);
+        h.w *= X->albedo;
+        h.w_mu *= X->albedo;
+        h.w_r *= X->albedo;
+        if (h.w < X->cut && h.w_r < X->cut) break;
+    }
+    if (X->gdose) {
+        A->gmap[e_idx] += h.a_e;
+        A->gmu += h.a_mu;
+        A->gg += h.a_g;
+        A->gr += h.a_r;
+    }
+}
+
+static void *replay_worker(void *arg)
+{
+    void **a = (void **)arg;
+    trep *X = (trep *)a[0];
+    trep_acc *A = (trep_acc *)a[1];
+    for (;;) {
+        const int64_t p0 = __atomic_fetch_add(&X->next, X->grain, __ATOMIC_RELAXED);
+        if (p0 >= X->n_total) break;
+        const int64_t p1 = p0 + X->grain < X->n_total ? p0 + X->grain : X->n_total;
+        for (int64_t p = p0; p < p1; p++) replay_packet(X, A, p);
+    }
+    return NULL;
+}
+
+/* one replay pass; gdose == NULL: dose (n), else gmap (tw * th) and grads[3] */
+static int replay_pass(const orc_crystals *cs, const float *map, const orc_expected_cfg *e,
+                       const double *gdose, double *dose, double *gmap, double *grads,
+                       int64_t *bounces_events)
+{
+    const orc_expose_cfg *c = &e->x;
+    if (c->ppp <= 0 || c->tw <= 0 || c->th <= 0) return -1;
+    orc_crystals plain = *cs;
+    plain.developed = NULL;
+    plain.fe = NULL;
+    obvh b;
+    bvh_build(&plain, 1, &b);
+    ocrys *cr = make_recs(&plain);
+    trep X;
+    memset(&X, 0, sizeof X);
+    X.b = &b;
+    X.cr = cr;
+    X.e = e;
+    X.map = map;
+    X.gdose = gdose;
+    X.n = cs->n;
+    X.mu_t = c->mu_a + c->mu_s;
+    X.albedo = X.mu_t > 0.0 ? c->mu_s / X.mu_t : 0.0;
+    const double rr = (1.0 - c->n_refr) / (1.0 + c->n_refr);
+    X.r0 = rr * rr;
+    X.hal_unit = pow(10.0, -2.0 * c->od_ahu);
+    X.hal_eff = c->r_ahu * X.hal_unit;
+    X.cut = e->cut > 0.0 ? e->cut : 1e-6;
+    X.n_total = (int64_t)c->ppp * c->tw * c->th;
+    X.grain = 2048;
+    const int nt = online_threads(c->threads);
+    const int64_t nmap = (int64_t)c->tw * c->th;
+    trep_acc *acc = (trep_acc *)calloc((size_t)nt, sizeof(trep_acc));
+    for (int t = 0; t < nt; t++) {
+        if (gdose) acc[t].gmap = (double *)calloc((size_t)nmap, sizeof(double));
+        else acc[t].dose = (double *)calloc((size_t)(cs->n > 0 ? cs->n : 1), sizeof(double));
+    }
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nt);
+    void **args = (void **)malloc(sizeof(void *) * 2 * (size_t)nt);
+    for (int t = 0; t < nt; t++) {
+        args[2 * t] = &X;
+        args[2 * t + 1] = &acc[t];
+        pthread_create(&th[t], NULL, replay_worker, &args[2 * t]);
+    }
+    for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+    if (gdose) {
+        memset(gmap, 0, sizeof(double) * (size_t)nmap);
+        grads[0] = grads[1] = grads[2] = 0.0;
+    } else {
+        memset(dose, 0, sizeof(double) * (size_t)cs->n);
+    }
+    if (bounces_events) bounces_events[0] = bounces_events[1] = 0;
+    for (int t = 0; t < nt; t++) {
+        if (gdose) {
+            for (int64_t k = 0; k < nmap; k++) gmap[k] += acc[t].gmap[k];
+            grads[0] += acc[t].gmu;
+            grads[1] += acc[t].gg;
+            grads[2] += acc[t].gr;
+            free(acc[t].gmap);
+        } else {
+            for (int64_t k = 0; k < cs->n; k++) dose[k] += acc[t].dose[k];
+            free(acc[t].dose);
+        }
+        if (bounces_events) {
+            bounces_events[0] += acc[t].bounces;
+            bounces_events[1] += acc[t].events;
+        }
+    }
+    free(args);
+    free(th);
+    free(acc);
+    free(cr);
+    bvh_free(&b);
+    return 0;
+}
+
+int orc_expected_dose(const orc_crystals *cs, const float *map, const orc_expected_cfg *e,
+                      double *dose, int64_t *bounces_events)
+{
+    return replay_pass(cs, map, e, NULL, dose, NULL, NULL, bounces_events);
+}
+
+int orc_replay_grad(const orc_crystals *cs, const float *map, const orc_expected_cfg *e,
+                    const double *gdose, double *gmap, double grads[3])
+{
+    return replay_pass(cs, map, e, gdose, NULL, gmap, grads, NULL);
+}
+
+/* P[Poisson(lam) >= thr] and its derivative (the Poisson pmf at thr - 1) */
+void orc_poisson_tail(const double *lam, int64_t n, uint32_t thr, double *p, double *dp)
+{
+    for (int64_t i = 0; i < n; i++) {
+        const double l = lam[i];
+        if (thr == 0) {
+            p[i] = 1.0;
+            dp[i] = 0.0;
+            continue;
+        }
+        if (!(l > 0.0)) {
+            p[i] = 0.0;
+            dp[i] = thr == 1 ? 1.0 : 0.0;
+            continue;
+        }
+        /* terms l^k e^-l / k! in log space: the tail sum below thr, else 1 - head */
+        if (l < (double)thr) {
+            double tail = 0.0;
+            for (uint32_t k = thr; k < thr + 4000u; k++) {
+                const double t = exp((double)k * log(l) - l - lgamma((double)k + 1.0));
+                tail += t;
+                if (t < 1e-18 * tail) break;
+            }
+            p[i] = tail;
+        } else {
+            double head = 0.0;
+            for (uint32_t k = 0; k < thr; k++) head += exp((double)k * log(l) - l - lgamma((double)k + 1.0));
+            p[i] = 1.0 - head;
+        }
+        dp[i] = exp((double)(thr - 1) * log(l) - l - lgamma((double)thr));
+    }
+}
+
+/* ----------------------------------------------------------- expected scan ------ */
+typedef struct {
+    tscan S;                 /* geometry, rays, seed */
+    const double *prob;      /* per crystal */
+    const double *gT;        /* adjoint: dL/dT per pixel, or NULL */
+    double *T;               /* forward output */
+    double *gprob;           /* adjoint: per thread n, folded after */
+    int64_t n;
+} texpscan;
+
+typedef struct {
+    const ocrys *cr;
+    const double *prob;
+    double o[3], d[3], inv[3];
+    double sigma, prod, coef;
+    double *gprob;           /* NULL: product pass */
+} exphit;
+
+static inline int ex_box(const onode *nd, exphit *h)
+{
+    return box_hit_d(nd->mn, nd->mx, h->o, h->inv);
+}
+static inline void ex_prim(uint32_t id, exphit *h)
+{
+    const ocrys *r = &h->cr[id];
+    double t0, t1;
+    if (!crystal_intersect_d(r->c, r->q, r->dims, r->morph, h->o, h->d, &t0, &t1)) return;
+    const double a = 1.0 - exp(-h->sigma * fmax(t1 - t0, 0.0));
+    const double pr = h->prob[id];
+    const double f = 1.0 - pr * a;
+    if (h->gprob) {
+        if (f > 0.0) h->gprob[id] += -a * h->coef / f;
+    } else {
+        h->prod *= f;
+    }
+}
+
+typedef struct {
+    texpscan *X;
+    double *gprob;
+} expscan_arg;
+
+static void *expscan_worker(void *arg)
+{
+    expscan_arg *a = (expscan_arg *)arg;
+    texpscan *X = a->X;
+    tscan *S = &X->S;
+    const double nrays = (double)(S->g * S->g * S->ss * S->ss);
+    for (;;) {
+        const int64_t py = __atomic_fetch_add(&S->next_row, 1, __ATOMIC_RELAXED);
+        if (py >= S->h_px) break;
+        for (int32_t px = 0; px < S->w_px; px++) {
+            double gT = X->gT ? X->gT[py * S->w_px + px] : 0.0;
+            if (X->gT && gT == 0.0) continue;
+            double acc = 0.0;
+            for (int32_t ay = 0; ay < S->ss; ay++)
+                for (int32_t ax = 0; ax < S->ss; ax++)
+                    for (int32_t sy = 0; sy < S->g; sy++)
+                        for (int32_t sx = 0; sx < S->g; sx++) {
+                            float ox, oy;
+                            scan_origin(S, px, (int32_t)py, ax, ay, sx, sy, &ox, &oy);
+                            exphit h;
+                            memset(&h, 0, sizeof h);
+                            h.cr = S->cr;
+                            h.prob = X->prob;
+                            h.o[0] = ox;
+                            h.o[1] = oy;
+                            h.o[2] = S->D + 0.1;
+                            h.d[2] = -1.0;
+                            for (int k = 0; k < 3; k++) h.inv[k] = 1.0 / h.d[k];
+                            h.sigma = S->sigma;
+                            h.prod = 1.0;
+                            tcount cnt = {0, 0};
+                            if (S->b->nn > 0) DFS_TRAVERSE(S->b, ex_box, ex_prim, &h, &cnt);
+                            if (X->gT) {
+                                h.coef = gT / nrays * h.prod;
+                                h.gprob = a->gprob;
+                                if (S->b->nn > 0) DFS_TRAVERSE(S->b, ex_box, ex_prim, &h, &cnt);
+                            }
+                            acc += h.prod;
+                        }
+            if (!X->gT) X->T[py * S->w_px + px] = acc / nrays;
+        }
+    }
+    return NULL;
+}
+
+/* Expected scan over every crystal with semi-axes times fe; gT == NULL: T per pixel
+ * (mean ray product); else gprob = sum over rays of (gT / rays) * dT_ray / dprob. */
+int orc_expected_scan(const orc_crystals *cs, const double *prob, const orc_expected_cfg *e,
+                      const double *gT, double *T, double *gprob)
+{
+    const int64_t n = cs->n;
+    uint8_t *dev = (uint8_t *)malloc((size_t)(n > 0 ? n : 1));
+    double *fe = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) {
+        dev[i] = 1;
+        fe[i] = e->fe;
+    }
+    orc_crystals dc = *cs;
+    dc.developed = dev;
+    dc.fe = fe;
+    obvh b;
+    bvh_build(&dc, 1, &b);
+    ocrys *cr = make_recs(&dc);
+    texpscan X;
+    memset(&X, 0, sizeof X);
+    tscan *S = &X.S;
+    S->b = &b;
+    S->cr = cr;
+    S->W = e->x.W;
+    S->H = e->x.H;
+    S->D = e->x.D;
+    S->sigma = e->sigma;
+    S->w_px = e->w_px;
+    S->h_px = e->h_px;
+    const int32_t sp = e->spp > 1 ? e->spp : 1;
+    S->g = (int32_t)round(sqrt((double)sp));
+    if (S->g < 1) S->g = 1;
+    S->ss = e->ss > 1 ? e->ss : 1;
+    S->pw = (float)(e->x.W / (double)e->w_px);
+    S->ph = (float)(e->x.H / (double)e->h_px);
+    S->seed = e->scan_seed;
+    X.prob = prob;
+    X.gT = gT;
+    X.T = T;
+    X.n = n;
+    const int nt = online_threads(e->x.threads);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nt);
+    expscan_arg *args = (expscan_arg *)calloc((size_t)nt, sizeof(expscan_arg));
+    for (int t = 0; t < nt; t++) {
+        args[t].X = &X;
+        if (gT) args[t].gprob = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+        pthread_create(&th[t], NULL, expscan_worker, &args[t]);
+    }
+    for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+    if (gT) {
+        memset(gprob, 0, sizeof(double) * (size_t)n);
+        for (int t = 0; t < nt; t++) {
+            for (int64_t i = 0; i < n; i++) gprob[i] += args[t].gprob[i];
+            free(args[t].gprob);
+        }
+    }
+    free(args);
+    free(th);
+    free(cr);
+    bvh_free(&b);
+    free(fe);
+    free(dev);
+    return 0;
 }

@@ -119,6 +119,29 @@ int orc_antihalation(double r_ahu, double od, uint64_t seed, double *energy, dou
 int64_t orc_synthesize(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, double *pos,
                        double *dims, double *quat, int32_t *morph, double *sens);
 
+/* --- expected (differentiable) model: no reference counterpart ---------------------- */
+typedef struct {
+    orc_expose_cfg x;       /* sampler 1 with ppp > 0 and an exposure map; r_ahu = R */
+    double cut;             /* relative weight cutoff of a replayed packet (0 = 1e-6) */
+    double fe;              /* filament expansion of the expected-scan geometry */
+    double sigma;           /* silver opacity */
+    int32_t w_px, h_px, spp, ss;
+    uint64_t scan_seed;
+} orc_expected_cfg;
+
+/* expected dose per crystal (crystal order); bounces_events[2] optional */
+int orc_expected_dose(const orc_crystals *cs, const float *map, const orc_expected_cfg *e,
+                      double *dose, int64_t *bounces_events);
+/* P[Poisson(dose) >= thr] and d/d(dose) */
+void orc_poisson_tail(const double *dose, int64_t n, uint32_t thr, double *p, double *dp);
+/* gT == NULL: T (w_px * h_px mean ray products); else gprob (n) for dL/dT = gT */
+int orc_expected_scan(const orc_crystals *cs, const double *prob, const orc_expected_cfg *e,
+                      const double *gT, double *T, double *gprob);
+/* gradient replay: gmap (tw * th) = dL/dE, grads = dL/d(mu_abs, g, R) for dL/ddose = gdose */
+int orc_replay_grad(const orc_crystals *cs, const float *map, const orc_expected_cfg *e,
+                    const double *gdose, double *gmap, double grads[3]);
+double orc_hg_score(double g, double mu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,12 @@ class OrcSynthCfg(C.Structure):
         ("morph", C.c_int32), ("k_samples", C.c_int32)]
 
 
+class OrcExpectedCfg(C.Structure):
+    _fields_ = [("x", OrcExposeCfg), ("cut", C.c_double), ("fe", C.c_double),
+                ("sigma", C.c_double), ("w_px", C.c_int32), ("h_px", C.c_int32),
+                ("spp", C.c_int32), ("ss", C.c_int32), ("scan_seed", C.c_uint64)]
+
+
 def build() -> str:
     """Compile liboracle.so with the committed Makefile (gcc, -ffp-contract=off)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -112,6 +118,15 @@ def lib():
         L.orc_fresnel.argtypes = [C.c_double, C.c_double]
         L.orc_fresnel.restype = C.c_double
         L.orc_antihalation.argtypes = [C.c_double, C.c_double, C.c_uint64, _f64p, _f64p]
+        L.orc_expected_dose.argtypes = [C.POINTER(OrcCrystals), _f32p, C.POINTER(OrcExpectedCfg),
+                                        _f64p, _i64p]
+        L.orc_poisson_tail.argtypes = [_f64p, C.c_int64, C.c_uint32, _f64p, _f64p]
+        L.orc_expected_scan.argtypes = [C.POINTER(OrcCrystals), _f64p, C.POINTER(OrcExpectedCfg),
+                                        _f64p, _f64p, _f64p]
+        L.orc_replay_grad.argtypes = [C.POINTER(OrcCrystals), _f32p, C.POINTER(OrcExpectedCfg),
+                                      _f64p, _f64p, _f64p]
+        L.orc_hg_score.argtypes = [C.c_double, C.c_double]
+        L.orc_hg_score.restype = C.c_double
         _lib = L
     return _lib
 
@@ -317,3 +332,93 @@ def antihalation(r_ahu, od, seed, energy, d):
     d_ = np.ascontiguousarray(d, np.float64).copy()
     alive = lib().orc_antihalation(r_ahu, od, seed, C.byref(e), _p(d_, _f64p))
     return bool(alive), e.value, d_
+
+
+# ---------------------------------------------------------------------------------------
+# Expected (differentiable) model: FP64 gradient oracle.  No reference counterpart; the
+# C source states the model.  The scan mode map and its derivative are evaluated here.
+
+def expected_cfg(exposure_shape, *, W, H, D, mu_a, mu_s, g, n_refr, r_ahu, od_ahu, mu_abs,
+                 theta_max, max_bounces, ppp, seed, fe, sigma, w_px, h_px, spp=4, ss=1,
+                 scan_seed=0, cut=1e-6, base_thickness=0.0, threads=0) -> OrcExpectedCfg:
+    th, tw = exposure_shape
+    x = OrcExposeCfg(W, H, D, mu_a, mu_s, g, n_refr, r_ahu, od_ahu, mu_abs, theta_max, 0.0,
+                     max_bounces, 1, 0, ppp, threads, seed, tw, th, 0, base_thickness)
+    return OrcExpectedCfg(x, cut, fe, sigma, w_px, h_px, spp, ss, scan_seed)
+
+
+def poisson_tail(dose, threshold):
+    """P[Poisson(dose) >= threshold] and its derivative."""
+    d = np.ascontiguousarray(dose, np.float64)
+    p = np.zeros_like(d)
+    dp = np.zeros_like(d)
+    lib().orc_poisson_tail(_p(d, _f64p), d.size, threshold, _p(p, _f64p), _p(dp, _f64p))
+    return p, dp
+
+
+def mode_map(T, mode):
+    T = np.asarray(T, np.float64)
+    if mode == 0:
+        return T
+    if mode == 1:
+        return 1.0 - T
+    if mode == 2:
+        return np.clip(-np.log10(np.maximum(T, 1e-4)) / 3.0, 0.0, 1.0)
+    return -np.log10(np.clip(T, 1e-4, 1.0)) + 0.05
+
+
+def mode_map_grad(T, mode):
+    T = np.asarray(T, np.float64)
+    ln10 = np.log(10.0)
+    if mode == 0:
+        return np.ones_like(T)
+    if mode == 1:
+        return -np.ones_like(T)
+    if mode == 2:
+        return np.where(T > 1e-3, -1.0 / (3.0 * np.maximum(T, 1e-300) * ln10), 0.0)
+    return np.where(T > 1e-4, -1.0 / (np.maximum(T, 1e-300) * ln10), 0.0)
+
+
+def expected_forward(cs: Crystals, exposure_map, cfg: OrcExpectedCfg, threshold, mode):
+    """-> dict(dose, prob, dprob, T, out): the expected model in FP64."""
+    em = np.ascontiguousarray(exposure_map, np.float32)
+    cc = cs.c()
+    dose = np.zeros(max(cs.n, 1), np.float64)
+    be = np.zeros(2, np.int64)
+    lib().orc_expected_dose(C.byref(cc), _p(em, _f32p), C.byref(cfg), _p(dose, _f64p), _p(be, _i64p))
+    dose = dose[:cs.n]
+    prob, dprob = poisson_tail(dose, threshold)
+    T = np.zeros((cfg.h_px, cfg.w_px), np.float64)
+    lib().orc_expected_scan(C.byref(cc), _p(np.ascontiguousarray(prob), _f64p), C.byref(cfg), None,
+                            _p(T, _f64p), None)
+    return {"dose": dose, "prob": prob, "dprob": dprob, "T": T, "out": mode_map(T, mode),
+            "bounces": int(be[0]), "events": int(be[1])}
+
+
+def expected_vjp(cs: Crystals, exposure_map, cfg: OrcExpectedCfg, threshold, mode, grad_out,
+                 dose=None):
+    """-> dict(grad_exposure, grads [mu_abs, g, R], grad_dose, grad_prob) in FP64, for
+    dL/dout = grad_out; dose from a forward (recomputed when None)."""
+    em = np.ascontiguousarray(exposure_map, np.float32)
+    cc = cs.c()
+    if dose is None:
+        dose = expected_forward(cs, em, cfg, threshold, mode)["dose"]
+    dose = np.ascontiguousarray(dose, np.float64)
+    prob, dprob = poisson_tail(dose, threshold)
+    T = np.zeros((cfg.h_px, cfg.w_px), np.float64)
+    lib().orc_expected_scan(C.byref(cc), _p(prob, _f64p), C.byref(cfg), None, _p(T, _f64p), None)
+    gT = np.ascontiguousarray(np.asarray(grad_out, np.float64) * mode_map_grad(T, mode))
+    gprob = np.zeros(max(cs.n, 1), np.float64)
+    lib().orc_expected_scan(C.byref(cc), _p(prob, _f64p), C.byref(cfg), _p(gT, _f64p), None,
+                            _p(gprob, _f64p))
+    gprob = gprob[:cs.n]
+    gdose = np.ascontiguousarray(gprob * dprob)
+    gmap = np.zeros(em.shape, np.float64)
+    grads = np.zeros(3, np.float64)
+    lib().orc_replay_grad(C.byref(cc), _p(em, _f32p), C.byref(cfg), _p(gdose, _f64p),
+                          _p(gmap, _f64p), _p(grads, _f64p))
+    return {"grad_exposure": gmap, "grads": grads, "grad_dose": gdose, "grad_prob": gprob}
+
+
+def hg_score(g, mu):
+    return lib().orc_hg_score(g, mu)

@@ -30,6 +30,30 @@ def test_library_exports_every_header_symbol():
     assert L.fg_abi_version() == 1
 
 
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Every ctypes mirror has the size the C compiler gives the header's struct."""
+    pairs = {
+        "fg_optics": _lib.Optics, "fg_emulsion_config": _lib.EmulsionConfigC,
+        "fg_exposure_config": _lib.ExposureConfigC, "fg_expose_stats": _lib.ExposeStatsC,
+        "fg_development_config": _lib.DevelopmentConfigC,
+        "fg_development_report": _lib.DevelopmentReportC, "fg_scan_config": _lib.ScanConfigC,
+        "fg_tuning": _lib.TuningC, "fg_tile_params": _lib.TileParamsC,
+        "fg_tile_stats": _lib.TileStatsC, "fg_frame_desc": _lib.FrameDescC,
+        "fg_frame_stats": _lib.FrameStatsC, "fg_expected_model": _lib.ExpectedModelC,
+        "fg_param_grads": _lib.ParamGradsC, "fg_adjoint_stats": _lib.AdjointStatsC,
+    }
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include "fgrain_hip.h"\nint main(void){\n' +
+                   "".join(f'printf("%zu\\n", sizeof({k}));\n' for k in pairs) + "return 0;}\n")
+    exe = tmp_path / "sz"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                            check=True).stdout.split()]
+    for (k, cls), n in zip(pairs.items(), sizes):
+        assert C.sizeof(cls) == n, k
+
+
 @pytest.mark.parametrize("morph", [0, 1, 2])
 def test_host_synthesis_equals_oracle(morph):
     cfg = trix_config(48, 40, morph=morph, aspect=8.0 if morph == 1 else 1.0)
