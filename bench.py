@@ -6,7 +6,12 @@ paths), supersample 2 (16 scan rays per pixel, 24 Mpix out).  One step renders t
 whole frame: transport -> development -> scan -> OTF, with the per-tile emulsions
 (crystals + BVH) already resident in HBM.  Prints one JSON line.
 
+The line also carries the adjoint leg (tile512_trix_adjoint: expected-model forward and
+its path-replay VJP, with their time ratio) and the CPU baseline.  The inverse-fit
+workload prints its own line:
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|batched]
+  python bench.py --workload fit1024_trix1600_dye3 [--steps K]
 """
 from __future__ import annotations
 
@@ -74,18 +79,112 @@ def cpu_baseline(cfg, exposure, tile_px=320):
     }
 
 
+def adjoint_leg(engine, name="tile512_trix_adjoint", reps=3):
+    """Expected-model forward and its VJP (path replay) on the differentiation tile,
+    beside the plain stochastic forward of the same tile (fg_expose -> fg_develop ->
+    fg_scan with the same seeds)."""
+    import numpy as np
+    import fgrain_amd as F
+    from fgrain_amd import workloads as W
+    from fgrain_amd.adjoint import ExpectedFilm, mse_loss
+
+    cfg = W.load_config(name)
+    w, h, seed = cfg["frame"]["width"], cfg["frame"]["height"], cfg["seed"]
+    em = F.VirtualEmulsionLayer.synthesize(W.emulsion_config(cfg, float(w), float(h)), seed)
+    E = W.workload_exposure(cfg) * np.float32(W.packet_weight(cfg))
+    film = ExpectedFilm(em, W.expected_model(cfg, w, h, seed), engine)
+    tgt, _ = film.forward(E * np.float32(2.0 ** cfg["loss"]["target_shift_stops"]))
+    out, dose = film.forward(E)
+    L, G = mse_loss(out, tgt)
+    film.vjp(E, G, dose=dose)
+    tf, tb, ts = [], [], []
+    kf, kb = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out, dose = film.forward(E)
+        t1 = time.perf_counter()
+        kf.append(dict(film.last_stats))
+        L, G = mse_loss(out, tgt)
+        t2 = time.perf_counter()
+        gE, pg = film.vjp(E, G, dose=dose)
+        t3 = time.perf_counter()
+        kb.append(dict(film.last_stats))
+        tf.append((t1 - t0) * 1e3)
+        tb.append((t3 - t2) * 1e3)
+    zero = np.zeros(len(em), np.uint32)
+    for _ in range(reps):
+        em.set_state(counts=zero)
+        t0 = time.perf_counter()
+        F.expose_emulsion(em, E, W.exposure_config(cfg), seed + 1, engine)
+        F.develop_emulsion(em, W.development_config(cfg), engine)
+        F.scan_emulsion(em, W.scan_config(cfg, w, h), seed + 2, engine)
+        ts.append((time.perf_counter() - t0) * 1e3)
+    packets = kf[0]["packets"]
+    mf, mb, ms = float(np.median(tf)), float(np.median(tb)), float(np.median(ts))
+    return {
+        "workload": name, "crystals": len(em), "packets": packets,
+        "ms_forward": mf, "ms_vjp": mb, "adjoint_forward_ratio": mb / mf,
+        "ms_plain_forward": ms, "adjoint_plain_forward_ratio": mb / ms,
+        "ms_kernel_forward": {"replay": float(np.median([k["ms_replay"] for k in kf])),
+                              "scan": float(np.median([k["ms_scan"] for k in kf]))},
+        "ms_kernel_vjp": {"replay": float(np.median([k["ms_replay"] for k in kb])),
+                          "scan_adjoint": float(np.median([k["ms_scan"] for k in kb]))},
+        "replay_mpaths_per_s": packets / (float(np.median([k["ms_replay"] for k in kb])) * 1e-3) / 1e6,
+        "loss": L, "grads": {"crystal_absorption_coeff": pg.crystal_absorption_coeff,
+                             "anisotropy_g": pg.anisotropy_g,
+                             "antihalation_reflectance": pg.antihalation_reflectance,
+                             "exposure_l2": float(np.linalg.norm(gE))},
+    }
+
+
+def fit_main(args, engine, cfg):
+    """Inverse fit workload: one JSON line with time per step, the adjoint/forward
+    ratio and the plain-forward density RMSE."""
+    from fgrain_amd.fit import DyeFit
+
+    t = time.perf_counter()
+    fit = DyeFit(engine, cfg, log=log)
+    log(f"fit ready in {time.perf_counter() - t:.1f} s (host synthesis)")
+    steps = args.steps if args.steps_given else cfg["fit"]["steps"]
+    r = fit.run(steps=steps, log=log)
+    fr = cfg["frame"]
+    packets = fr["width"] * fr["height"] * cfg["exposure"]["photons_per_pixel"] * len(fit.layers)
+    out = {
+        "metric": "inverse fit: time per Adam step (expected forward + adjoint over all tiles)",
+        "value": r["ms_per_step"], "unit": "ms/step", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "frame": [fr["width"], fr["height"]],
+                   "tile": fr["tile"], "layers": len(fit.layers),
+                   "photons_per_pixel": cfg["exposure"]["photons_per_pixel"],
+                   "packets_per_forward": packets},
+        "adjoint_forward_ratio": r["adjoint_forward_ratio"],
+        "ms_forward_per_step": r["ms_forward_per_step"], "ms_adjoint_per_step": r["ms_adjoint_per_step"],
+        "loss_first": r["loss_first"], "loss_last": r["loss_last"],
+        "rmse_plain_fit": r["rmse_plain_fit"], "rmse_plain_init": r["rmse_plain_init"],
+        "rmse_plain_truth_resampled": r["rmse_plain_truth_resampled"],
+        "exposure_rmse_log2": r["rel_exposure_rmse_log2"],
+        "exposure_rmse_log2_init": r["rel_exposure_rmse_log2_init"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile comparison step")
     ap.add_argument("--workload", default="frame24mp_trix_cubic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (counter calibration)")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 3
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(tiles would shard across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -107,6 +206,8 @@ def main():
     cfg = W.load_config(args.workload)
     engine = F.GpuEngine.new()
     log(f"engine: {engine.adapter_name()}")
+    if "fit" in cfg:
+        return fit_main(args, engine, cfg)
     t = time.perf_counter()
     exposure = W.workload_exposure(cfg)
     frame = W.Frame(engine, cfg, exposure)
@@ -189,6 +290,9 @@ def main():
         plain_ms = (time.perf_counter() - t1) * 1e3
         out["plain_ms_per_step"] = plain_ms
         out["batched_speedup_vs_plain"] = plain_ms / ms_step
+    if not args.no_adjoint:
+        log("adjoint leg (tile512_trix_adjoint) ...")
+        out["adjoint"] = adjoint_leg(engine)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, host cores) ...")
         out["cpu_baseline"] = cpu_baseline(cfg, exposure)

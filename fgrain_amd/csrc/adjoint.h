@@ -22,8 +22,9 @@ struct ReplayArgs {
     float pad;
 };
 
-// Expected-scan parameters: the BVH holds every crystal with semi-axes scaled by the
-// filament expansion; the sens word of each record holds the crystal's transport slot.
+// Expected-scan parameters: the BVH holds every crystal with boxes for the largest
+// boosted expansion, records with semi-axes scaled by the unboosted filament expansion;
+// the sens word of each record holds the crystal's transport slot.
 struct ExpScanArgs {
     const Node *nodes;
     const float4 *recs;
@@ -32,9 +33,11 @@ struct ExpScanArgs {
     float pw, ph, z0, sigma;
     uint32_t key0, key1;
     const float *prob;     // development probability per transport slot
+    const float *boost;    // expected clump boost of the expansion per transport slot
     float *out;            // forward: w_px * h_px
     const float *gout;     // adjoint: dL/d(out)
     double *gprob;         // adjoint: dL/d(prob) accumulators per transport slot
+    double *gboost;        // adjoint: dL/d(boost) accumulators per transport slot
 };
 
 int replay_blocks(uint64_t n_photons);
@@ -45,6 +48,13 @@ hipError_t launch_dev_prob(const double *dose64, float *dose32, float *prob, flo
                            uint32_t n, uint32_t thr, hipStream_t s);
 hipError_t launch_dev_prob_from(const float *dose32, float *prob, float *dprob, uint32_t n,
                                 uint32_t thr, hipStream_t s);
+// Expected clump boost: K_i = sum of prob over the touch neighbours (CSR in slot order),
+// boost_i = B(1 + K_i) and dboost_i = B'(1 + K_i)
+hipError_t launch_boost(const uint32_t *ptr, const uint32_t *idx, const float *prob, float *boost,
+                        float *dboost, uint32_t n, hipStream_t s);
+// gprob_j += sum over neighbours i of gboost_i * dboost_i (the touch graph is symmetric)
+hipError_t launch_boost_grad(const uint32_t *ptr, const uint32_t *idx, const double *gboost,
+                             const float *dboost, double *gprob, uint32_t n, hipStream_t s);
 hipError_t launch_expected_scan(const ExpScanArgs &a, hipStream_t s);
 hipError_t launch_expected_scan_adjoint(const ExpScanArgs &a, hipStream_t s);
 // gdose = gprob * dprob (float)

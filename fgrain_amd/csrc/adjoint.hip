@@ -302,9 +302,50 @@ __global__ void __launch_bounds__(256) dev_prob_kernel(const double *dose64, flo
     dprob[k] = (float)dp;
 }
 
+// Expected clump boost of development.rs:164-171: a clump of size s multiplies the
+// expansion by 1.15 + 0.18 (min(s, 6) - 1), a lone crystal by 1.  The expected model
+// takes s = 1 + K (K = expected number of developed touching neighbours) and joins
+// 1 and the size-2 boost linearly for 1 <= s <= 2.
+__device__ __forceinline__ void boost_of(float s, float &b, float &db)
+{
+    if (s <= 1.0f) { b = 1.0f; db = 0.0f; }
+    else if (s <= 2.0f) { b = 1.0f + 0.33f * (s - 1.0f); db = 0.33f; }
+    else if (s < 6.0f) { b = 1.15f + 0.18f * (s - 1.0f); db = 0.18f; }
+    else { b = 2.05f; db = 0.0f; }
+}
+
+__global__ void __launch_bounds__(256) boost_kernel(const uint32_t *ptr, const uint32_t *idx,
+                                                    const float *prob, float *boost, float *dboost,
+                                                    uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float K = 0.0f;
+    for (uint32_t e = ptr[i]; e < ptr[i + 1]; e++) K += prob[idx[e]];
+    float b, db;
+    boost_of(1.0f + K, b, db);
+    boost[i] = b;
+    dboost[i] = db;
+}
+
+__global__ void __launch_bounds__(256) boost_grad_kernel(const uint32_t *ptr, const uint32_t *idx,
+                                                         const double *gboost, const float *dboost,
+                                                         double *gprob, uint32_t n)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double acc = 0.0;
+    for (uint32_t e = ptr[j]; e < ptr[j + 1]; e++) {
+        const uint32_t i = idx[e];
+        acc += gboost[i] * (double)dboost[i];
+    }
+    gprob[j] += acc;
+}
+
 // Transmittance product along the ray at (ox, oy) going -z: every crystal whose
-// expanded shape the ray crosses contributes 1 - prob * (1 - exp(-sigma * chord)).
-// Node and crystal tests are those of the scan (strict x-y containment).
+// boosted shape the ray crosses contributes 1 - prob * (1 - exp(-sigma * chord)).
+// Node tests are those of the scan (strict x-y containment on boxes that hold the
+// largest boost).
 template <bool kAdjoint>
 __device__ __forceinline__ float ray_product(const ExpScanArgs &A, float ox, float oy, float coef)
 {
@@ -316,17 +357,28 @@ __device__ __forceinline__ float ray_product(const ExpScanArgs &A, float ox, flo
         if (hit && nd.prim != kInterior) {
             const uint32_t first = nd.prim >> 4, cnt = nd.prim & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
-                if (!sphere_may_hit_z(A.recs[3 * k], A.recs[3 * k + 2], ox, oy)) continue;
+                const float4 a = A.recs[3 * k], c = A.recs[3 * k + 2];
+                const uint32_t slot = __float_as_uint(a.w);
+                const float bs = A.boost[slot];
+                const float4 cs = make_float4(c.x * bs, c.y * bs, c.z * bs, c.w);
+                if (!sphere_may_hit_z(a, cs, ox, oy)) continue;
                 const Rec r = load_rec(A.recs, k);
-                float t0, t1;
-                if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, A.z0, 0.0f, 0.0f, -1.0f, t0, t1)) continue;
-                const uint32_t slot = __float_as_uint(r.sens);
-                const float a = 1.0f - expf(-A.sigma * fmaxf(t1 - t0, 0.0f));
-                const float f = 1.0f - A.prob[slot] * a;
+                float chord, dchord;
+                if (!crystal_chord_scaled(r.c, r.q, r.h, r.morph, bs, ox, oy, A.z0, 0.0f, 0.0f, -1.0f,
+                                          chord, dchord))
+                    continue;
+                const float ex = expf(-A.sigma * chord);
+                const float am = 1.0f - ex;
+                const float p = A.prob[slot];
+                const float f = 1.0f - p * am;
                 if (kAdjoint) {
-                    // coef * dT_ray/dprob = coef * (-a) * prod_{j != i} f_j; coef holds the
-                    // full ray product, f > 0 since prob <= 1 and a < 1
-                    if (f > 0.0f) atomicAdd(&A.gprob[slot], (double)(-a * (coef / f)));
+                    // coef holds dL/dT_ray (mean weight included) times the full product;
+                    // f > 0 since prob <= 1 and exp(-sigma chord) > 0
+                    if (f > 0.0f) {
+                        const float rest = coef / f;
+                        atomicAdd(&A.gprob[slot], (double)(-am * rest));
+                        atomicAdd(&A.gboost[slot], (double)(-p * A.sigma * ex * dchord * rest));
+                    }
                 } else {
                     prod *= f;
                 }
@@ -472,6 +524,22 @@ hipError_t launch_expected_scan_adjoint(const ExpScanArgs &a, hipStream_t s)
 {
     if ((uint64_t)a.w_px * a.h_px == 0) return hipSuccess;
     expected_scan_kernel<true><<<expected_scan_grid(a), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_boost(const uint32_t *ptr, const uint32_t *idx, const float *prob, float *boost,
+                        float *dboost, uint32_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    boost_kernel<<<blocks_for(n), 256, 0, s>>>(ptr, idx, prob, boost, dboost, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_boost_grad(const uint32_t *ptr, const uint32_t *idx, const double *gboost,
+                             const float *dboost, double *gprob, uint32_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    boost_grad_kernel<<<blocks_for(n), 256, 0, s>>>(ptr, idx, gboost, dboost, gprob, n);
     return hipGetLastError();
 }
 

@@ -40,16 +40,20 @@ bool valid_model(const fg_expected_model *m, const float *emap, uint32_t tw, uin
     return true;
 }
 
-// Expected-model scan BVH: every crystal with semi-axes times fe (the reference's
-// developed-crystal box rule, crystal.rs:106-133), records carrying transport slots.
-fg_status ensure_expected_bvh(fg_engine *E, Emulsion &em, double fe)
+// Largest expected clump boost (development.rs:164-171 at clump size >= 6).
+constexpr double kMaxBoost = 2.05;
+
+// Expected-model scan BVH: every crystal, boxes by the reference's developed-crystal
+// rule (crystal.rs:106-133) at the largest boosted expansion fe * kMaxBoost, records with
+// semi-axes times fe carrying transport slots; and the touch graph in slot order.
+fg_status ensure_expected_bvh(fg_engine *E, Emulsion &em, double fe, double pf)
 {
     DeviceEmulsion &d = *em.dev;
-    if (d.exp_fe == fe) return FG_OK;
+    if (d.exp_fe == fe && d.exp_pf == pf) return FG_OK;
     std::vector<uint8_t> all((size_t)em.n, 1u);
-    std::vector<double> fes((size_t)em.n, fe);
+    std::vector<double> fes((size_t)em.n, fe), fmax_((size_t)em.n, fe * kMaxBoost);
     RefBvh b;
-    build_ref_bvh(em.n, em.pos.data(), em.dims.data(), all.data(), fes.data(), nullptr, b);
+    build_ref_bvh(em.n, em.pos.data(), em.dims.data(), all.data(), fmax_.data(), nullptr, b);
     std::vector<Node> nodes;
     std::string msg;
     if (!flatten_bvh(b, nodes, msg)) return fail(FG_ERR_INVALID_ARG, "expected scan: " + msg);
@@ -61,24 +65,41 @@ fg_status ensure_expected_bvh(fg_engine *E, Emulsion &em, double fe)
         const uint32_t s = slot_of[b.idx[k]];
         memcpy(&recs[k].sens, &s, sizeof s);
     }
+    std::vector<uint32_t> cptr, cidx, sptr((size_t)em.n + 1, 0u), sidx;
+    touch_graph(em, fe, pf, cptr, cidx);
+    sidx.reserve(cidx.size());
+    for (size_t k = 0; k < em.bvh.idx.size(); k++) {
+        const uint32_t i = em.bvh.idx[k];
+        for (uint32_t e = cptr[i]; e < cptr[i + 1]; e++) sidx.push_back(slot_of[cidx[e]]);
+        sptr[k + 1] = (uint32_t)sidx.size();
+    }
     HIP_TRY(d.exp_nodes.ensure(nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(d.exp_recs.ensure(recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(d.exp_ptr.ensure(sptr.size() * sizeof(uint32_t)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(d.exp_idx.ensure(std::max<size_t>(sidx.size(), 1) * sizeof(uint32_t)), FG_ERR_BUFFER_RANGE);
     if (!nodes.empty())
         HIP_TRY(hipMemcpyAsync(d.exp_nodes.p, nodes.data(), nodes.size() * sizeof(Node),
                                hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
     if (!recs.empty())
         HIP_TRY(hipMemcpyAsync(d.exp_recs.p, recs.data(), recs.size() * sizeof(Crystal),
                                hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipMemcpyAsync(d.exp_ptr.p, sptr.data(), sptr.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           E->stream), FG_ERR_BUFFER_MAP);
+    if (!sidx.empty())
+        HIP_TRY(hipMemcpyAsync(d.exp_idx.p, sidx.data(), sidx.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
     // the host vectors go out of scope: wait for the copies
     HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
     d.exp_n_nodes = (uint32_t)nodes.size();
     d.exp_fe = fe;
+    d.exp_pf = pf;
     return FG_OK;
 }
 
 struct Setup {
     ReplayArgs r{};
     ExpScanArgs s{};
+    const uint32_t *ptr = nullptr, *idx = nullptr;  // touch graph, slot order
     uint32_t n = 0;
     size_t npx = 0, nmap = 0;
 };
@@ -88,7 +109,8 @@ fg_status setup(fg_engine *E, Emulsion &em, const fg_expected_model &m, const fl
 {
     fg_status st = ensure_device(E, em);
     if (st != FG_OK) return st;
-    st = ensure_expected_bvh(E, em, m.development.filament_expansion);
+    st = ensure_expected_bvh(E, em, m.development.filament_expansion,
+                             m.development.clumping_proximity_factor);
     if (st != FG_OK) return st;
     DeviceEmulsion &d = *em.dev;
     S.n = (uint32_t)em.n;
@@ -113,9 +135,12 @@ fg_status setup(fg_engine *E, Emulsion &em, const fg_expected_model &m, const fl
     S.r.refl = (float)m.optics.antihalation_reflectance;
     S.r.cut = (float)(m.weight_cutoff > 0.0 ? m.weight_cutoff : 1e-6);
     const size_t n = std::max<size_t>(S.n, 1);
-    for (DevBuf *b : {&E->dose32, &E->prob, &E->dprob, &E->gdose})
+    for (DevBuf *b : {&E->dose32, &E->prob, &E->dprob, &E->gdose, &E->boost, &E->dboost})
         HIP_TRY(b->ensure(n * sizeof(float)), FG_ERR_BUFFER_RANGE);
-    for (DevBuf *b : {&E->dose64, &E->gprob}) HIP_TRY(b->ensure(n * sizeof(double)), FG_ERR_BUFFER_RANGE);
+    for (DevBuf *b : {&E->dose64, &E->gprob, &E->gboost})
+        HIP_TRY(b->ensure(n * sizeof(double)), FG_ERR_BUFFER_RANGE);
+    S.ptr = d.exp_ptr.as<uint32_t>();
+    S.idx = d.exp_idx.as<uint32_t>();
     ExpScanArgs &s = S.s;
     s.nodes = d.exp_nodes.as<Node>();
     s.recs = d.exp_recs.as<float4>();
@@ -133,6 +158,7 @@ fg_status setup(fg_engine *E, Emulsion &em, const fg_expected_model &m, const fl
     s.key0 = (uint32_t)m.scan_seed;
     s.key1 = (uint32_t)(m.scan_seed >> 32);
     s.prob = E->prob.as<float>();
+    s.boost = E->boost.as<float>();
     return FG_OK;
 }
 
@@ -147,6 +173,8 @@ fg_status dose_pass(fg_engine *E, Setup &S, const fg_expected_model &m, float *m
     HIP_TRY(launch_dev_prob(E->dose64.as<double>(), E->dose32.as<float>(), E->prob.as<float>(),
                             E->dprob.as<float>(), S.n, m.development.threshold_photons, E->stream),
             FG_ERR_PIPELINE);
+    HIP_TRY(launch_boost(S.ptr, S.idx, E->prob.as<float>(), E->boost.as<float>(), E->dboost.as<float>(), S.n,
+                         E->stream), FG_ERR_PIPELINE);
     HIP_TRY(hipEventSynchronize(E->ev1), FG_ERR_CHANNEL);
     if (ms) (void)hipEventElapsedTime(ms, E->ev0, E->ev1);
     return FG_OK;
@@ -241,6 +269,8 @@ extern "C" fg_status fg_vjp(fg_engine *E, const fg_emulsion *emw, const fg_expec
         HIP_TRY(launch_dev_prob_from(E->dose32.as<float>(), E->prob.as<float>(), E->dprob.as<float>(), S.n,
                                      model->development.threshold_photons, E->stream),
                 FG_ERR_PIPELINE);
+        HIP_TRY(launch_boost(S.ptr, S.idx, E->prob.as<float>(), E->boost.as<float>(), E->dboost.as<float>(),
+                             S.n, E->stream), FG_ERR_PIPELINE);
         HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
     } else {
         st = dose_pass(E, S, *model, nullptr);
@@ -253,11 +283,15 @@ extern "C" fg_status fg_vjp(fg_engine *E, const fg_emulsion *emw, const fg_expec
         HIP_TRY(hipMemcpyAsync(E->gout.p, grad_out, S.npx * sizeof(float), hipMemcpyHostToDevice, E->stream),
                 FG_ERR_BUFFER_MAP);
     if (S.n) HIP_TRY(hipMemsetAsync(E->gprob.p, 0, (size_t)S.n * sizeof(double), E->stream), FG_ERR_BUFFER_MAP);
+    if (S.n) HIP_TRY(hipMemsetAsync(E->gboost.p, 0, (size_t)S.n * sizeof(double), E->stream), FG_ERR_BUFFER_MAP);
     S.s.gout = E->gout.as<float>();
     S.s.gprob = E->gprob.as<double>();
+    S.s.gboost = E->gboost.as<double>();
     HIP_TRY(hipEventRecord(E->ev2, E->stream), FG_ERR_CHANNEL);
     HIP_TRY(launch_expected_scan_adjoint(S.s, E->stream), FG_ERR_PIPELINE);
     HIP_TRY(hipEventRecord(E->ev3, E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(launch_boost_grad(S.ptr, S.idx, E->gboost.as<double>(), E->dboost.as<float>(), E->gprob.as<double>(),
+                              S.n, E->stream), FG_ERR_PIPELINE);
     HIP_TRY(launch_dose_grad(E->gprob.as<double>(), E->dprob.as<float>(), E->gdose.as<float>(), S.n, E->stream),
             FG_ERR_PIPELINE);
     // gradient replay

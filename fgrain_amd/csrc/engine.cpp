@@ -68,7 +68,8 @@ extern "C" void fg_engine_destroy(fg_engine *E)
     for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->out, &E->out2,
                       &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
                       &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
-                      &E->gmap, &E->partials, &E->pgrad, &E->gout})
+                      &E->gmap, &E->partials, &E->pgrad, &E->gout, &E->boost, &E->dboost,
+                      &E->gboost})
         b->release();
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
@@ -182,6 +183,8 @@ static void free_device(Emulsion &em)
         em.dev->counts.release();
         em.dev->exp_nodes.release();
         em.dev->exp_recs.release();
+        em.dev->exp_ptr.release();
+        em.dev->exp_idx.release();
         delete em.dev;
         em.dev = nullptr;
     }

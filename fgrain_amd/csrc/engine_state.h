@@ -52,11 +52,13 @@ struct DeviceEmulsion {
     DevBuf nodes, recs, counts;
     uint32_t n_nodes = 0;
     int64_t n = 0;
-    // expected-model scan BVH: every crystal, semi-axes times exp_fe, record sens words
-    // holding transport slots (built on first use for a given expansion)
-    DevBuf exp_nodes, exp_recs;
+    // expected-model scan BVH: every crystal, boxes for the largest boosted expansion,
+    // records with semi-axes times exp_fe and sens words holding transport slots; touch
+    // graph (CSR, slot order) for the expected clump boost.  Built on first use for a
+    // given (expansion, proximity factor).
+    DevBuf exp_nodes, exp_recs, exp_ptr, exp_idx;
     uint32_t exp_n_nodes = 0;
-    double exp_fe = -1.0;
+    double exp_fe = -1.0, exp_pf = -1.0;
 };
 }  // namespace fg
 
@@ -72,7 +74,8 @@ struct fg_engine {
     DevBuf emap, stats, scan_nodes, scan_recs, out, out2, rays_o, rays_d, hits_n, hits_i,
         hits_t0, hits_t1;
     // expected model and its adjoint
-    DevBuf dose64, dose32, prob, dprob, gprob, gdose, gmap, partials, pgrad, gout;
+    DevBuf dose64, dose32, prob, dprob, gprob, gdose, gmap, partials, pgrad, gout, boost, dboost,
+        gboost;
 };
 
 struct fg_emulsion {

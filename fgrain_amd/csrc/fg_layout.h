@@ -142,6 +142,49 @@ FG_HD bool crystal_hit(const float c[3], const float q[4], const float h[3], uin
     return false;
 }
 
+// Chord of a crystal whose semi-axes are scaled by s (about its centre) and the chord's
+// derivative with respect to s.  The chord is crystal_hit's on the scaled record; the
+// derivative is the closed form in the crystal frame: for the box, the exit and (when
+// the ray starts outside) entry planes move by h_k / |d_k| per unit s; for the
+// ellipsoid, with P = l / h and Q = d / h in the unit frame, the roots are
+// (-P.Q -+ sqrt((P.Q)^2 - Q.Q (P.P - s^2))) / Q.Q, so each root the chord uses moves
+// by s / sqrt(...).
+FG_HD bool crystal_chord_scaled(const float c[3], const float q[4], const float h[3], uint32_t morph,
+                                float s, float ox, float oy, float oz, float dx, float dy, float dz,
+                                float &chord, float &dchord)
+{
+    const float hs[3] = {h[0] * s, h[1] * s, h[2] * s};
+    float t0, t1;
+    if (!crystal_hit(c, q, hs, morph, ox, oy, oz, dx, dy, dz, t0, t1)) return false;
+    chord = fmaxf(t1 - t0, 0.0f);
+    float lx, ly, lz, ex, ey, ez;
+    qinv_rotate(q, ox - c[0], oy - c[1], oz - c[2], lx, ly, lz);
+    qinv_rotate(q, dx, dy, dz, ex, ey, ez);
+    if (morph == 0u) {
+        const float l[3] = {lx, ly, lz}, e[3] = {ex, ey, ez};
+        float tmin = -INFINITY, tmax = INFINITY, gmin = 0.0f, gmax = 0.0f;
+        for (int k = 0; k < 3; k++) {
+            const float inv = 1.0f / e[k];
+            const float a1 = (-hs[k] - l[k]) * inv, a2 = (hs[k] - l[k]) * inv;
+            const float lo = fminf(a1, a2), hi = fmaxf(a1, a2);
+            const float g = h[k] * fabsf(inv);
+            if (lo > tmin) { tmin = lo; gmin = g; }
+            if (hi < tmax) { tmax = hi; gmax = g; }
+        }
+        dchord = gmax + (tmin > 0.0f ? gmin : 0.0f);
+        if (!(dchord < INFINITY)) dchord = 0.0f;
+        return true;
+    }
+    const float Px = lx / h[0], Py = ly / h[1], Pz = lz / h[2];
+    const float Qx = ex / h[0], Qy = ey / h[1], Qz = ez / h[2];
+    const float QQ = Qx * Qx + Qy * Qy + Qz * Qz, PQ = Px * Qx + Py * Qy + Pz * Qz;
+    const float PP = Px * Px + Py * Py + Pz * Pz;
+    const float r = sqrtf(fmaxf(PQ * PQ - QQ * (PP - s * s), 0.0f));
+    const float enter = (-PQ - r) / QQ;
+    dchord = r > 0.0f ? (enter > 0.0f ? 2.0f : 1.0f) * s / r : 0.0f;
+    return true;
+}
+
 // AABB::intersect_ray (bvh.rs:96-124)
 FG_HD bool box_hit(const float lo[3], const float hi[3], float ox, float oy, float oz,
                    float ix, float iy, float iz)

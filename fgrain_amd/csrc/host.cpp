@@ -464,3 +464,56 @@ void develop_host(Emulsion &em, const fg_development_config &cfg, fg_development
 }
 
 }  // namespace fg
+
+namespace fg {
+
+void touch_graph(const Emulsion &em, double fe, double pf, std::vector<uint32_t> &ptr,
+                 std::vector<uint32_t> &idx)
+{
+    const int64_t n = em.n;
+    ptr.assign((size_t)n + 1, 0u);
+    idx.clear();
+    if (n == 0) return;
+    std::vector<double> mdim((size_t)n);
+    double maxb = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        const double *d = &em.dims[(size_t)(3 * i)];
+        mdim[(size_t)i] = fmax(fmax(d[0], d[1]), d[2]);
+        maxb = fmax(maxb, mdim[(size_t)i]);
+    }
+    const double cell = std::max((2.0 * maxb * fe) * pf, 1e-9);
+    std::unordered_map<int64_t, std::vector<int64_t>> grid;
+    auto key = [](int64_t x, int64_t y, int64_t z) {
+        return (x + (1 << 20)) | ((y + (1 << 20)) << 21) | ((z + (1 << 20)) << 42);
+    };
+    for (int64_t i = 0; i < n; i++) {
+        const double *p = &em.pos[(size_t)(3 * i)];
+        grid[key((int64_t)floor(p[0] / cell), (int64_t)floor(p[1] / cell), (int64_t)floor(p[2] / cell))].push_back(i);
+    }
+    std::vector<uint32_t> row;
+    for (int64_t i = 0; i < n; i++) {
+        const double *p = &em.pos[(size_t)(3 * i)];
+        const int64_t gx = (int64_t)floor(p[0] / cell), gy = (int64_t)floor(p[1] / cell),
+                      gz = (int64_t)floor(p[2] / cell);
+        const double ri = mdim[(size_t)i] * fe;
+        row.clear();
+        for (int64_t z = gz - 1; z <= gz + 1; z++)
+            for (int64_t y = gy - 1; y <= gy + 1; y++)
+                for (int64_t x = gx - 1; x <= gx + 1; x++) {
+                    auto itc = grid.find(key(x, y, z));
+                    if (itc == grid.end()) continue;
+                    for (int64_t j : itc->second) {
+                        if (j == i) continue;
+                        const double touch = (ri + mdim[(size_t)j] * fe) * pf;
+                        const double *q = &em.pos[(size_t)(3 * j)];
+                        const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+                        if (dx * dx + dy * dy + dz * dz <= touch * touch) row.push_back((uint32_t)j);
+                    }
+                }
+        std::sort(row.begin(), row.end());
+        idx.insert(idx.end(), row.begin(), row.end());
+        ptr[(size_t)i + 1] = (uint32_t)idx.size();
+    }
+}
+
+}  // namespace fg

@@ -66,6 +66,11 @@ void make_records(const Emulsion &em, const RefBvh &b, const uint8_t *developed,
 bool finalize_emulsion(Emulsion &em, std::string &msg);
 float round_down_f(double x);
 float round_up_f(double x);
+// Touch graph of development.rs:132-155 with unboosted expansion fe: j is a neighbour of
+// i when |p_i - p_j| <= (m_i + m_j) * fe * pf (m = largest semi-axis); CSR in crystal
+// order, rows sorted.
+void touch_graph(const Emulsion &em, double fe, double pf, std::vector<uint32_t> &ptr,
+                 std::vector<uint32_t> &idx);
 // threshold + clumping on the host from em.counts (union-find over a uniform grid)
 void develop_host(Emulsion &em, const fg_development_config &cfg, fg_development_report &rep);
 

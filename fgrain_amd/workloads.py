@@ -83,8 +83,34 @@ def blob_exposure(w: int, h: int, seed: int, blobs=8, sigma=(200.0, 800.0), amp=
     return (lo + (hi - lo) * l8 / 255.0).astype(np.float32)
 
 
+def smooth_field(w: int, h: int, seed: int, lo: float, hi: float, bumps: int = 8) -> np.ndarray:
+    """Seeded smooth field: a sum of Gaussian bumps (widths 10-35 % of the larger side)
+    rescaled to [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    y = np.arange(h, dtype=np.float64)[:, None]
+    x = np.arange(w, dtype=np.float64)[None, :]
+    f = np.zeros((h, w))
+    for _ in range(bumps):
+        cx, cy = rng.uniform(0, w), rng.uniform(0, h)
+        s = rng.uniform(0.10, 0.35) * max(w, h)
+        f += rng.uniform(0.3, 1.0) * (np.exp(-((y - cy) ** 2) / (2 * s * s)) *
+                                      np.exp(-((x - cx) ** 2) / (2 * s * s)))
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-300)
+    return (lo + (hi - lo) * f).astype(np.float32)
+
+
+def packet_weight(cfg: dict) -> float:
+    """Photon weight per packet at relative exposure 1: photons per px^2 over packets per
+    pixel (configs whose exposure names photons_per_px2; else 1)."""
+    x = cfg["exposure"]
+    d = x.get("photons_per_px2")
+    return float(d) / x["photons_per_pixel"] if d else 1.0
+
+
 def workload_exposure(cfg: dict) -> np.ndarray:
     fr, x = cfg["frame"], cfg["exposure"]
+    if x["kind"] == "smooth_field":
+        return smooth_field(fr["width"], fr["height"], cfg["seed"], x["lo"], x["hi"], x.get("bumps", 8))
     if x["kind"] == "ramp_x":
         return ramp_exposure(fr["width"], fr["height"], x["lo"], x["hi"])
     if x["kind"] == "blobs":
@@ -100,6 +126,17 @@ def density_multipliers(cfg: dict, n_tiles: int):
         return None
     rng = np.random.default_rng(cfg["seed"] + 1)
     return rng.uniform(dm["low"], dm["high"], n_tiles).astype(np.float32)
+
+
+def expected_model(cfg: dict, w: int, h: int, seed: int, mode=ScanMode.NegativeDensity):
+    """ExpectedModel of one tile of the workload with tile seed `seed` (transport seed + 1,
+    scan seed + 2, as the per-tile path)."""
+    from .adjoint import ExpectedModel
+    e = cfg["emulsion"]["optics"]
+    optics = GelatinOptics(e["refractive_index"], e["absorption_coeff"], e["scattering_coeff"],
+                           e["anisotropy_g"], e["antihalation_reflectance"], e["antihalation_density"])
+    return ExpectedModel(exposure_config(cfg), optics, development_config(cfg), scan_config(cfg, w, h, mode),
+                         expose_seed=(seed + 1) % (1 << 64), scan_seed=(seed + 2) % (1 << 64))
 
 
 class Frame:

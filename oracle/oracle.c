@@ -1407,6 +1407,57 @@ int orc_antihalation(double r_ahu, double od, uint64_t seed, double *energy, dou
     return antihalation(&X, energy, d, &rng);
 }
 
+/* Chord of crystal record r with semi-axes scaled by s and d chord / d s.  The chord is
+ * crystal_intersect on the scaled record; the derivative: box -- the exit plane (and the
+ * entry plane when the ray starts outside) moves by dims_k / |d_k| per unit s; ellipsoid
+ * -- in the unit frame (P = l / dims, Q = d / dims) the roots are
+ * (-P.Q -+ sqrt((P.Q)^2 - Q.Q (P.P - s^2))) / Q.Q, each moving by s / sqrt(...). */
+static int chord_scaled_d(const ocrys *r, double s, const double o[3], const double d[3],
+                          double *chord, double *dchord)
+{
+    const double hs[3] = {r->dims[0] * s, r->dims[1] * s, r->dims[2] * s};
+    double t0, t1;
+    if (!crystal_intersect_d(r->c, r->q, hs, r->morph, o, d, &t0, &t1)) return 0;
+    *chord = fmax(t1 - t0, 0.0);
+    double rel[3] = {o[0] - r->c[0], o[1] - r->c[1], o[2] - r->c[2]}, l[3], e[3];
+    qinv_rotate_d(r->q, rel, l);
+    qinv_rotate_d(r->q, d, e);
+    if (r->morph == 0) {
+        double tmin = -INFINITY, tmax = INFINITY, gmin = 0.0, gmax = 0.0;
+        for (int k = 0; k < 3; k++) {
+            const double inv = 1.0 / e[k];
+            const double a1 = (-hs[k] - l[k]) * inv, a2 = (hs[k] - l[k]) * inv;
+            const double lo = fmin(a1, a2), hi = fmax(a1, a2);
+            const double g = r->dims[k] * fabs(inv);
+            if (lo > tmin) { tmin = lo; gmin = g; }
+            if (hi < tmax) { tmax = hi; gmax = g; }
+        }
+        *dchord = gmax + (tmin > 0.0 ? gmin : 0.0);
+        if (!(*dchord < INFINITY)) *dchord = 0.0;
+        return 1;
+    }
+    double P[3], Q[3];
+    for (int k = 0; k < 3; k++) {
+        P[k] = l[k] / r->dims[k];
+        Q[k] = e[k] / r->dims[k];
+    }
+    const double QQ = Q[0] * Q[0] + Q[1] * Q[1] + Q[2] * Q[2];
+    const double PQ = P[0] * Q[0] + P[1] * Q[1] + P[2] * Q[2];
+    const double PP = P[0] * P[0] + P[1] * P[1] + P[2] * P[2];
+    const double rt = sqrt(fmax(PQ * PQ - QQ * (PP - s * s), 0.0));
+    const double enter = (-PQ - rt) / QQ;
+    *dchord = rt > 0.0 ? (enter > 0.0 ? 2.0 : 1.0) * s / rt : 0.0;
+    return 1;
+}
+
+int orc_chord_scaled(const orc_crystals *cs, int64_t i, double s, const double o[3],
+                     const double d[3], double out[2])
+{
+    ocrys r;
+    make_rec(cs, i, &r);
+    return chord_scaled_d(&r, s, o, d, &out[0], &out[1]);
+}
+
 /* =====================================================================================
  * Expected (differentiable) model -- no reference counterpart.  FP64 twin of the
  * library's path replay: the packets of the photon-keyed sampler (sampler 1, ppp > 0)
@@ -1712,6 +1763,7 @@ void orc_poisson_tail(const double *lam, int64_t n, uint32_t thr, double *p, dou
 typedef struct {
     tscan S;                 /* geometry, rays, seed */
     const double *prob;      /* per crystal */
+    const double *boost;     /* expected clump boost per crystal */
     const double *gT;        /* adjoint: dL/dT per pixel, or NULL */
     double *T;               /* forward output */
     double *gprob;           /* adjoint: per thread n, folded after */
@@ -1720,10 +1772,10 @@ typedef struct {
 
 typedef struct {
     const ocrys *cr;
-    const double *prob;
+    const double *prob, *boost;
     double o[3], d[3], inv[3];
     double sigma, prod, coef;
-    double *gprob;           /* NULL: product pass */
+    double *gprob, *gboost;  /* NULL: product pass */
 } exphit;
 
 static inline int ex_box(const onode *nd, exphit *h)
@@ -1732,14 +1784,17 @@ static inline int ex_box(const onode *nd, exphit *h)
 }
 static inline void ex_prim(uint32_t id, exphit *h)
 {
-    const ocrys *r = &h->cr[id];
-    double t0, t1;
-    if (!crystal_intersect_d(r->c, r->q, r->dims, r->morph, h->o, h->d, &t0, &t1)) return;
-    const double a = 1.0 - exp(-h->sigma * fmax(t1 - t0, 0.0));
+    double c, dc;
+    if (!chord_scaled_d(&h->cr[id], h->boost[id], h->o, h->d, &c, &dc)) return;
+    const double ex = exp(-h->sigma * c);
+    const double a = 1.0 - ex;
     const double pr = h->prob[id];
     const double f = 1.0 - pr * a;
     if (h->gprob) {
-        if (f > 0.0) h->gprob[id] += -a * h->coef / f;
+        if (f > 0.0) {
+            h->gprob[id] += -a * h->coef / f;
+            h->gboost[id] += -pr * h->sigma * ex * dc * h->coef / f;
+        }
     } else {
         h->prod *= f;
     }
@@ -1747,7 +1802,7 @@ static inline void ex_prim(uint32_t id, exphit *h)
 
 typedef struct {
     texpscan *X;
-    double *gprob;
+    double *gprob, *gboost;
 } expscan_arg;
 
 static void *expscan_worker(void *arg)
@@ -1773,6 +1828,7 @@ static void *expscan_worker(void *arg)
                             memset(&h, 0, sizeof h);
                             h.cr = S->cr;
                             h.prob = X->prob;
+                            h.boost = X->boost;
                             h.o[0] = ox;
                             h.o[1] = oy;
                             h.o[2] = S->D + 0.1;
@@ -1785,6 +1841,7 @@ static void *expscan_worker(void *arg)
                             if (X->gT) {
                                 h.coef = gT / nrays * h.prod;
                                 h.gprob = a->gprob;
+                                h.gboost = a->gboost;
                                 if (S->b->nn > 0) DFS_TRAVERSE(S->b, ex_box, ex_prim, &h, &cnt);
                             }
                             acc += h.prod;
@@ -1795,23 +1852,29 @@ static void *expscan_worker(void *arg)
     return NULL;
 }
 
-/* Expected scan over every crystal with semi-axes times fe; gT == NULL: T per pixel
- * (mean ray product); else gprob = sum over rays of (gT / rays) * dT_ray / dprob. */
-int orc_expected_scan(const orc_crystals *cs, const double *prob, const orc_expected_cfg *e,
-                      const double *gT, double *T, double *gprob)
+/* Expected scan over every crystal with semi-axes times fe * boost (boxes for the largest
+ * boost, fe * 2.05); gT == NULL: T per pixel (mean ray product); else gprob and gboost =
+ * sum over rays of (gT / rays) * dT_ray / d(prob, boost). */
+int orc_expected_scan(const orc_crystals *cs, const double *prob, const double *boost,
+                      const orc_expected_cfg *e, const double *gT, double *T, double *gprob,
+                      double *gboost)
 {
     const int64_t n = cs->n;
     uint8_t *dev = (uint8_t *)malloc((size_t)(n > 0 ? n : 1));
     double *fe = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *fbox = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
     for (int64_t i = 0; i < n; i++) {
         dev[i] = 1;
         fe[i] = e->fe;
+        fbox[i] = e->fe * 2.05;
     }
-    orc_crystals dc = *cs;
+    orc_crystals dc = *cs, db = *cs;
     dc.developed = dev;
     dc.fe = fe;
+    db.developed = dev;
+    db.fe = fbox;
     obvh b;
-    bvh_build(&dc, 1, &b);
+    bvh_build(&db, 1, &b);
     ocrys *cr = make_recs(&dc);
     texpscan X;
     memset(&X, 0, sizeof X);
@@ -1832,6 +1895,7 @@ int orc_expected_scan(const orc_crystals *cs, const double *prob, const orc_expe
     S->ph = (float)(e->x.H / (double)e->h_px);
     S->seed = e->scan_seed;
     X.prob = prob;
+    X.boost = boost;
     X.gT = gT;
     X.T = T;
     X.n = n;
@@ -1840,22 +1904,82 @@ int orc_expected_scan(const orc_crystals *cs, const double *prob, const orc_expe
     expscan_arg *args = (expscan_arg *)calloc((size_t)nt, sizeof(expscan_arg));
     for (int t = 0; t < nt; t++) {
         args[t].X = &X;
-        if (gT) args[t].gprob = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+        if (gT) {
+            args[t].gprob = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+            args[t].gboost = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+        }
         pthread_create(&th[t], NULL, expscan_worker, &args[t]);
     }
     for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
     if (gT) {
         memset(gprob, 0, sizeof(double) * (size_t)n);
+        memset(gboost, 0, sizeof(double) * (size_t)n);
         for (int t = 0; t < nt; t++) {
-            for (int64_t i = 0; i < n; i++) gprob[i] += args[t].gprob[i];
+            for (int64_t i = 0; i < n; i++) {
+                gprob[i] += args[t].gprob[i];
+                gboost[i] += args[t].gboost[i];
+            }
             free(args[t].gprob);
+            free(args[t].gboost);
         }
     }
     free(args);
     free(th);
     free(cr);
     bvh_free(&b);
+    free(fbox);
     free(fe);
     free(dev);
     return 0;
+}
+
+/* Touch graph of development.rs:132-155 at unboosted expansion fe: j ~ i when
+ * |p_i - p_j| <= (m_i + m_j) fe pf, m the largest semi-axis.  CSR (ptr n + 1, idx);
+ * returns the number of entries, or minus it when cap is too small. */
+int64_t orc_touch_graph(const orc_crystals *cs, double fe, double pf, int64_t *ptr, int32_t *idx,
+                        int64_t cap)
+{
+    const int64_t n = cs->n;
+    double maxm = 0.0;
+    for (int64_t i = 0; i < n; i++) maxm = fmax(maxm, max_dim(cs, i));
+    const double r = 2.0 * maxm * fe * pf;
+    orc_crystals plain = *cs;
+    plain.developed = NULL;
+    plain.fe = NULL;
+    obvh b;
+    bvh_build(&plain, 0, &b);
+    int64_t ncap = 1024, k = 0;
+    int32_t *nb = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncap);
+    ptr[0] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const double *p = cs->pos + 3 * i;
+        int64_t m = query_radius(&b, cs, p, r, nb, ncap);
+        if (m > ncap) {
+            ncap = m * 2;
+            nb = (int32_t *)realloc(nb, sizeof(int32_t) * (size_t)ncap);
+            m = query_radius(&b, cs, p, r, nb, ncap);
+        }
+        /* sorted neighbour list */
+        for (int64_t a = 1; a < m; a++)
+            for (int64_t c = a; c > 0 && nb[c - 1] > nb[c]; c--) {
+                const int32_t t = nb[c];
+                nb[c] = nb[c - 1];
+                nb[c - 1] = t;
+            }
+        for (int64_t t = 0; t < m; t++) {
+            const int64_t j = nb[t];
+            if (j == i) continue;
+            const double touch = (max_dim(cs, i) + max_dim(cs, j)) * fe * pf;
+            const double dx = cs->pos[3 * j] - p[0], dy = cs->pos[3 * j + 1] - p[1],
+                         dz = cs->pos[3 * j + 2] - p[2];
+            if (dx * dx + dy * dy + dz * dz <= touch * touch) {
+                if (k < cap) idx[k] = (int32_t)j;
+                k++;
+            }
+        }
+        ptr[i + 1] = k;
+    }
+    free(nb);
+    bvh_free(&b);
+    return k <= cap ? k : -k;
 }

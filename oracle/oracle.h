@@ -125,6 +125,7 @@ typedef struct {
     double cut;             /* relative weight cutoff of a replayed packet (0 = 1e-6) */
     double fe;              /* filament expansion of the expected-scan geometry */
     double sigma;           /* silver opacity */
+    double pf;              /* clumping proximity factor of the touch graph */
     int32_t w_px, h_px, spp, ss;
     uint64_t scan_seed;
 } orc_expected_cfg;
@@ -134,9 +135,17 @@ int orc_expected_dose(const orc_crystals *cs, const float *map, const orc_expect
                       double *dose, int64_t *bounces_events);
 /* P[Poisson(dose) >= thr] and d/d(dose) */
 void orc_poisson_tail(const double *dose, int64_t n, uint32_t thr, double *p, double *dp);
-/* gT == NULL: T (w_px * h_px mean ray products); else gprob (n) for dL/dT = gT */
-int orc_expected_scan(const orc_crystals *cs, const double *prob, const orc_expected_cfg *e,
-                      const double *gT, double *T, double *gprob);
+/* semi-axes times fe * boost per crystal; gT == NULL: T (w_px * h_px mean ray products);
+ * else gprob, gboost (n) for dL/dT = gT */
+int orc_expected_scan(const orc_crystals *cs, const double *prob, const double *boost,
+                      const orc_expected_cfg *e, const double *gT, double *T, double *gprob,
+                      double *gboost);
+/* touch graph at unboosted expansion (CSR); returns entries, or -entries if cap is short */
+int64_t orc_touch_graph(const orc_crystals *cs, double fe, double pf, int64_t *ptr, int32_t *idx,
+                        int64_t cap);
+/* chord of crystal i scaled by s about its centre and d chord / d s */
+int orc_chord_scaled(const orc_crystals *cs, int64_t i, double s, const double o[3],
+                     const double d[3], double out[2]);
 /* gradient replay: gmap (tw * th) = dL/dE, grads = dL/d(mu_abs, g, R) for dL/ddose = gdose */
 int orc_replay_grad(const orc_crystals *cs, const float *map, const orc_expected_cfg *e,
                     const double *gdose, double *gmap, double grads[3]);

@@ -70,7 +70,7 @@ class OrcSynthCfg(C.Structure):
 
 class OrcExpectedCfg(C.Structure):
     _fields_ = [("x", OrcExposeCfg), ("cut", C.c_double), ("fe", C.c_double),
-                ("sigma", C.c_double), ("w_px", C.c_int32), ("h_px", C.c_int32),
+                ("sigma", C.c_double), ("pf", C.c_double), ("w_px", C.c_int32), ("h_px", C.c_int32),
                 ("spp", C.c_int32), ("ss", C.c_int32), ("scan_seed", C.c_uint64)]
 
 
@@ -121,8 +121,12 @@ def lib():
         L.orc_expected_dose.argtypes = [C.POINTER(OrcCrystals), _f32p, C.POINTER(OrcExpectedCfg),
                                         _f64p, _i64p]
         L.orc_poisson_tail.argtypes = [_f64p, C.c_int64, C.c_uint32, _f64p, _f64p]
-        L.orc_expected_scan.argtypes = [C.POINTER(OrcCrystals), _f64p, C.POINTER(OrcExpectedCfg),
-                                        _f64p, _f64p, _f64p]
+        L.orc_expected_scan.argtypes = [C.POINTER(OrcCrystals), _f64p, _f64p, C.POINTER(OrcExpectedCfg),
+                                        _f64p, _f64p, _f64p, _f64p]
+        L.orc_touch_graph.argtypes = [C.POINTER(OrcCrystals), C.c_double, C.c_double, _i64p, _i32p,
+                                      C.c_int64]
+        L.orc_touch_graph.restype = C.c_int64
+        L.orc_chord_scaled.argtypes = [C.POINTER(OrcCrystals), C.c_int64, C.c_double, _f64p, _f64p, _f64p]
         L.orc_replay_grad.argtypes = [C.POINTER(OrcCrystals), _f32p, C.POINTER(OrcExpectedCfg),
                                       _f64p, _f64p, _f64p]
         L.orc_hg_score.argtypes = [C.c_double, C.c_double]
@@ -340,11 +344,11 @@ def antihalation(r_ahu, od, seed, energy, d):
 
 def expected_cfg(exposure_shape, *, W, H, D, mu_a, mu_s, g, n_refr, r_ahu, od_ahu, mu_abs,
                  theta_max, max_bounces, ppp, seed, fe, sigma, w_px, h_px, spp=4, ss=1,
-                 scan_seed=0, cut=1e-6, base_thickness=0.0, threads=0) -> OrcExpectedCfg:
+                 scan_seed=0, cut=1e-6, base_thickness=0.0, threads=0, pf=1.25) -> OrcExpectedCfg:
     th, tw = exposure_shape
     x = OrcExposeCfg(W, H, D, mu_a, mu_s, g, n_refr, r_ahu, od_ahu, mu_abs, theta_max, 0.0,
                      max_bounces, 1, 0, ppp, threads, seed, tw, th, 0, base_thickness)
-    return OrcExpectedCfg(x, cut, fe, sigma, w_px, h_px, spp, ss, scan_seed)
+    return OrcExpectedCfg(x, cut, fe, sigma, pf, w_px, h_px, spp, ss, scan_seed)
 
 
 def poisson_tail(dose, threshold):
@@ -354,6 +358,41 @@ def poisson_tail(dose, threshold):
     dp = np.zeros_like(d)
     lib().orc_poisson_tail(_p(d, _f64p), d.size, threshold, _p(p, _f64p), _p(dp, _f64p))
     return p, dp
+
+
+def touch_graph(cs: Crystals, fe, pf):
+    """Touch graph of development.rs:132-155 at unboosted expansion: scipy CSR matrix."""
+    from scipy.sparse import csr_matrix
+    cc = cs.c()
+    ptr = np.zeros(cs.n + 1, np.int64)
+    cap = max(16 * cs.n, 16)
+    while True:
+        idx = np.zeros(cap, np.int32)
+        k = lib().orc_touch_graph(C.byref(cc), fe, pf, _p(ptr, _i64p), _p(idx, _i32p), cap)
+        if k >= 0:
+            break
+        cap = -k
+    return csr_matrix((np.ones(k), idx[:k], ptr), shape=(cs.n, cs.n))
+
+
+def clump_boost(s):
+    """Expected clump boost for expected clump size s (development.rs:164-171: a clump of
+    size s >= 2 boosts by 1.15 + 0.18 (min(s, 6) - 1); 1 for s <= 1, linear between) and
+    its derivative."""
+    s = np.asarray(s, np.float64)
+    b = np.where(s <= 1, 1.0, np.where(s <= 2, 1.0 + 0.33 * (s - 1),
+                                       1.15 + 0.18 * (np.minimum(s, 6.0) - 1)))
+    db = np.where(s <= 1, 0.0, np.where(s <= 2, 0.33, np.where(s < 6, 0.18, 0.0)))
+    return b, db
+
+
+def chord_scaled(cs: Crystals, i, s, o, d):
+    out = np.zeros(2)
+    cc = cs.c()
+    o = np.ascontiguousarray(o, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    hit = lib().orc_chord_scaled(C.byref(cc), i, s, _p(o, _f64p), _p(d, _f64p), _p(out, _f64p))
+    return (out[0], out[1]) if hit else None
 
 
 def mode_map(T, mode):
@@ -379,45 +418,67 @@ def mode_map_grad(T, mode):
     return np.where(T > 1e-4, -1.0 / (np.maximum(T, 1e-300) * ln10), 0.0)
 
 
-def expected_forward(cs: Crystals, exposure_map, cfg: OrcExpectedCfg, threshold, mode):
-    """-> dict(dose, prob, dprob, T, out): the expected model in FP64."""
+def expected_scan(cs: Crystals, prob, boost, cfg: OrcExpectedCfg, gT=None):
+    """T (h, w) for gT None, else (gprob, gboost)."""
+    cc = cs.c()
+    prob = np.ascontiguousarray(prob, np.float64)
+    boost = np.ascontiguousarray(boost, np.float64)
+    if gT is None:
+        T = np.zeros((cfg.h_px, cfg.w_px), np.float64)
+        lib().orc_expected_scan(C.byref(cc), _p(prob, _f64p), _p(boost, _f64p), C.byref(cfg), None,
+                                _p(T, _f64p), None, None)
+        return T
+    gT = np.ascontiguousarray(gT, np.float64)
+    gp = np.zeros(max(cs.n, 1))
+    gb = np.zeros(max(cs.n, 1))
+    lib().orc_expected_scan(C.byref(cc), _p(prob, _f64p), _p(boost, _f64p), C.byref(cfg), _p(gT, _f64p),
+                            None, _p(gp, _f64p), _p(gb, _f64p))
+    return gp[:cs.n], gb[:cs.n]
+
+
+def _dev_stage(cs, dose, cfg, threshold, graph):
+    prob, dprob = poisson_tail(dose, threshold)
+    if graph is None:
+        graph = touch_graph(cs, cfg.fe, cfg.pf)
+    boost, dboost = clump_boost(1.0 + graph @ prob)
+    return prob, dprob, boost, dboost, graph
+
+
+def expected_forward(cs: Crystals, exposure_map, cfg: OrcExpectedCfg, threshold, mode, graph=None):
+    """-> dict(dose, prob, dprob, boost, T, out): the expected model in FP64."""
     em = np.ascontiguousarray(exposure_map, np.float32)
     cc = cs.c()
     dose = np.zeros(max(cs.n, 1), np.float64)
     be = np.zeros(2, np.int64)
     lib().orc_expected_dose(C.byref(cc), _p(em, _f32p), C.byref(cfg), _p(dose, _f64p), _p(be, _i64p))
     dose = dose[:cs.n]
-    prob, dprob = poisson_tail(dose, threshold)
-    T = np.zeros((cfg.h_px, cfg.w_px), np.float64)
-    lib().orc_expected_scan(C.byref(cc), _p(np.ascontiguousarray(prob), _f64p), C.byref(cfg), None,
-                            _p(T, _f64p), None)
-    return {"dose": dose, "prob": prob, "dprob": dprob, "T": T, "out": mode_map(T, mode),
-            "bounces": int(be[0]), "events": int(be[1])}
+    prob, dprob, boost, dboost, graph = _dev_stage(cs, dose, cfg, threshold, graph)
+    T = expected_scan(cs, prob, boost, cfg)
+    return {"dose": dose, "prob": prob, "dprob": dprob, "boost": boost, "T": T,
+            "out": mode_map(T, mode), "bounces": int(be[0]), "events": int(be[1]), "graph": graph}
 
 
 def expected_vjp(cs: Crystals, exposure_map, cfg: OrcExpectedCfg, threshold, mode, grad_out,
-                 dose=None):
+                 dose=None, graph=None):
     """-> dict(grad_exposure, grads [mu_abs, g, R], grad_dose, grad_prob) in FP64, for
     dL/dout = grad_out; dose from a forward (recomputed when None)."""
     em = np.ascontiguousarray(exposure_map, np.float32)
     cc = cs.c()
     if dose is None:
-        dose = expected_forward(cs, em, cfg, threshold, mode)["dose"]
+        dose = expected_forward(cs, em, cfg, threshold, mode, graph)["dose"]
     dose = np.ascontiguousarray(dose, np.float64)
-    prob, dprob = poisson_tail(dose, threshold)
-    T = np.zeros((cfg.h_px, cfg.w_px), np.float64)
-    lib().orc_expected_scan(C.byref(cc), _p(prob, _f64p), C.byref(cfg), None, _p(T, _f64p), None)
-    gT = np.ascontiguousarray(np.asarray(grad_out, np.float64) * mode_map_grad(T, mode))
-    gprob = np.zeros(max(cs.n, 1), np.float64)
-    lib().orc_expected_scan(C.byref(cc), _p(prob, _f64p), C.byref(cfg), _p(gT, _f64p), None,
-                            _p(gprob, _f64p))
-    gprob = gprob[:cs.n]
+    prob, dprob, boost, dboost, graph = _dev_stage(cs, dose, cfg, threshold, graph)
+    T = expected_scan(cs, prob, boost, cfg)
+    gT = np.asarray(grad_out, np.float64) * mode_map_grad(T, mode)
+    gprob, gboost = expected_scan(cs, prob, boost, cfg, gT)
+    gprob = gprob + graph.T @ (gboost * dboost)
     gdose = np.ascontiguousarray(gprob * dprob)
     gmap = np.zeros(em.shape, np.float64)
     grads = np.zeros(3, np.float64)
     lib().orc_replay_grad(C.byref(cc), _p(em, _f32p), C.byref(cfg), _p(gdose, _f64p),
                           _p(gmap, _f64p), _p(grads, _f64p))
-    return {"grad_exposure": gmap, "grads": grads, "grad_dose": gdose, "grad_prob": gprob}
+    return {"grad_exposure": gmap, "grads": grads, "grad_dose": gdose, "grad_prob": gprob,
+            "grad_boost": gboost}
 
 
 def hg_score(g, mu):

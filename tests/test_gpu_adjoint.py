@@ -48,7 +48,8 @@ def _orc_cfg(E, m: ExpectedModel, w, h):
                           r_ahu=o.antihalation_reflectance, od_ahu=o.antihalation_density,
                           mu_abs=m.exposure.crystal_absorption_coeff, theta_max=0.12, max_bounces=48,
                           ppp=m.exposure.photons_per_pixel, seed=m.expose_seed, fe=1.2,
-                          sigma=0.1041666667, w_px=w, h_px=h, spp=4, ss=1, scan_seed=m.scan_seed)
+                          sigma=0.1041666667, w_px=w, h_px=h, spp=4, ss=1, scan_seed=m.scan_seed,
+                          pf=m.development.clumping_proximity_factor)
 
 
 @pytest.fixture(scope="module")
@@ -137,9 +138,11 @@ def test_pathwise_gradients_match_central_differences(engine, tile):
 
 
 def test_hg_score_gradient_agrees_with_finite_differences(engine):
-    """Score-function dL/dg against central differences (h = 3e-2 relative) over seeds."""
+    """Score-function dL/dg against central differences (h = 3e-2 relative) over seeds,
+    at enough packets per crystal that the loss of the expected dose (what the score
+    estimate differentiates) and the expected loss (what differences see) agree."""
     w = h = 128
-    ppp = 32
+    ppp = 256
     em = F.VirtualEmulsionLayer.synthesize(trix_config(w, h, r_ahu=0.14), 8)
     E = _field(h, w, ppp, seed=1)
     tgt = _target(engine, em, E, w, h, ppp)

@@ -94,51 +94,94 @@ def test_single_cube_dose_and_absorption_tangent_closed_form():
 
 
 def test_expected_scan_single_cube_closed_form():
-    """T = 1 - p (1 - exp(-sigma 2 a fe)) inside the expanded footprint, 1 outside;
-    dT/dp = -(1 - exp(-sigma 2 a fe)) for a pixel fully inside."""
+    """T = 1 - p (1 - exp(-sigma 2 a fe b)) inside the boosted footprint, 1 outside;
+    dT/dp = -(1 - exp(-sigma 2 a fe b)) and dT/db = -p sigma 2 a fe exp(-sigma 2 a fe b)
+    for a pixel fully inside."""
     a, fe, sig, p = 2.0, 1.2, 0.3, 0.37
     cs = _one([10, 10, 5], a)
     E = np.ones((20, 20), np.float32)
     cfg = O.expected_cfg(E.shape, W=20.0, H=20.0, D=10.0, mu_a=0.0, mu_s=0.0, g=0.0, n_refr=1.0,
                          r_ahu=0.0, od_ahu=0.0, mu_abs=1.0, theta_max=0.0, max_bounces=1, ppp=1,
                          seed=1, fe=fe, sigma=sig, w_px=20, h_px=20, spp=4)
-    import ctypes as C
-    cc = cs.c()
-    prob = np.array([p])
-    T = np.zeros((20, 20))
-    O.lib().orc_expected_scan(C.byref(cc), O._p(prob, O._f64p), C.byref(cfg), None, O._p(T, O._f64p), None)
-    inside = 1 - p * (1 - np.exp(-sig * 2 * a * fe))
-    assert T[10, 10] == pytest.approx(inside, rel=1e-12) and T[0, 0] == 1.0
-    gT = np.zeros((20, 20))
-    gT[10, 10] = 1.0
-    gp = np.zeros(1)
-    O.lib().orc_expected_scan(C.byref(cc), O._p(prob, O._f64p), C.byref(cfg), O._p(gT, O._f64p), None,
-                              O._p(gp, O._f64p))
-    assert gp[0] == pytest.approx(-(1 - np.exp(-sig * 2 * a * fe)), rel=1e-12)
+    for b in (1.0, 1.3):
+        T = O.expected_scan(cs, [p], [b], cfg)
+        L = 2 * a * fe * b
+        assert T[10, 10] == pytest.approx(1 - p * (1 - np.exp(-sig * L)), rel=1e-12) and T[0, 0] == 1.0
+        gT = np.zeros((20, 20))
+        gT[10, 10] = 1.0
+        gp, gb = O.expected_scan(cs, [p], [b], cfg, gT)
+        assert gp[0] == pytest.approx(-(1 - np.exp(-sig * L)), rel=1e-12)
+        assert gb[0] == pytest.approx(-p * sig * 2 * a * fe * np.exp(-sig * L), rel=1e-12)
+    # one crystal has no neighbours: no boost
+    assert O.touch_graph(cs, fe, 1.25).nnz == 0
 
 
-def test_scan_adjoint_matches_finite_differences_of_prob():
+def test_scan_adjoint_matches_finite_differences_of_prob_and_boost():
     cs = _tile(32)
     E = _field(32, 32, 8)
     cfg = _cfg(E, 32, 8)
     rng = np.random.default_rng(1)
     prob = rng.uniform(0.05, 0.95, cs.n)
-    import ctypes as C
-    cc = cs.c()
-
-    def T_of(pr):
-        T = np.zeros((32, 32))
-        O.lib().orc_expected_scan(C.byref(cc), O._p(np.ascontiguousarray(pr), O._f64p), C.byref(cfg), None,
-                                  O._p(T, O._f64p), None)
-        return T
+    boost = rng.uniform(1.0, 2.0, cs.n)
     gT = rng.standard_normal((32, 32))
-    gp = np.zeros(cs.n)
-    O.lib().orc_expected_scan(C.byref(cc), O._p(prob, O._f64p), C.byref(cfg), O._p(gT, O._f64p), None,
-                              O._p(gp, O._f64p))
+    gp, gb = O.expected_scan(cs, prob, boost, cfg, gT)
     d = rng.standard_normal(cs.n)
     h = 1e-6
-    fd = ((T_of(prob + h * d) - T_of(prob - h * d)) * gT).sum() / (2 * h)
+    fd = ((O.expected_scan(cs, prob + h * d, boost, cfg) - O.expected_scan(cs, prob - h * d, boost, cfg)) * gT).sum() / (2 * h)
     assert fd == pytest.approx(gp @ d, rel=1e-6)
+    fd = ((O.expected_scan(cs, prob, boost + h * d, cfg) - O.expected_scan(cs, prob, boost - h * d, cfg)) * gT).sum() / (2 * h)
+    assert fd == pytest.approx(gb @ d, rel=1e-5)
+
+
+@pytest.mark.parametrize("morph", [0, 1])
+def test_chord_scale_derivative_against_finite_differences(morph):
+    """d chord / d s of a rotated crystal scaled about its centre, for rays from outside
+    and from inside."""
+    rng = np.random.default_rng(7 + morph)
+    n = 40
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    dims = rng.uniform(0.5, 2.0, (n, 3))
+    cs = O.Crystals(np.zeros((n, 3)), dims, q, np.full(n, morph, np.int32), np.ones(n))
+    checked = 0
+    for i in range(n):
+        for inside in (False, True):
+            d = rng.standard_normal(3)
+            d /= np.linalg.norm(d)
+            o = (rng.uniform(-0.3, 0.3, 3) if inside else -6.0 * d + rng.uniform(-0.6, 0.6, 3))
+            s = 1.3
+            r = O.chord_scaled(cs, i, s, o, d)
+            rp, rm = O.chord_scaled(cs, i, s + 1e-6, o, d), O.chord_scaled(cs, i, s - 1e-6, o, d)
+            if r is None or rp is None or rm is None:
+                continue
+            assert r[1] == pytest.approx((rp[0] - rm[0]) / 2e-6, rel=1e-5, abs=1e-7)
+            checked += 1
+    assert checked > 40
+
+
+def test_touch_graph_against_brute_force():
+    cs = _tile(28)
+    fe, pf = 1.32, 1.35
+    A = O.touch_graph(cs, fe, pf).toarray() > 0
+    m = cs.dims.max(axis=1)
+    dist = np.linalg.norm(cs.pos[:, None, :] - cs.pos[None, :, :], axis=2)
+    ref = dist <= (m[:, None] + m[None, :]) * fe * pf
+    np.fill_diagonal(ref, False)
+    assert (A == ref).all() and (A == A.T).all() and A.sum() > cs.n
+
+
+def test_clump_boost_follows_the_reference_boost_at_integer_sizes():
+    """development.rs:164-171: 1.15 + 0.18 (min(s, 6) - 1) for a clump of s >= 2."""
+    s = np.array([2, 3, 4, 5, 6, 7, 20], float)
+    b, _ = O.clump_boost(s)
+    assert np.allclose(b, 1.15 + 0.18 * (np.minimum(s, 6) - 1))
+    b, db = O.clump_boost(np.array([0.0, 1.0, 1.5, 2.0 - 1e-12, 2.0 + 1e-12]))
+    assert np.allclose(b, [1.0, 1.0, 1.165, 1.33, 1.33])
+    x = np.linspace(1.01, 7.9, 300)
+    bb, dd = O.clump_boost(x)
+    fd = (O.clump_boost(x + 1e-7)[0] - O.clump_boost(x - 1e-7)[0]) / 2e-7
+    ok = (np.abs(x - 2) > 1e-3) & (np.abs(x - 6) > 1e-3)
+    assert np.allclose(fd[ok], dd[ok], atol=1e-6)
 
 
 @pytest.fixture(scope="module")
@@ -203,8 +246,10 @@ def test_hg_score_gradient_agrees_with_finite_differences_on_average():
     """g enters through the scattering directions, so central differences of one sample
     jump where paths cross the slab faces; over seeds the score-function gradient and
     h = 3e-2 central differences estimate the same derivative (within their standard
-    errors, measured over the seeds)."""
-    w, ppp = 48, 64
+    errors, measured over the seeds).  The score estimate differentiates the loss of the
+    expected dose; finite differences, the expected loss of the sampled dose: the two
+    differ by O(1 / packets per crystal), so the packet count is kept high."""
+    w, ppp = 48, 256
     cs = _tile(w)
     E = _field(w, w, ppp)
     tgt = O.expected_forward(cs, E * np.float32(2 ** 0.3), _cfg(E, w, ppp, seed=99), 4, 3)["out"]

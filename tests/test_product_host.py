@@ -227,3 +227,28 @@ def test_workload_exposures():
     r = W.ramp_exposure(320, 2, 0.01, 4.0)
     assert r[0, 0] == np.float32(0.01) and r[1, -1] == np.float32(4.0)
     assert cfg["frame"]["width"] == 6000 and cfg["exposure"]["photons_per_pixel"] == 128
+
+
+def test_fit_tile_seeds_follow_the_tiled_renderer():
+    """The fit's per-tile seeds are the frame path's (image_pipeline.rs:430-434)."""
+    from fgrain_amd.fit import tile_seed
+    cfg = W.load_config("frame24mp_trix_cubic")
+    fr = W.Frame.__new__(W.Frame)
+    fr.seed, fr.tile, fr.width, fr.height = cfg["seed"], 320, 1000, 700
+    seeds = [t[4] for t in W.Frame.tile_ranges(fr)]
+    idx = 0
+    ours = []
+    for ty in range(3):
+        for tx in range(4):
+            ours.append(tile_seed(cfg["seed"], idx, tx, ty))
+            idx += 1
+    assert seeds == ours
+
+
+def test_smooth_field_range_and_packet_weight():
+    f = W.smooth_field(64, 48, 3, 0.05, 3.0, 6)
+    assert f.shape == (48, 64) and f.min() == pytest.approx(0.05) and f.max() == pytest.approx(3.0)
+    assert np.array_equal(f, W.smooth_field(64, 48, 3, 0.05, 3.0, 6))
+    cfg = W.load_config("tile512_trix_adjoint")
+    assert W.packet_weight(cfg) == pytest.approx(0.10417 / 256)
+    assert W.packet_weight(W.load_config("tile320_trix_cubic")) == 1.0
