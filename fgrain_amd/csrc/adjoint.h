@@ -9,18 +9,6 @@
 
 namespace fg {
 
-struct ReplayArgs {
-    TransportArgs t;       // geometry, optics, exposure map, keys (t.counts unused)
-    double *dose;          // dose pass: expected dose accumulators, transport slot order
-    const float *gdose;    // gradient pass: dL/d(dose) per transport slot
-    double *gmap;          // gradient pass: dL/dE accumulators (tw * th)
-    double *partials;      // gradient pass: per block (mu_abs, g, R) sums, 4 doubles each
-    unsigned long long *stats;  // packets, bounces, events
-    float hal_unit;        // 10^(-2 OD): t.hal_eff = R * hal_unit
-    float refl;            // antihalation reflectance R
-    float cut;             // a packet ends once its relative weight and R tangent are below
-    float pad;
-};
 
 // Expected-scan parameters: the BVH holds every crystal with boxes for the largest
 // boosted expansion, records with semi-axes scaled by the unboosted filament expansion;
@@ -41,6 +29,11 @@ struct ExpScanArgs {
 };
 
 int replay_blocks(uint64_t n_photons);
+// Persistent replay over resident blocks (grid = CUs x replay_persistent_blocks_per_cu());
+// work must point to a zeroed 64-bit counter; the gradient pass writes grid partials.
+int replay_persistent_blocks_per_cu();
+hipError_t launch_replay_persistent_dose(const ReplayArgs &a, unsigned long long *work, int grid, hipStream_t s);
+hipError_t launch_replay_persistent_grad(const ReplayArgs &a, unsigned long long *work, int grid, hipStream_t s);
 hipError_t launch_replay_dose(const ReplayArgs &a, hipStream_t s);
 hipError_t launch_replay_grad(const ReplayArgs &a, hipStream_t s);
 // prob = P[Poisson(dose) >= thr] and dprob/ddose, from the float dose

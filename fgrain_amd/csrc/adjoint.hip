@@ -16,6 +16,8 @@
 // result beyond the last double bit).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "adjoint.h"
 #include "device_common.h"
 
@@ -29,12 +31,6 @@ struct ReplayAcc {
     float e = 0.f, mu = 0.f, g = 0.f, r = 0.f;
     unsigned long long bounces = 0, events = 0;
 };
-
-// Henyey-Greenstein score: d/dg log p_g(mu), p_g(mu) = (1 - g^2) / (2 (1 + g^2 - 2 g mu)^1.5)
-__device__ __forceinline__ float hg_score(float g, float ct)
-{
-    return -2.0f * g / (1.0f - g * g) + 3.0f * (ct - g) / (1.0f + g * g - 2.0f * g * ct);
-}
 
 template <bool kGrad>
 __device__ __forceinline__ void replay_photon(const ReplayArgs &R, uint64_t p, ReplayAcc &acc)
@@ -234,6 +230,52 @@ __global__ void __launch_bounds__(256) replay_grad_kernel(ReplayArgs R)
     if (R.stats) flush_replay_stats(acc, R.stats);
     const double m = wave_sum_d((double)acc.mu), g = wave_sum_d((double)acc.g),
                  r = wave_sum_d((double)acc.r);
+    const uint32_t wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = m;
+        red[wv][1] = g;
+        red[wv][2] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const uint32_t j = threadIdx.x;
+        R.partials[4 * (size_t)blockIdx.x + j] = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
+    }
+    if (threadIdx.x == 3) R.partials[4 * (size_t)blockIdx.x + 3] = 0.0;
+}
+
+// Persistent replay (lanes take the next packet as soon as theirs ends; the path of
+// every packet is that of replay_photon).  Block partial sums as replay_grad_kernel.
+__global__ void __launch_bounds__(256) replay_persistent_dose_kernel(ReplayArgs R, unsigned long long *work)
+{
+    PhotonCounters cnt;
+    transport_persistent<false, kReplayDose>(R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work,
+                                             R.t.n_photons, cnt, &R, nullptr);
+    if (R.stats) {
+        const unsigned long long b = wave_sum(cnt.bounces), e = wave_sum(cnt.abs);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&R.stats[1], b);
+            atomicAdd(&R.stats[2], e);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) replay_persistent_grad_kernel(ReplayArgs R, unsigned long long *work)
+{
+    __shared__ double red[4][3];
+    PhotonCounters cnt;
+    ReplaySums sums;
+    transport_persistent<false, kReplayGrad>(R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work,
+                                             R.t.n_photons, cnt, &R, &sums);
+    if (R.stats) {
+        const unsigned long long b = wave_sum(cnt.bounces), e = wave_sum(cnt.abs);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&R.stats[1], b);
+            atomicAdd(&R.stats[2], e);
+        }
+    }
+    const double m = wave_sum_d((double)sums.mu), g = wave_sum_d((double)sums.g),
+                 r = wave_sum_d((double)sums.r);
     const uint32_t wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[wv][0] = m;
@@ -487,6 +529,32 @@ hipError_t launch_replay_grad(const ReplayArgs &a, hipStream_t s)
 {
     if (a.t.n_photons == 0) return hipSuccess;
     replay_grad_kernel<<<blocks_for(a.t.n_photons), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+int replay_persistent_blocks_per_cu()
+{
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void *>(replay_persistent_dose_kernel),
+                                                     256, 0) != hipSuccess)
+        a = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void *>(replay_persistent_grad_kernel),
+                                                     256, 0) != hipSuccess)
+        b = 1;
+    return std::max(1, std::min(a, b));
+}
+
+hipError_t launch_replay_persistent_dose(const ReplayArgs &a, unsigned long long *work, int grid, hipStream_t s)
+{
+    if (a.t.n_photons == 0) return hipSuccess;
+    replay_persistent_dose_kernel<<<grid, 256, 0, s>>>(a, work);
+    return hipGetLastError();
+}
+
+hipError_t launch_replay_persistent_grad(const ReplayArgs &a, unsigned long long *work, int grid, hipStream_t s)
+{
+    if (a.t.n_photons == 0) return hipSuccess;
+    replay_persistent_grad_kernel<<<grid, 256, 0, s>>>(a, work);
     return hipGetLastError();
 }
 

@@ -162,13 +162,26 @@ fg_status setup(fg_engine *E, Emulsion &em, const fg_expected_model &m, const fl
     return FG_OK;
 }
 
+// The replay kernel the tuning selects (persistent lanes unless one thread per packet
+// is asked for); the 64-bit work counter is stats slot 6, zeroed before each pass.
+bool replay_persistent(fg_engine *E)
+{
+    if (E->tuning.transport_kernel == 1) return false;
+    if (E->replay_grid == 0) E->replay_grid = E->cus * replay_persistent_blocks_per_cu();
+    return true;
+}
+
 // expected dose (slot order) into dose32, then prob and dprob
 fg_status dose_pass(fg_engine *E, Setup &S, const fg_expected_model &m, float *ms)
 {
     if (S.n > 0) HIP_TRY(hipMemsetAsync(E->dose64.p, 0, (size_t)S.n * sizeof(double), E->stream), FG_ERR_BUFFER_MAP);
     S.r.dose = E->dose64.as<double>();
     HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
-    HIP_TRY(launch_replay_dose(S.r, E->stream), FG_ERR_PIPELINE);
+    if (replay_persistent(E))
+        HIP_TRY(launch_replay_persistent_dose(S.r, E->stats.as<unsigned long long>() + 6, E->replay_grid, E->stream),
+                FG_ERR_PIPELINE);
+    else
+        HIP_TRY(launch_replay_dose(S.r, E->stream), FG_ERR_PIPELINE);
     HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
     HIP_TRY(launch_dev_prob(E->dose64.as<double>(), E->dose32.as<float>(), E->prob.as<float>(),
                             E->dprob.as<float>(), S.n, m.development.threshold_photons, E->stream),
@@ -295,7 +308,8 @@ extern "C" fg_status fg_vjp(fg_engine *E, const fg_emulsion *emw, const fg_expec
     HIP_TRY(launch_dose_grad(E->gprob.as<double>(), E->dprob.as<float>(), E->gdose.as<float>(), S.n, E->stream),
             FG_ERR_PIPELINE);
     // gradient replay
-    const uint32_t blocks = (uint32_t)replay_blocks(S.r.t.n_photons);
+    const bool persistent = replay_persistent(E);
+    const uint32_t blocks = persistent ? (uint32_t)E->replay_grid : (uint32_t)replay_blocks(S.r.t.n_photons);
     HIP_TRY(E->gmap.ensure(S.nmap * sizeof(double)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(E->partials.ensure(std::max<size_t>(blocks, 1) * 4 * sizeof(double)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(E->pgrad.ensure(4 * sizeof(double)), FG_ERR_BUFFER_RANGE);
@@ -307,7 +321,11 @@ extern "C" fg_status fg_vjp(fg_engine *E, const fg_emulsion *emw, const fg_expec
     S.r.gmap = E->gmap.as<double>();
     S.r.partials = E->partials.as<double>();
     HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
-    HIP_TRY(launch_replay_grad(S.r, E->stream), FG_ERR_PIPELINE);
+    if (persistent)
+        HIP_TRY(launch_replay_persistent_grad(S.r, E->stats.as<unsigned long long>() + 6, E->replay_grid, E->stream),
+                FG_ERR_PIPELINE);
+    else
+        HIP_TRY(launch_replay_grad(S.r, E->stream), FG_ERR_PIPELINE);
     HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
     HIP_TRY(launch_reduce_partials(E->partials.as<double>(), blocks, E->pgrad.as<double>(), E->stream),
             FG_ERR_PIPELINE);

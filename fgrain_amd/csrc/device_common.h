@@ -105,6 +105,17 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
+// Henyey-Greenstein score: d/dg log p_g(mu), p_g(mu) = (1 - g^2) / (2 (1 + g^2 - 2 g mu)^1.5)
+__device__ __forceinline__ float hg_score(float g, float ct)
+{
+    return -2.0f * g / (1.0f - g * g) + 3.0f * (ct - g) / (1.0f + g * g - 2.0f * g * ct);
+}
+
+// Per-lane sums of a gradient replay: d/d(mu_abs), d/dg, d/dR.
+struct ReplaySums {
+    float mu = 0.f, g = 0.f, r = 0.f;
+};
+
 struct PhotonCounters {
     unsigned long long bounces = 0, nodes = 0, prims = 0, abs = 0, tot = 0;
 };
@@ -357,13 +368,20 @@ __device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32
 
 enum : uint32_t { kNeed = 0, kStart = 1, kWalk = 2, kDone = 3 };
 
-template <bool kFrame>
+// kMode 0: latent counts (the transport); 1: expected-dose replay; 2: gradient replay
+// (single tile only), with R the replay outputs and sums the lane's parameter sums.
+enum : int { kCounts = 0, kReplayDose = 1, kReplayGrad = 2 };
+
+template <bool kFrame, int kMode = kCounts>
 __device__ __forceinline__ void transport_persistent(const TransportArgs &C, const TileDev *tiles,
                                                      uint32_t n_tiles, const Node *nodes,
                                                      const float4 *recs, uint32_t *counts,
                                                      const float *emap, unsigned long long *work,
-                                                     uint64_t total, PhotonCounters &cnt)
+                                                     uint64_t total, PhotonCounters &cnt,
+                                                     const ReplayArgs *R = nullptr,
+                                                     ReplaySums *sums = nullptr)
 {
+    static_assert(kMode == kCounts || !kFrame, "replay runs on one tile");
     const uint32_t lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t chunk = max(C.chunk, 64u);
@@ -377,8 +395,22 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     float step = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, h0 = 0.f, h1 = 0.f, h2 = 0.f;
     uint32_t b = 0, i = 0, leaf = 0;
+    // replay: exposure of the packet's pixel, relative weight tangents, HG score and the
+    // packet's gradient sums (flushed when the lane takes its next packet)
+    float E = 0.f, w_mu = 0.f, w_r = 0.f, score = 0.f, a_e = 0.f, a_mu = 0.f, a_g = 0.f, a_r = 0.f;
+    size_t e_idx = 0;
+    bool active = false;
     const bool prune = C.reach >= 0.0f;
     for (;;) {
+        if constexpr (kMode == kReplayGrad) {
+            if (phase == kNeed && active) {
+                if (a_e != 0.0f) atomicAdd(&R->gmap[e_idx], (double)a_e);
+                sums->mu += a_mu;
+                sums->g += a_g;
+                sums->r += a_r;
+                active = false;
+            }
+        }
         // ---- hand out packets to lanes without one
         const unsigned long long nm = __ballot(phase == kNeed);
         if (nm) {
@@ -420,9 +452,11 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                         const float vv = fminf(fmaxf(y / T.H, 0.0f), 1.0f);
                         const uint32_t ex = min((uint32_t)roundf(uu * (float)(T.tw - 1)), T.tw - 1);
                         const uint32_t ey = min((uint32_t)roundf(vv * (float)(T.th - 1)), T.th - 1);
-                        w = T.emap[(size_t)ey * C.estride + ex];
+                        e_idx = (size_t)ey * C.estride + ex;
+                        w = T.emap[e_idx];
                     }
                     w = fmaxf(w, 0.0f);
+                    E = w;
                     if (w > 0.0f) {
                         const float th = rng.u() * C.theta_max;
                         const float ph = rng.u() * kTwoPi;
@@ -440,6 +474,12 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                             oz = 0.001f;
                             b = 0;
                             phase = kStart;
+                            if constexpr (kMode != kCounts) {
+                                w = 1.0f;
+                                w_mu = w_r = score = 0.0f;
+                                a_e = a_mu = a_g = a_r = 0.0f;
+                                active = true;
+                            }
                         }
                     }
                 }
@@ -504,17 +544,36 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 const float L = fmaxf(fminf(t1, step) - t0, 0.0f);
                 if (!(L > 0.0f)) continue;
                 const float pa = 1.0f - __expf(-C.mu_abs * L);
-                const float lam = w * pa * r.sens * 5.0f;
-                const float fl = fminf(floorf(lam), 4294967040.0f);
-                const uint32_t whole = fl > 0.0f ? (uint32_t)fl : 0u;
-                const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
-                const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
-                if (k_lat > 0u) {
-                    atomicAdd(&T.counts[k], k_lat);
+                if constexpr (kMode == kCounts) {
+                    const float lam = w * pa * r.sens * 5.0f;
+                    const float fl = fminf(floorf(lam), 4294967040.0f);
+                    const uint32_t whole = fl > 0.0f ? (uint32_t)fl : 0u;
+                    const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
+                    const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
+                    if (k_lat > 0u) {
+                        atomicAdd(&T.counts[k], k_lat);
+                        cnt.abs++;
+                        cnt.tot += k_lat;
+                    }
+                    w *= fmaxf(1.0f - pa * 0.9f, 0.05f);
+                } else {
                     cnt.abs++;
-                    cnt.tot += k_lat;
+                    if constexpr (kMode == kReplayGrad) {
+                        const float base = 5.0f * r.sens * R->gdose[k];
+                        const float dep = w * pa;
+                        a_e += base * dep;
+                        a_mu += base * E * (w_mu * pa + w * L * (1.0f - pa));
+                        a_r += base * E * (w_r * pa);
+                        a_g += base * E * dep * score;
+                    } else {
+                        atomicAdd(&R->dose[k], (double)(((E * w) * pa) * r.sens * 5.0f));
+                    }
+                    const float m = 1.0f - pa * 0.9f;
+                    const float dm = -0.9f * L * (1.0f - pa);
+                    w_mu = w_mu * m + w * dm;
+                    w_r = w_r * m;
+                    w = w * m;
                 }
-                w *= fmaxf(1.0f - pa * 0.9f, 0.05f);
             }
             leaf = 0u;
         }
@@ -529,10 +588,16 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 oz = C.D - 0.001f;
                 const float inx = dx, iny = dy, inz = dz;
                 phase = kStart;
-                w *= C.hal_eff;
-                if (w < 1e-4f) {
-                    if (rng.u() < 0.1f) w /= 0.1f;
-                    else phase = kNeed;
+                if constexpr (kMode == kCounts) {
+                    w *= C.hal_eff;
+                    if (w < 1e-4f) {
+                        if (rng.u() < 0.1f) w /= 0.1f;
+                        else phase = kNeed;
+                    }
+                } else {
+                    w_r = w_r * C.hal_eff + w * R->hal_unit;
+                    w_mu = w_mu * C.hal_eff;
+                    w = w * C.hal_eff;
                 }
                 if (phase == kStart) {
                     if (rng.u() < 0.4f) {
@@ -546,7 +611,11 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                         dz = -fabsf(dz);
                         normalize3(dx, dy, dz);
                     }
-                    if (!(w > 0.0f)) phase = kNeed;
+                    if constexpr (kMode == kCounts) {
+                        if (!(w > 0.0f)) phase = kNeed;
+                    } else {
+                        if (w < R->cut && w_r < R->cut) phase = kNeed;
+                    }
                     if (phase == kStart && C.base > 0.0f) {
                         if (!(inz > 1e-6f) || !(-dz > 1e-6f)) {
                             phase = kNeed;
@@ -570,6 +639,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     ct = (1.0f + C.g * C.g - term * term) / (2.0f * C.g);
                 }
                 ct = fminf(fmaxf(ct, -1.0f), 1.0f);
+                if constexpr (kMode == kReplayGrad) score += hg_score(C.g, ct);
                 const float stt = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
                 const float phi = kTwoPi * xi2;
                 float wx = dx, wy = dy, wz = dz;
@@ -586,9 +656,15 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 normalize3(dx, dy, dz);
                 w *= C.albedo;
                 phase = kStart;
-                if (w < 1e-3f) {
-                    if (rng.u() < 0.1f) w /= 0.1f;
-                    else phase = kNeed;
+                if constexpr (kMode == kCounts) {
+                    if (w < 1e-3f) {
+                        if (rng.u() < 0.1f) w /= 0.1f;
+                        else phase = kNeed;
+                    }
+                } else {
+                    w_mu *= C.albedo;
+                    w_r *= C.albedo;
+                    if (w < R->cut && w_r < R->cut) phase = kNeed;
                 }
             }
         }

@@ -40,6 +40,21 @@ struct TransportArgs {
     float pad2;
 };
 
+// Path replay of the expected model (adjoint.hip): the transport's packets without the
+// rounding and roulette draws, depositing expected dose or accumulating gradients.
+struct ReplayArgs {
+    TransportArgs t;       // geometry, optics, exposure map, keys (t.counts unused)
+    double *dose;          // dose pass: expected dose accumulators, transport slot order
+    const float *gdose;    // gradient pass: dL/d(dose) per transport slot
+    double *gmap;          // gradient pass: dL/dE accumulators (tw * th)
+    double *partials;      // gradient pass: per block (mu_abs, g, R) sums, 4 doubles each
+    unsigned long long *stats;  // packets, bounces, events
+    float hal_unit;        // 10^(-2 OD): t.hal_eff = R * hal_unit
+    float refl;            // antihalation reflectance R
+    float cut;             // a packet ends once its relative weight and R tangent are below
+    float pad;
+};
+
 // Per-tile descriptor shared by host and device.
 struct TileDev {
     uint32_t x0, y0, tw, th;        // frame placement (pixels)

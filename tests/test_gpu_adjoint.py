@@ -210,3 +210,29 @@ def test_model_validation_errors(engine, tile):
         ExpectedFilm(em, m, engine).forward(E)
     with pytest.raises(ValueError):
         ExpectedFilm(em, _model(w, h, ppp), engine).vjp(E, np.zeros((3, 3), np.float32))
+
+
+def test_persistent_replay_equals_thread_per_packet(engine, tile):
+    """Both replay kernels follow identical packet paths; only the order of the double
+    accumulations differs."""
+    em, E, w, h, ppp = tile
+    m = _model(w, h, ppp)
+    G = np.random.default_rng(5).standard_normal((h, w)).astype(np.float32)
+    res = []
+    old = engine.tuning()
+    try:
+        for kernel in (0, 1):
+            engine.set_tuning(transport_kernel=kernel)
+            film = ExpectedFilm(em, m, engine)
+            out, dose = film.forward(E)
+            st = dict(film.last_stats)
+            gE, pg, gd = film.vjp(E, G, dose=dose, want_grad_dose=True)
+            res.append((out, dose, gE, pg.array(), gd, st))
+    finally:
+        engine.set_tuning(**old)
+    a, b = res
+    assert np.allclose(a[1], b[1], rtol=1e-6, atol=1e-6)
+    assert np.abs(a[0] - b[0]).max() <= 1e-5
+    assert np.allclose(a[2], b[2], rtol=1e-5, atol=1e-12)
+    assert np.allclose(a[3], b[3], rtol=1e-5)
+    assert a[5]["bounces"] == b[5]["bounces"] and a[5]["events"] == b[5]["events"]
