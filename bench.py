@@ -12,6 +12,7 @@ workload prints its own line:
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|batched]
   python bench.py --workload fit1024_trix1600_dye3 [--steps K]
+  python bench.py --workload tile512_trix_adjoint [--steps K]
 """
 from __future__ import annotations
 
@@ -137,6 +138,20 @@ def adjoint_leg(engine, name="tile512_trix_adjoint", reps=3):
     }
 
 
+def adjoint_main(args, engine):
+    """Differentiation workload alone: one JSON line with the VJP / forward time ratio."""
+    r = adjoint_leg(engine, args.workload, reps=args.steps)
+    out = {
+        "metric": "adjoint/forward time ratio (expected-model VJP by path replay over its forward)",
+        "value": r["adjoint_forward_ratio"], "unit": "ratio", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": r["ms_forward"] + r["ms_vjp"], "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "packets": r["packets"], "crystals": r["crystals"]},
+    }
+    out.update({k: v for k, v in r.items() if k not in ("workload", "packets", "crystals")})
+    print(json.dumps(out), flush=True)
+
+
 def fit_main(args, engine, cfg):
     """Inverse fit workload: one JSON line with time per step, the adjoint/forward
     ratio and the plain-forward density RMSE."""
@@ -208,6 +223,8 @@ def main():
     log(f"engine: {engine.adapter_name()}")
     if "fit" in cfg:
         return fit_main(args, engine, cfg)
+    if "loss" in cfg:
+        return adjoint_main(args, engine)
     t = time.perf_counter()
     exposure = W.workload_exposure(cfg)
     frame = W.Frame(engine, cfg, exposure)
