@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--workload", default="frame24mp_trix_cubic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="ignore fgrain_amd/configs/tuning/<workload>.json (autotuner output)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (counter calibration)")
     args = ap.parse_args()
@@ -218,9 +220,15 @@ def main():
     import fgrain_amd as F
     from fgrain_amd import workloads as W
 
+    from fgrain_amd.autotune import load_tuning
+
     cfg = W.load_config(args.workload)
     engine = F.GpuEngine.new()
     log(f"engine: {engine.adapter_name()}")
+    tuned = None if args.no_tuning else load_tuning(args.workload)
+    if tuned:
+        engine.set_tuning(**tuned["tuning"])
+        log(f"tuning ({tuned['measured']}): {tuned['tuning']}")
     if "fit" in cfg:
         return fit_main(args, engine, cfg)
     if "loss" in cfg:
@@ -299,6 +307,7 @@ def main():
             "bytes": dev_bytes / photons, "nodes": agg["nodes_visited"] / photons,
             "prims": agg["prims_tested"] / photons, "bounces": agg["bounces"] / photons},
         "transport_per_path_reference": per_path,
+        "tuning": engine.tuning(),
     }
     if args.mode == "batched" and not args.no_plain:
         log("plain per-tile comparison step ...")

@@ -20,6 +20,7 @@ struct ExpScanArgs {
     uint32_t w_px, h_px, g, ss, mode;
     float pw, ph, z0, sigma;
     uint32_t key0, key1;
+    uint32_t rays_per_lane, pad;   // scan shape (scan_lanes)
     const float *prob;     // development probability per transport slot
     const float *boost;    // expected clump boost of the expansion per transport slot
     float *out;            // forward: w_px * h_px

@@ -445,7 +445,7 @@ template <bool kAdjoint>
 __global__ void __launch_bounds__(256) expected_scan_kernel(ExpScanArgs A)
 {
     const uint32_t Rn = A.g * A.g * A.ss * A.ss;
-    const uint32_t L = scan_lanes(Rn);
+    const uint32_t L = scan_lanes(Rn, A.rays_per_lane);
     const uint32_t pix = blockIdx.x * (256u / L) + threadIdx.x / L;
     const bool valid = pix < A.w_px * A.h_px;
     const uint32_t px = valid ? pix % A.w_px : 0u, py = valid ? pix / A.w_px : 0u;
@@ -576,7 +576,7 @@ hipError_t launch_dev_prob_from(const float *dose32, float *prob, float *dprob, 
 
 static uint32_t expected_scan_grid(const ExpScanArgs &a)
 {
-    const uint32_t L = scan_lanes(a.g * a.g * a.ss * a.ss);
+    const uint32_t L = scan_lanes(a.g * a.g * a.ss * a.ss, a.rays_per_lane);
     const uint64_t px = (uint64_t)a.w_px * a.h_px;
     return (uint32_t)((px + (256u / L) - 1u) / (256u / L));
 }

@@ -157,6 +157,7 @@ fg_status setup(fg_engine *E, Emulsion &em, const fg_expected_model &m, const fl
     s.sigma = (float)m.scan.silver_opacity;
     s.key0 = (uint32_t)m.scan_seed;
     s.key1 = (uint32_t)(m.scan_seed >> 32);
+    s.rays_per_lane = E->tuning.scan_rays_per_thread;
     s.prob = E->prob.as<float>();
     s.boost = E->boost.as<float>();
     return FG_OK;

@@ -17,7 +17,7 @@ struct DevelopParams {
 };
 
 struct ScanCommon {
-    uint32_t g, ss, mode, pad;
+    uint32_t g, ss, mode, rays_per_lane;
     float z0, sigma;
 };
 

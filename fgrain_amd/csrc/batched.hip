@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, 
 {
     const uint32_t t = tile_of_block(B.tiles, n_tiles, blockIdx.x);
     const TileDev &T = B.tiles[t];
-    const uint32_t L = scan_lanes(S.g * S.g * S.ss * S.ss);
+    const uint32_t L = scan_lanes(S.g * S.g * S.ss * S.ss, S.rays_per_lane);
     const uint32_t pix = (blockIdx.x - T.block_base) * (256u / L) + threadIdx.x / L;
     const bool valid = pix < T.tw * T.th;
     const uint32_t px = valid ? pix % T.tw : 0u, py = valid ? pix / T.tw : 0u;
@@ -600,7 +600,8 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
         items += (T.n_photons + 63) / 64;
         T.photon_base = b->total_photons;
         b->total_photons += T.n_photons;
-        const uint32_t ppb = 256u / scan_lanes(in.scan.g * in.scan.g * in.scan.ss * in.scan.ss);
+        const uint32_t ppb = 256u / scan_lanes(in.scan.g * in.scan.g * in.scan.ss * in.scan.ss,
+                                               in.scan.rays_per_lane);
         T.blocks_x = 0;
         T.blocks = (uint32_t)(((uint64_t)T.tw * T.th + ppb - 1) / ppb);
         T.block_base = blocks;

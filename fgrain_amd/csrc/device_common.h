@@ -719,10 +719,17 @@ __device__ __forceinline__ void scan_origin(uint32_t pix, uint32_t px, uint32_t 
     oy = ((float)py + apy) * ph;
 }
 
-// Lanes per pixel of the scan kernels: all rays of a pixel in adjacent lanes when their
-// number divides the wavefront, else one lane per pixel.
-FG_HD uint32_t scan_lanes(uint32_t rays)
+// Lanes per pixel of the scan kernels: each lane takes rays_per_lane of the pixel's rays
+// (1: one lane per ray, the default), the pixel's lanes adjacent in the wave; a shape
+// that does not tile the wavefront falls back to one lane per ray when the rays divide
+// the wavefront, else one lane per pixel.
+FG_HD uint32_t scan_lanes(uint32_t rays, uint32_t rays_per_lane = 1u)
 {
+    const uint32_t r = rays_per_lane ? rays_per_lane : 1u;
+    if (rays % r == 0u) {
+        const uint32_t L = rays / r;
+        if (L <= 64u && (64u % L) == 0u) return L;
+    }
     return (rays <= 64u && (64u % rays) == 0u) ? rays : 1u;
 }
 

@@ -84,8 +84,9 @@ extern "C" const char *fg_engine_adapter_name(const fg_engine *E) { return E ? E
 extern "C" fg_status fg_engine_set_tuning(fg_engine *E, const fg_tuning *t)
 {
     if (!E || !t) return fail(FG_ERR_INVALID_ARG, "fg_engine_set_tuning: null argument");
-    if (t->transport_kernel > 1 || t->transport_wait_pct > 100)
-        return fail(FG_ERR_INVALID_ARG, "fg_engine_set_tuning: transport_kernel must be 0 or 1, wait_pct <= 100");
+    if (t->transport_kernel > 1 || t->transport_wait_pct > 100 || t->scan_rays_per_thread > 64)
+        return fail(FG_ERR_INVALID_ARG,
+                    "fg_engine_set_tuning: transport_kernel must be 0 or 1, wait_pct <= 100, scan_rays_per_thread <= 64");
     std::lock_guard<std::mutex> lk(E->mu);
     E->tuning.transport_kernel = t->transport_kernel;
     E->tuning.transport_chunk = t->transport_chunk ? std::max(t->transport_chunk, 64u) : 64u;
@@ -479,6 +480,7 @@ static fg_status scan_locked(fg_engine *E, const Emulsion &em, const fg_scan_con
     a.key1 = (uint32_t)(seed >> 32);
     a.out = E->out.as<float>();
     a.ostride = cfg.width_px;
+    a.rays_per_lane = E->tuning.scan_rays_per_thread;
     HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
     HIP_TRY(launch_scan(a, E->stream), FG_ERR_PIPELINE);
     HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
@@ -911,6 +913,7 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
         s.key1 = (uint32_t)((t.seed + 2) >> 32);
         s.out = F->d_dneg.as<float>() + (size_t)t.y0 * D.width + t.x0;
         s.ostride = D.width;
+        s.rays_per_lane = E->tuning.scan_rays_per_thread;
         HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
         HIP_TRY(launch_scan(s, E->stream), FG_ERR_PIPELINE);
         HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
@@ -1001,6 +1004,7 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
         in.scan.g = std::max((uint32_t)llround(sqrt((double)spp)), 1u);
         in.scan.ss = std::max(D.scan.supersample, 1u);
         in.scan.mode = D.scan.mode;
+        in.scan.rays_per_lane = E->tuning.scan_rays_per_thread;
         in.scan.z0 = (float)(D.emulsion.depth + 0.1);
         in.scan.sigma = (float)D.scan.silver_opacity;
         if ((uint64_t)F->n_crystals * 2 >= 0xFFFFFFFFull)

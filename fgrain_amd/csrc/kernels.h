@@ -85,6 +85,7 @@ struct ScanArgs {
     uint32_t key0, key1;
     float *out;              // w_px * h_px
     uint32_t ostride;        // row stride of out
+    uint32_t rays_per_lane;  // scan shape (scan_lanes); 0 or 1: one lane per ray
 };
 
 hipError_t launch_transport(const TransportArgs &a, hipStream_t s);

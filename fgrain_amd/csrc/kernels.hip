@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(256) transport_persistent_kernel(TransportArgs
 // tested as in the double-precision reference.
 __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
 {
-    const uint32_t L = scan_lanes(A.g * A.g * A.ss * A.ss);
+    const uint32_t L = scan_lanes(A.g * A.g * A.ss * A.ss, A.rays_per_lane);
     const uint32_t pix = blockIdx.x * (256u / L) + threadIdx.x / L;
     const bool valid = pix < A.w_px * A.h_px;
     const uint32_t px = valid ? pix % A.w_px : 0u, py = valid ? pix / A.w_px : 0u;
@@ -192,7 +192,7 @@ int transport_persistent_blocks_per_cu()
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s)
 {
     if (a.w_px == 0 || a.h_px == 0) return hipSuccess;
-    const uint32_t ppb = 256u / scan_lanes(a.g * a.g * a.ss * a.ss);
+    const uint32_t ppb = 256u / scan_lanes(a.g * a.g * a.ss * a.ss, a.rays_per_lane);
     const uint32_t blocks = (uint32_t)(((uint64_t)a.w_px * a.h_px + ppb - 1) / ppb);
     hipLaunchKernelGGL(scan_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();

@@ -168,12 +168,15 @@ const char *fg_last_error(void);
 uint32_t fg_abi_version(void);
 
 /* Launch parameters chosen by the autotuner (fgrain_amd/autotune.py); zero fields keep
- * the defaults.  They change speed only: every setting gives the same results. */
+ * the defaults.  They change speed only: transport settings give bit-identical results;
+ * the scan shape changes only the summation order of a pixel's rays (float rounding). */
 typedef struct fg_tuning {
-    uint32_t transport_kernel;   /* 0 persistent lanes (default), 1 one thread per packet */
+    uint32_t transport_kernel;   /* 0 persistent lanes (default), 1 one thread per packet;
+                                    per-tile calls and the replay (batched frames: persistent) */
     uint32_t transport_chunk;    /* packets per work fetch of a wave, >= 64 (default 64) */
     uint32_t transport_wait_pct; /* BVH steps continue until this % of lanes hold a leaf (80) */
-    uint32_t scan_rays_per_thread; /* reserved (0) */
+    uint32_t scan_rays_per_thread; /* scan rays per lane: 0 or 1 one lane per ray (default);
+                                      r > 1 gives a pixel rays / r adjacent lanes (<= 64) */
 } fg_tuning;
 fg_status fg_engine_set_tuning(fg_engine *engine, const fg_tuning *tuning);
 fg_status fg_engine_get_tuning(const fg_engine *engine, fg_tuning *tuning);

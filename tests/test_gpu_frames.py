@@ -136,3 +136,33 @@ def test_tmax_frame_with_uneven_tiles_batched_equals_plain(engine):
     assert np.array_equal(fp, fb)
     assert np.abs(Dp - Db).max() <= 1e-5
     assert sp["developed"] == sb["developed"] > 0
+
+
+def test_scan_shape_changes_only_the_ray_sum_order(engine, frame):
+    cfg, fr = frame
+    old = engine.tuning()
+    try:
+        outs = []
+        for rpl in (0, 2, 4, 16):
+            engine.set_tuning(**dict(old, scan_rays_per_thread=rpl))
+            fr.render(W.Frame.BATCHED)
+            outs.append((fr.read(), fr.counts()))
+    finally:
+        engine.set_tuning(**old)
+    for D, c in outs[1:]:
+        assert np.array_equal(c, outs[0][1])
+        assert np.abs(D - outs[0][0]).max() <= 1e-5
+
+
+def test_autotuner_on_a_small_crop(engine, monkeypatch):
+    """The tuner times every candidate, checks that transport candidates reproduce the
+    default latent counts, and returns the fastest of each kernel's knobs."""
+    from fgrain_amd import autotune as A
+    monkeypatch.setattr(A, "TRANSPORT_GRID", {"transport_chunk": [64, 256], "transport_wait_pct": [80, 100]})
+    monkeypatch.setattr(A, "SCAN_RAYS_PER_LANE", [1, 4])
+    r = A.tune(engine, "tile320_trix_cubic", tiles=(1, 1), reps=1, log=lambda m: None)
+    assert len(r["results"]) == 4 + 2
+    best = min(x["ms_kernel_transport"] for x in r["results"] if "ms_kernel_transport" in x)
+    assert r["ms_kernel_transport"] == best
+    assert r["tuning"]["scan_rays_per_thread"] in (1, 4)
+    assert engine.tuning()["transport_chunk"] == 64   # restored
