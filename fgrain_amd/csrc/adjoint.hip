@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(256) replay_grad_kernel(ReplayArgs R)
 
 // Persistent replay (lanes take the next packet as soon as theirs ends; the path of
 // every packet is that of replay_photon).  Block partial sums as replay_grad_kernel.
-__global__ void __launch_bounds__(256) replay_persistent_dose_kernel(ReplayArgs R, unsigned long long *work)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) replay_persistent_dose_kernel(ReplayArgs R, unsigned long long *work)
 {
     PhotonCounters cnt;
     transport_persistent<false, kReplayDose>(R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work,
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(256) replay_persistent_dose_kernel(ReplayArgs 
     }
 }
 
-__global__ void __launch_bounds__(256) replay_persistent_grad_kernel(ReplayArgs R, unsigned long long *work)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) replay_persistent_grad_kernel(ReplayArgs R, unsigned long long *work)
 {
     __shared__ double red[4][3];
     PhotonCounters cnt;

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
 }
 
 // persistent transport over every tile's packets
-__global__ void __launch_bounds__(256) k_transport(TransportArgs C, Bufs B, const Node *nodes,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) k_transport(TransportArgs C, Bufs B, const Node *nodes,
                                                    const float4 *recs, uint32_t *counts,
                                                    const float *emap, uint32_t n_tiles,
                                                    uint64_t total_photons)
