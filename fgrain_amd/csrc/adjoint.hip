@@ -248,13 +248,13 @@ __global__ void __launch_bounds__(256) replay_grad_kernel(ReplayArgs R)
 // every packet is that of replay_photon).  Block partial sums as replay_grad_kernel.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) replay_persistent_dose_kernel(ReplayArgs R, unsigned long long *work)
 {
-    PhotonCounters cnt;
+    LaneCounters cnt;
     transport_persistent<false, kReplayDose>(R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work,
                                              R.t.n_photons, cnt, &R, nullptr);
     if (R.stats) {
-        const unsigned long long b = wave_sum(cnt.bounces), e = wave_sum(cnt.abs);
+        const unsigned long long e = wave_sum((unsigned long long)cnt.abs);
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&R.stats[1], b);
+            atomicAdd(&R.stats[1], cnt.bounces);
             atomicAdd(&R.stats[2], e);
         }
     }
@@ -263,14 +263,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) replay_persistent_grad_kernel(ReplayArgs R, unsigned long long *work)
 {
     __shared__ double red[4][3];
-    PhotonCounters cnt;
+    LaneCounters cnt;
     ReplaySums sums;
     transport_persistent<false, kReplayGrad>(R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work,
                                              R.t.n_photons, cnt, &R, &sums);
     if (R.stats) {
-        const unsigned long long b = wave_sum(cnt.bounces), e = wave_sum(cnt.abs);
+        const unsigned long long e = wave_sum((unsigned long long)cnt.abs);
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&R.stats[1], b);
+            atomicAdd(&R.stats[1], cnt.bounces);
             atomicAdd(&R.stats[2], e);
         }
     }

@@ -137,14 +137,14 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
 }
 
 // persistent transport over every tile's packets
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) k_transport(TransportArgs C, Bufs B, const Node *nodes,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) k_transport(TransportArgs C, Bufs B, const Node *nodes,
                                                    const float4 *recs, uint32_t *counts,
                                                    const float *emap, uint32_t n_tiles,
                                                    uint64_t total_photons)
 {
-    PhotonCounters cnt;
+    LaneCounters cnt;
     transport_persistent<true>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work, total_photons, cnt);
-    flush_counters(cnt, C.stats);
+    flush_lane_counters(cnt, C.stats);
 }
 
 __global__ void __launch_bounds__(256) k_threshold(Bufs B, const uint32_t *counts, uint64_t n,

@@ -120,6 +120,29 @@ struct PhotonCounters {
     unsigned long long bounces = 0, nodes = 0, prims = 0, abs = 0, tot = 0;
 };
 
+// Counters of the persistent transport: bounces and node steps are counted per wave
+// (popcounts of ballots, wave-uniform, so they live in scalar registers), crystal tests,
+// absorbing hits and latent photons per lane in 32 bits (a lane handles far fewer than
+// 2^32 of each).
+struct LaneCounters {
+    unsigned long long bounces = 0, nodes = 0;
+    uint32_t prims = 0, abs = 0, tot = 0;
+};
+
+__device__ __forceinline__ void flush_lane_counters(const LaneCounters &c, unsigned long long *stats)
+{
+    const unsigned long long pr = wave_sum((unsigned long long)c.prims),
+                             ab = wave_sum((unsigned long long)c.abs),
+                             t = wave_sum((unsigned long long)c.tot);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], c.bounces);
+        atomicAdd(&stats[1], c.nodes);
+        atomicAdd(&stats[2], pr);
+        atomicAdd(&stats[3], ab);
+        atomicAdd(&stats[4], t);
+    }
+}
+
 __device__ __forceinline__ void flush_counters(PhotonCounters &c, unsigned long long *stats)
 {
     const unsigned long long b = wave_sum(c.bounces), n = wave_sum(c.nodes),
@@ -377,7 +400,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                                                      uint32_t n_tiles, const Node *nodes,
                                                      const float4 *recs, uint32_t *counts,
                                                      const float *emap, unsigned long long *work,
-                                                     uint64_t total, PhotonCounters &cnt,
+                                                     uint64_t total, LaneCounters &cnt,
                                                      const ReplayArgs *R = nullptr,
                                                      ReplaySums *sums = nullptr)
 {
@@ -487,12 +510,13 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
         }
         if (__ballot(phase != kDone) == 0ull) break;
         // ---- start a bounce: free path, step, segment box
+        bool started = false;
         if (phase == kStart) {
             if (b >= C.max_bounces) {
                 phase = kNeed;
             } else {
                 b++;
-                cnt.bounces++;
+                started = true;
                 float s;
                 if (C.mu_t <= 0.0f) s = 1e6f;
                 else s = -__logf(1.0f - rng.u()) / C.mu_t;
@@ -513,6 +537,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 phase = kWalk;
             }
         }
+        cnt.bounces += (unsigned long long)__popcll(__ballot(started));
         // ---- node steps (reference DFS order via skip links)
         {
             const uint32_t walking = (uint32_t)__popcll(__ballot(phase == kWalk));
@@ -521,9 +546,9 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 const bool stepping = phase == kWalk && leaf == 0u && i < T.n_nodes;
                 const uint32_t ns = (uint32_t)__popcll(__ballot(stepping));
                 if (ns == 0u || (it > 0u && ns <= stop)) break;
+                cnt.nodes += ns;
                 if (stepping) {
                     const NodeView nd = load_node(T.nodes, i);
-                    cnt.nodes++;
                     const bool near = !prune || (nd.lo[0] <= h0 && nd.hi[0] >= s0 && nd.lo[1] <= h1 &&
                                                  nd.hi[1] >= s1 && nd.lo[2] <= h2 && nd.hi[2] >= s2);
                     const bool hit = near && box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);

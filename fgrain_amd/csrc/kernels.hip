@@ -27,11 +27,11 @@ __global__ void __launch_bounds__(256) transport_kernel(TransportArgs A)
 }
 
 // Persistent transport of one tile.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) transport_persistent_kernel(TransportArgs A, unsigned long long *work)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) transport_persistent_kernel(TransportArgs A, unsigned long long *work)
 {
-    PhotonCounters cnt;
+    LaneCounters cnt;
     transport_persistent<false>(A, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work, A.n_photons, cnt);
-    if (A.stats) flush_counters(cnt, A.stats);
+    if (A.stats) flush_lane_counters(cnt, A.stats);
 }
 
 // ------------------------------------------------------------------------ scan ---
