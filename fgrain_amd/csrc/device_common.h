@@ -334,37 +334,40 @@ __device__ __forceinline__ void run_photon(const TransportArgs &A, uint64_t p, P
 // walking lanes hold a hit leaf, then those leaves are processed together.  For every
 // packet the draws, the order of its hits and its deposits are those of run_photon.
 
+// Per-lane tile state kept through the walk: offsets into the frame's node, record and
+// count arrays, the node count and the patch extents.  The constants a packet needs only
+// when it starts (keys, pixel grid, exposure placement) are read then (PacketStart).
 struct LaneTile {
-    const Node *nodes;
-    const float4 *recs;
-    uint32_t *counts;
-    const float *emap;
-    uint32_t n_nodes, tw, th, key0, key1;
-    float W, H, pw, ph;
+    uint32_t node_base, crystal_base, n_nodes;
+    float W, H;
 };
 
-__device__ __forceinline__ void lane_tile_single(const TransportArgs &C, LaneTile &T)
+struct PacketStart {
+    const float *emap;
+    uint32_t tw, th, key0, key1;
+    float pw, ph;
+};
+
+__device__ __forceinline__ void lane_tile_single(const TransportArgs &C, LaneTile &T, PacketStart &S)
 {
-    T.nodes = C.nodes;
-    T.recs = C.recs;
-    T.counts = C.counts;
-    T.emap = C.emap;
+    T.node_base = 0u;
+    T.crystal_base = 0u;
     T.n_nodes = C.n_nodes;
-    T.tw = C.tw;
-    T.th = C.th;
-    T.key0 = C.key0;
-    T.key1 = C.key1;
     T.W = C.W;
     T.H = C.H;
-    T.pw = C.pw;
-    T.ph = C.ph;
+    S.emap = C.emap;
+    S.tw = C.tw;
+    S.th = C.th;
+    S.key0 = C.key0;
+    S.key1 = C.key1;
+    S.pw = C.pw;
+    S.ph = C.ph;
 }
 
 // frame packet id g -> tile (last tile with photon_base <= g) and its constants
 __device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32_t n_tiles, uint64_t g,
-                                                    const Node *nodes, const float4 *recs,
-                                                    uint32_t *counts, const float *emap,
-                                                    uint32_t estride, LaneTile &T)
+                                                    const float *emap, uint32_t estride, LaneTile &T,
+                                                    PacketStart &S)
 {
     uint32_t lo = 0, hi = n_tiles;
     while (hi - lo > 1) {
@@ -373,19 +376,18 @@ __device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32
         else hi = mid;
     }
     const TileDev &t = tiles[lo];
-    T.nodes = nodes + t.node_base;
-    T.recs = recs + 3 * (size_t)t.crystal_base;
-    T.counts = counts + t.crystal_base;
-    T.emap = emap + (size_t)t.y0 * estride + t.x0;
+    T.node_base = t.node_base;
+    T.crystal_base = t.crystal_base;
     T.n_nodes = t.n_nodes;
-    T.tw = t.tw;
-    T.th = t.th;
-    T.key0 = t.key0;
-    T.key1 = t.key1;
     T.W = t.W;
     T.H = t.H;
-    T.pw = t.pw;
-    T.ph = t.ph;
+    S.emap = emap + (size_t)t.y0 * estride + t.x0;
+    S.tw = t.tw;
+    S.th = t.th;
+    S.key0 = t.key0;
+    S.key1 = t.key1;
+    S.pw = t.pw;
+    S.ph = t.ph;
     return g - t.photon_base;
 }
 
@@ -411,8 +413,12 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     unsigned long long pool = 0;
     uint32_t pool_left = 0;
     uint32_t phase = kNeed;
+    const Node *nodes_b = kFrame ? nodes : C.nodes;
+    const float4 *recs_b = kFrame ? recs : C.recs;
+    uint32_t *counts_b = kFrame ? counts : C.counts;
     LaneTile T;
-    if (!kFrame) lane_tile_single(C, T);
+    PacketStart S0;
+    if (!kFrame) lane_tile_single(C, T, S0);
     PhotonRng rng;
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f, w = 0.f;
     float step = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
@@ -455,28 +461,29 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     phase = kDone;
                 } else {
                     uint64_t p = id;
-                    if (kFrame) p = lane_tile_frame(tiles, n_tiles, id, nodes, recs, counts, emap, C.estride, T);
-                    rng.init(p, T.key0, T.key1);
+                    PacketStart S = S0;
+                    if (kFrame) p = lane_tile_frame(tiles, n_tiles, id, emap, C.estride, T, S);
+                    rng.init(p, S.key0, S.key1);
                     float x, y;
                     if (C.ppp > 0) {
                         const uint64_t pix = p / C.ppp;
-                        const uint32_t px = (uint32_t)(pix % T.tw), py = (uint32_t)(pix / T.tw);
+                        const uint32_t px = (uint32_t)(pix % S.tw), py = (uint32_t)(pix / S.tw);
                         const float ux = rng.u(), uy = rng.u();
-                        x = ((float)px + ux) * T.pw;
-                        y = ((float)py + uy) * T.ph;
+                        x = ((float)px + ux) * S.pw;
+                        y = ((float)py + uy) * S.ph;
                     } else {
                         const float ux = rng.u(), uy = rng.u();
                         x = ux * T.W;
                         y = uy * T.H;
                     }
                     w = 1.0f;
-                    if (T.emap) {
+                    if (S.emap) {
                         const float uu = fminf(fmaxf(x / T.W, 0.0f), 1.0f);
                         const float vv = fminf(fmaxf(y / T.H, 0.0f), 1.0f);
-                        const uint32_t ex = min((uint32_t)roundf(uu * (float)(T.tw - 1)), T.tw - 1);
-                        const uint32_t ey = min((uint32_t)roundf(vv * (float)(T.th - 1)), T.th - 1);
+                        const uint32_t ex = min((uint32_t)roundf(uu * (float)(S.tw - 1)), S.tw - 1);
+                        const uint32_t ey = min((uint32_t)roundf(vv * (float)(S.th - 1)), S.th - 1);
                         e_idx = (size_t)ey * C.estride + ex;
-                        w = T.emap[e_idx];
+                        w = S.emap[e_idx];
                     }
                     w = fmaxf(w, 0.0f);
                     E = w;
@@ -548,7 +555,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 if (ns == 0u || (it > 0u && ns <= stop)) break;
                 cnt.nodes += ns;
                 if (stepping) {
-                    const NodeView nd = load_node(T.nodes, i);
+                    const NodeView nd = load_node(nodes_b, T.node_base + i);
                     const bool near = !prune || (nd.lo[0] <= h0 && nd.hi[0] >= s0 && nd.lo[1] <= h1 &&
                                                  nd.hi[1] >= s1 && nd.lo[2] <= h2 && nd.hi[2] >= s2);
                     const bool hit = near && box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
@@ -562,7 +569,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             const uint32_t first = leaf >> 4, nprim = leaf & 15u;
             for (uint32_t k = first; k < first + nprim; k++) {
                 cnt.prims++;
-                const Rec r = load_rec(T.recs, k);
+                const Rec r = load_rec(recs_b, T.crystal_base + k);
                 float t0, t1;
                 if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
                 if (!(t0 < step)) continue;
@@ -576,7 +583,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
                     const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
                     if (k_lat > 0u) {
-                        atomicAdd(&T.counts[k], k_lat);
+                        atomicAdd(&counts_b[T.crystal_base + k], k_lat);
                         cnt.abs++;
                         cnt.tot += k_lat;
                     }
