@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--workload", default="frame24mp_trix_cubic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--synthesis", choices=["host", "device"], default=None,
+                    help="emulsion synthesis of the frame (default: the workload's, host Bridson)")
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore fgrain_amd/configs/tuning/<workload>.json (autotuner output)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -235,8 +237,9 @@ def main():
         return adjoint_main(args, engine)
     t = time.perf_counter()
     exposure = W.workload_exposure(cfg)
-    frame = W.Frame(engine, cfg, exposure)
-    log(f"frame ready: {frame.crystals} crystals in {time.perf_counter() - t:.1f} s (host synthesis)")
+    frame = W.Frame(engine, cfg, exposure, synthesis=args.synthesis)
+    t_synth = time.perf_counter() - t
+    log(f"frame ready: {frame.crystals} crystals in {t_synth:.1f} s ({frame.synthesis} synthesis)")
     mode = W.Frame.BATCHED if args.mode == "batched" else W.Frame.PLAIN
     for i in range(args.warmup):
         st = frame.render(mode)
@@ -296,7 +299,8 @@ def main():
         "config": {"workload": args.workload, "frame": [frame.width, frame.height],
                    "tiles": agg["tiles"], "photons_per_pixel": cfg["exposure"]["photons_per_pixel"],
                    "supersample": cfg["scan"]["supersample"], "mode": args.mode,
-                   "crystals": agg["crystals"]},
+                   "crystals": agg["crystals"], "synthesis": frame.synthesis},
+        "s_frame_synthesis": t_synth,
         "scanned_mpix_per_s": pixels / (ms_step * 1e-3) / 1e6,
         "transport_kernel_mpaths_per_s": photons * args.steps / (agg["ms_kernel_transport"] * 1e-3) / 1e6,
         "scan_kernel_mpix_per_s": pixels * args.steps / (agg["ms_kernel_scan"] * 1e-3) / 1e6,

@@ -100,7 +100,8 @@ class FrameDescC(C.Structure):
                 ("threads", C.c_uint32), ("pixel_size", C.c_double),
                 ("emulsion", EmulsionConfigC), ("exposure", ExposureConfigC),
                 ("development", DevelopmentConfigC), ("scan", ScanConfigC),
-                ("seed", C.c_uint64), ("exposure_map", _f32p), ("density_mult", _f32p)]
+                ("seed", C.c_uint64), ("exposure_map", _f32p), ("density_mult", _f32p),
+                ("synthesis", C.c_uint32), ("pad_", C.c_uint32)]
 
 
 class FrameStatsC(C.Structure):
@@ -141,6 +142,7 @@ SIGNATURES = {
     "fg_engine_set_tuning": (_st, [_vp, C.POINTER(TuningC)]),
     "fg_engine_get_tuning": (_st, [_vp, C.POINTER(TuningC)]),
     "fg_emulsion_synthesize": (_st, [C.POINTER(EmulsionConfigC), C.c_uint64, C.POINTER(_vp)]),
+    "fg_emulsion_synthesize_device": (_st, [_vp, C.POINTER(EmulsionConfigC), C.c_uint64, C.POINTER(_vp)]),
     "fg_emulsion_from_crystals": (_st, [C.POINTER(EmulsionConfigC), C.c_int64, _f64p, _f64p,
                                         _f64p, _i32p, _f64p, C.POINTER(_vp)]),
     "fg_emulsion_destroy": (None, [_vp]),

@@ -120,6 +120,16 @@ class VirtualEmulsionLayer:
         return cls(h, config)
 
     @classmethod
+    def synthesize_device(cls, config: EmulsionConfig, seed: int, engine: "GpuEngine | None" = None):
+        """Device synthesis (no reference counterpart): Poisson-disk centres by parallel
+        dart throwing on the GPU, the reference's per-crystal draws and BVH on the host."""
+        e = _engine(engine)
+        h = C.c_void_p()
+        cfg = config.c()
+        check(L.lib().fg_emulsion_synthesize_device(e._h, C.byref(cfg), seed & (2**64 - 1), C.byref(h)))
+        return cls(h, config)
+
+    @classmethod
     def from_crystals(cls, config: EmulsionConfig, pos, dims, quat, morph, sens):
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         n = pos.shape[0]

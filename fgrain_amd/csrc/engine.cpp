@@ -14,6 +14,7 @@
 
 #include "batched.h"
 #include "engine_state.h"
+#include "synth.h"
 
 using namespace fg;
 
@@ -69,7 +70,7 @@ extern "C" void fg_engine_destroy(fg_engine *E)
                       &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
                       &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
                       &E->gmap, &E->partials, &E->pgrad, &E->gout, &E->boost, &E->dboost,
-                      &E->gboost})
+                      &E->gboost, &E->synth_cells})
         b->release();
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
@@ -145,6 +146,117 @@ extern "C" fg_status fg_emulsion_synthesize(const fg_emulsion_config *cfg, uint6
         return fail(FG_ERR_INVALID_ARG, "fg_emulsion_synthesize: " + msg);
     }
     *out = E;
+    return FG_OK;
+}
+
+// Dart-throwing grid of one emulsion (synth.hip): cell r / sqrt(3), extents in float.
+static bool dart_grid(const fg_emulsion_config &cfg, uint64_t seed, DartArgs &a)
+{
+    a = DartArgs{};
+    if (!(cfg.width > 0 && cfg.height > 0 && cfg.depth > 0)) return false;
+    const float cellf = (float)(cfg.r_min / sqrt(3.0));
+    a.gx = (uint32_t)ceil(cfg.width / (double)cellf);
+    a.gy = (uint32_t)ceil(cfg.height / (double)cellf);
+    a.gz = (uint32_t)ceil(cfg.depth / (double)cellf);
+    a.cell = cellf;
+    a.r2 = (float)(cfg.r_min * cfg.r_min);
+    a.W = (float)cfg.width;
+    a.H = (float)cfg.height;
+    a.D = (float)cfg.depth;
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    return true;
+}
+
+static void compact_cells(const float4 *cells, size_t n, std::vector<double> &pts)
+{
+    pts.clear();
+    for (size_t c = 0; c < n; c++)
+        if (cells[c].w != 0.0f) {
+            pts.push_back(cells[c].x);
+            pts.push_back(cells[c].y);
+            pts.push_back(cells[c].z);
+        }
+}
+
+// Poisson-disk centres of many emulsions in one pass of device launches; cells come
+// back to the host in one buffer (tile k's grid at offsets[k]).  Caller holds E->mu.
+static fg_status device_cells_tiles(fg_engine *E, const std::vector<fg_emulsion_config> &cfgs,
+                                    const std::vector<uint64_t> &seeds, std::vector<float4> &cells,
+                                    std::vector<size_t> &offsets)
+{
+    const size_t n = cfgs.size();
+    std::vector<DartArgs> args(n);
+    offsets.assign(n + 1, 0);
+    uint32_t max_phase = 0;
+    for (size_t k = 0; k < n; k++) {
+        DartArgs &a = args[k];
+        size_t cells_k = 0;
+        if (dart_grid(cfgs[k], seeds[k], a)) {
+            const double g = (double)a.gx * a.gy * a.gz;
+            if (g > (double)(1u << 28))
+                return fail(FG_ERR_INVALID_ARG, "device synthesis: tile grid too large");
+            cells_k = (size_t)a.gx * a.gy * a.gz;
+            max_phase = std::max(max_phase, ((a.gx + 2u) / 3u) * ((a.gy + 2u) / 3u) * ((a.gz + 2u) / 3u));
+        }
+        offsets[k + 1] = offsets[k] + cells_k;
+    }
+    const size_t total = offsets[n];
+    HIP_TRY(E->synth_cells.ensure(std::max<size_t>(total, 1) * sizeof(float4)), FG_ERR_BUFFER_RANGE);
+    for (size_t k = 0; k < n; k++) args[k].cells = E->synth_cells.as<float4>() + offsets[k];
+    DevBuf dargs;
+    HIP_TRY(dargs.ensure(n * sizeof(DartArgs)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipMemcpyAsync(dargs.p, args.data(), n * sizeof(DartArgs), hipMemcpyHostToDevice, E->stream),
+            FG_ERR_BUFFER_MAP);
+    if (total) HIP_TRY(hipMemsetAsync(E->synth_cells.p, 0, total * sizeof(float4), E->stream), FG_ERR_BUFFER_MAP);
+    const uint32_t rounds = (n && cfgs[0].k_samples) ? cfgs[0].k_samples : 30u;
+    fg_status st = FG_OK;
+    hipError_t e = launch_darts_tiles(dargs.as<DartArgs>(), (uint32_t)n, max_phase, rounds, E->stream);
+    cells.resize(total);
+    if (e == hipSuccess && total)
+        e = hipMemcpyAsync(cells.data(), E->synth_cells.p, total * sizeof(float4), hipMemcpyDeviceToHost, E->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+    if (e != hipSuccess) st = fail(FG_ERR_PIPELINE, std::string("device synthesis: ") + hipGetErrorString(e));
+    dargs.release();
+    return st;
+}
+
+// Poisson-disk centres on the device (synth.hip), in cell order.
+static fg_status device_points(fg_engine *E, const fg_emulsion_config &cfg, uint64_t seed,
+                               std::vector<double> &pts)
+{
+    std::vector<float4> cells;
+    std::vector<size_t> off;
+    fg_status s = device_cells_tiles(E, {cfg}, {seed}, cells, off);
+    if (s != FG_OK) return s;
+    compact_cells(cells.data(), cells.size(), pts);
+    return FG_OK;
+}
+
+extern "C" fg_status fg_emulsion_synthesize_device(fg_engine *E, const fg_emulsion_config *cfg,
+                                                   uint64_t seed, fg_emulsion **out)
+{
+    std::string msg;
+    if (!E || !out || !valid_cfg(cfg, msg))
+        return fail(FG_ERR_INVALID_ARG, "fg_emulsion_synthesize_device: " + (msg.empty() ? std::string("null argument") : msg));
+    *out = nullptr;
+    std::vector<double> pts;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+        fg_status s = device_points(E, *cfg, seed, pts);
+        if (s != FG_OK) return s;
+    }
+    fg_emulsion *em = new fg_emulsion();
+    em->em.cfg = *cfg;
+    em->em.n = (int64_t)(pts.size() / 3);
+    em->em.pos = std::move(pts);
+    attributes_per_point(*cfg, seed, em->em);
+    if (!finalize_emulsion(em->em, msg)) {
+        delete em;
+        return fail(FG_ERR_INVALID_ARG, "fg_emulsion_synthesize_device: " + msg);
+    }
+    *out = em;
     return FG_OK;
 }
 
@@ -762,6 +874,41 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
             F->tiles.push_back(t);
         }
     F->ems.resize(F->tiles.size());
+    auto tile_cfg = [&](size_t k) {
+        const TileRec &t = F->tiles[k];
+        fg_emulsion_config c = desc->emulsion;
+        c.width = t.tw * desc->pixel_size;
+        c.height = t.th * desc->pixel_size;
+        c.r_min = desc->emulsion.r_min / cbrt(std::max(mult[k], 1e-3));
+        return c;
+    };
+    if (desc->synthesis > 1) {
+        delete F;
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: synthesis must be 0 (host) or 1 (device)");
+    }
+    std::vector<float4> dcells;
+    std::vector<size_t> doff;
+    if (desc->synthesis == 1) {
+        // Poisson-disk centres of every tile on the device at once; compaction,
+        // attributes and BVHs on the host threads below
+        std::vector<fg_emulsion_config> cfgs;
+        std::vector<uint64_t> seeds;
+        for (size_t k = 0; k < F->tiles.size(); k++) {
+            cfgs.push_back(tile_cfg(k));
+            seeds.push_back(F->tiles[k].seed);
+        }
+        std::lock_guard<std::mutex> lk(E->mu);
+        hipError_t e = hipSetDevice(E->device);
+        if (e != hipSuccess) {
+            delete F;
+            return fail(FG_ERR_DEVICE_REQUEST, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        }
+        fg_status s = device_cells_tiles(E, cfgs, seeds, dcells, doff);
+        if (s != FG_OK) {
+            delete F;
+            return s;
+        }
+    }
     std::atomic<size_t> next{0};
     std::atomic<bool> bad{false};
     std::string err;
@@ -773,11 +920,16 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
             const size_t k = next.fetch_add(1);
             if (k >= F->tiles.size()) break;
             const TileRec &t = F->tiles[k];
-            fg_emulsion_config c = desc->emulsion;
-            c.width = t.tw * desc->pixel_size;
-            c.height = t.th * desc->pixel_size;
-            c.r_min = desc->emulsion.r_min / cbrt(std::max(mult[k], 1e-3));
-            synthesize(c, t.seed, F->ems[k]);
+            const fg_emulsion_config c = tile_cfg(k);
+            if (desc->synthesis == 1) {
+                Emulsion &em = F->ems[k];
+                compact_cells(dcells.data() + doff[k], doff[k + 1] - doff[k], em.pos);
+                em.cfg = c;
+                em.n = (int64_t)(em.pos.size() / 3);
+                attributes_per_point(c, t.seed, em);
+            } else {
+                synthesize(c, t.seed, F->ems[k]);
+            }
             std::string m;
             if (!finalize_emulsion(F->ems[k], m)) {
                 std::lock_guard<std::mutex> g(emu);

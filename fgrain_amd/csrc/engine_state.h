@@ -77,6 +77,7 @@ struct fg_engine {
     // expected model and its adjoint
     DevBuf dose64, dose32, prob, dprob, gprob, gdose, gmap, partials, pgrad, gout, boost, dboost,
         gboost;
+    DevBuf synth_cells;        // device synthesis grid
 };
 
 struct fg_emulsion {

@@ -21,9 +21,13 @@ namespace fg {
 //   photon stream: ctr = (lo32(photon), hi32(photon), draw >> 2, kTagPhoton), word draw & 3
 //   scan jitter:   ctr = (pixel, ray >> 1, kTagScan, 0), words 2*(ray & 1) (x) and +1 (y)
 //   sequential:    ctr = (lo32(block), hi32(block), 0, kTagSeq); u64 = w0 | w1 << 32
+//   dart:          ctr = (cell, round, kTagDart, 0), words 0..2 the candidate's jitter
+//   crystal:       ctr = (lo32(point), hi32(point), block, kTagAttr), sequential words
 constexpr uint32_t kTagPhoton = 0x50484f54u;
 constexpr uint32_t kTagScan = 0x5343414eu;
 constexpr uint32_t kTagSeq = 0x53455120u;
+constexpr uint32_t kTagDart = 0x44415254u;
+constexpr uint32_t kTagAttr = 0x41545452u;
 
 // 32-byte BVH node in threaded ("skip-link") pre-order: a node's right subtree follows
 // it directly and its left subtree follows the right one, so walking "hit -> i + 1,

@@ -112,17 +112,48 @@ static void poisson3d(double W, double H, double D, double r, int k, SeqRng &rng
     }
 }
 
-// VirtualEmulsionLayer::synthesize (emulsion.rs:155-211)
-void synthesize(const fg_emulsion_config &cfg, uint64_t seed, Emulsion &em)
+// Per-point Philox stream of device synthesis: ctr = (lo32(p), hi32(p), block, kTagAttr).
+struct PointRng {
+    uint32_t key[2], p0, p1;
+    uint32_t block = 0, buf[4];
+    int pos = 4;
+    PointRng(uint64_t seed, uint64_t p)
+    {
+        key[0] = (uint32_t)seed;
+        key[1] = (uint32_t)(seed >> 32);
+        p0 = (uint32_t)p;
+        p1 = (uint32_t)(p >> 32);
+    }
+    uint32_t u32()
+    {
+        if (pos == 4) {
+            buf[0] = p0;
+            buf[1] = p1;
+            buf[2] = block++;
+            buf[3] = kTagAttr;
+            philox(buf, key[0], key[1]);
+            pos = 0;
+        }
+        return buf[pos++];
+    }
+    double uniform()
+    {
+        const uint64_t lo = u32(), hi = u32();
+        return (double)((lo | (hi << 32)) >> 11) * 0x1p-53;
+    }
+    double normal(double mean, double sd)
+    {
+        const double u1 = 1.0 - uniform(), u2 = uniform();
+        return mean + sd * (sqrt(-2.0 * log(u1)) * cos(kTau * u2));
+    }
+};
+
+// Crystal attributes of emulsion.rs:160-207 for every point of em.pos; rng_of(i) gives
+// the stream crystal i draws from.
+template <class RngOf>
+static void crystal_attributes(const fg_emulsion_config &cfg, Emulsion &em, RngOf &&rng_of)
 {
-    SeqRng rng(seed);
-    std::vector<double> pts;
-    const int k = cfg.k_samples ? (int)cfg.k_samples : 30;
-    poisson3d(cfg.width, cfg.height, cfg.depth, cfg.r_min, k, rng, pts);
-    const int64_t n = (int64_t)(pts.size() / 3);
-    em.cfg = cfg;
-    em.n = n;
-    em.pos = pts;
+    const int64_t n = em.n;
     em.dims.assign((size_t)(3 * n), 0.0);
     em.quat.assign((size_t)(4 * n), 0.0);
     em.sens.assign((size_t)n, 0.0);
@@ -130,6 +161,7 @@ void synthesize(const fg_emulsion_config &cfg, uint64_t seed, Emulsion &em)
     const double mu = log(cfg.mean_crystal_radius);
     const double sd_sens = cfg.sensitivity_sigma;
     for (int64_t i = 0; i < n; i++) {
+        auto &rng = rng_of(i);
         double rad = exp(rng.normal(mu, cfg.size_variance));
         rad = std::min(std::max(rad, cfg.min_radius), cfg.r_min * 0.48);
         const double s = std::max(rng.normal(cfg.base_sensitivity, sd_sens), 0.1);
@@ -164,6 +196,28 @@ void synthesize(const fg_emulsion_config &cfg, uint64_t seed, Emulsion &em)
         }
         em.sens[(size_t)i] = s;
     }
+}
+
+// VirtualEmulsionLayer::synthesize (emulsion.rs:155-211)
+void synthesize(const fg_emulsion_config &cfg, uint64_t seed, Emulsion &em)
+{
+    SeqRng rng(seed);
+    std::vector<double> pts;
+    const int k = cfg.k_samples ? (int)cfg.k_samples : 30;
+    poisson3d(cfg.width, cfg.height, cfg.depth, cfg.r_min, k, rng, pts);
+    em.cfg = cfg;
+    em.n = (int64_t)(pts.size() / 3);
+    em.pos = pts;
+    crystal_attributes(cfg, em, [&](int64_t) -> SeqRng & { return rng; });
+}
+
+void attributes_per_point(const fg_emulsion_config &cfg, uint64_t seed, Emulsion &em)
+{
+    PointRng rng(seed, 0);
+    crystal_attributes(cfg, em, [&](int64_t i) -> PointRng & {
+        rng = PointRng(seed, (uint64_t)i);
+        return rng;
+    });
 }
 
 // ------------------------------------------------------------------------ BVH ---

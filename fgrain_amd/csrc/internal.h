@@ -56,6 +56,9 @@ struct SeqRng {
 
 // host.cpp
 void synthesize(const fg_emulsion_config &cfg, uint64_t seed, Emulsion &em);
+// attributes of em.pos's points (em.n set), each crystal from its own Philox stream
+// (device synthesis, whose points arrive in cell order)
+void attributes_per_point(const fg_emulsion_config &cfg, uint64_t seed, Emulsion &em);
 void build_ref_bvh(int64_t n, const double *pos, const double *dims, const uint8_t *developed,
                    const double *fe, const uint32_t *subset, RefBvh &out);
 // Threaded layout of a reference BVH; false (with msg) if a leaf holds more than 15

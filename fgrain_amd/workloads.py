@@ -148,7 +148,7 @@ class Frame:
 
     def __init__(self, engine: GpuEngine, cfg: dict, exposure: np.ndarray | None = None,
                  density_mult: np.ndarray | None = None, threads: int = 0,
-                 frame: tuple | None = None):
+                 frame: tuple | None = None, synthesis: str | None = None):
         fr = cfg["frame"]
         w, h = (frame or (fr["width"], fr["height"]))
         self.width, self.height, self.tile = w, h, fr["tile"]
@@ -167,6 +167,11 @@ class Frame:
         d.scan = scan_config(cfg, 0, 0).c()
         d.seed = cfg["seed"]
         d.exposure_map = ptr(self.exposure, L._f32p)
+        # "host": the reference's Bridson sampler; "device": GPU dart throwing
+        self.synthesis = synthesis or fr.get("synthesis", "host")
+        if self.synthesis not in ("host", "device"):
+            raise ValueError("synthesis must be 'host' or 'device'")
+        d.synthesis = 1 if self.synthesis == "device" else 0
         if density_mult is None:
             n_tiles = -(-w // fr["tile"]) * -(-h // fr["tile"])
             density_mult = density_multipliers(cfg, n_tiles)

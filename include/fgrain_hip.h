@@ -187,6 +187,14 @@ fg_status fg_engine_get_tuning(const fg_engine *engine, fg_tuning *tuning);
 fg_status fg_emulsion_synthesize(const fg_emulsion_config *cfg, uint64_t seed,
                                  fg_emulsion **out);
 /* An emulsion from given crystals (dims = semi-axes, quat = (w,i,j,k)). */
+/* Device synthesis (no reference counterpart: replaces the serial Bridson sampler of
+ * poisson.rs:59-129 when speed matters).  Poisson-disk centres by parallel dart throwing
+ * on the GPU: a grid of cell r_min / sqrt(3), 27 phases of cells at least two cells
+ * apart, k_samples rounds of one candidate per empty cell; minimum distance r_min like
+ * the reference, density somewhat higher than Bridson's.  Crystal attributes follow
+ * emulsion.rs:160-207 with one Philox stream per crystal; the BVH is the reference's. */
+fg_status fg_emulsion_synthesize_device(fg_engine *engine, const fg_emulsion_config *cfg,
+                                        uint64_t seed, fg_emulsion **out);
 fg_status fg_emulsion_from_crystals(const fg_emulsion_config *cfg, int64_t n,
                                     const double *pos, const double *dims,
                                     const double *quat, const int32_t *morph,
@@ -266,6 +274,9 @@ typedef struct fg_frame_desc {
     uint64_t seed;
     const float *exposure_map;    /* width * height, row-major, host */
     const float *density_mult;    /* tiles, or NULL */
+    uint32_t synthesis;           /* 0 host Bridson (reference algorithm), 1 device dart
+                                     throwing (fg_emulsion_synthesize_device) */
+    uint32_t pad_;
 } fg_frame_desc;
 
 typedef struct fg_frame_stats {

@@ -32,6 +32,8 @@
 #define TAG_PHOTON 0x50484f54u
 #define TAG_SCAN 0x5343414eu
 #define TAG_SEQ 0x53455120u
+#define TAG_DART 0x44415254u
+#define TAG_ATTR 0x41545452u
 
 /* ---------------------------------------------------------------- Philox4x32-10 -- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
@@ -54,7 +56,8 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 typedef struct {
-    int mode;         /* 0 sequential (f64 uniforms), 1 photon-keyed (f32-word uniforms) */
+    int mode;         /* 0 sequential (f64 uniforms), 1 photon-keyed (f32-word uniforms),
+                         2 per-crystal stream of device synthesis (f64 uniforms) */
     uint32_t key[2];
     uint64_t ctr;     /* mode 0: next block; mode 1: photon id */
     uint32_t k;       /* mode 1: draw index */
@@ -82,12 +85,28 @@ static void rng_photon(orng *r, uint64_t seed, uint64_t photon)
     r->ctr = photon;
 }
 
+static void rng_point(orng *r, uint64_t seed, uint64_t point)
+{
+    memset(r, 0, sizeof(*r));
+    r->mode = 2;
+    r->key[0] = (uint32_t)seed;
+    r->key[1] = (uint32_t)(seed >> 32);
+    r->ctr = point;
+    r->pos = 4;
+}
+
 static uint32_t seq_u32(orng *r)
 {
     if (r->pos == 4) {
-        const uint32_t c[4] = {(uint32_t)r->ctr, (uint32_t)(r->ctr >> 32), 0u, TAG_SEQ};
-        orc_philox4x32_10(c, r->key, r->buf);
-        r->ctr++;
+        if (r->mode == 2) {
+            /* ctr = (lo32(point), hi32(point), block, TAG_ATTR) */
+            const uint32_t c[4] = {(uint32_t)r->ctr, (uint32_t)(r->ctr >> 32), r->k++, TAG_ATTR};
+            orc_philox4x32_10(c, r->key, r->buf);
+        } else {
+            const uint32_t c[4] = {(uint32_t)r->ctr, (uint32_t)(r->ctr >> 32), 0u, TAG_SEQ};
+            orc_philox4x32_10(c, r->key, r->buf);
+            r->ctr++;
+        }
         r->pos = 0;
     }
     return r->buf[r->pos++];
@@ -115,7 +134,7 @@ static uint32_t photon_word(orng *r)
 /* uniform in [0,1): 53-bit for sequential streams, 24-bit (exact float) for photon streams */
 static double rng_u(orng *r)
 {
-    if (r->mode == 0) return (double)(seq_u64(r) >> 11) * 0x1p-53;
+    if (r->mode != 1) return (double)(seq_u64(r) >> 11) * 0x1p-53;
     return (double)((float)(photon_word(r) >> 8) * 0x1p-24f);
 }
 
@@ -1214,6 +1233,43 @@ static double normal_sample(orng *r, double mean, double sd)
     return mean + sd * (sqrt(-2.0 * log(u1)) * cos(TAU * u2));
 }
 
+/* per-crystal draws of VirtualEmulsionLayer::synthesize (emulsion.rs:160-207) */
+static void crystal_attrs(const orc_synth_cfg *cfg, orng *rng, double *dims, double *q, int32_t *morph,
+                          double *sens)
+{
+    const double mu = log(cfg->mean_radius);
+    double rad = exp(normal_sample(rng, mu, cfg->size_sigma));
+    rad = fmin(fmax(rad, cfg->min_radius), cfg->r_min * 0.48);
+    const double s = fmax(normal_sample(rng, cfg->base_sens, cfg->sens_sigma), 0.1);
+    if (cfg->morph == 0 || cfg->morph == 2) {
+        const double u1 = rng_u(rng), u2 = rng_u(rng), u3 = rng_u(rng);
+        /* UnitQuaternion::from_euler_angles(roll, pitch, yaw) */
+        const double ro = u1 * TAU * 0.5, pi_ = u2 * PI_ * 0.5, ya = u3 * TAU * 0.5;
+        const double sr = sin(ro), cr = cos(ro), sp = sin(pi_), cp = cos(pi_),
+                     sy = sin(ya), cy = cos(ya);
+        q[0] = cr * cp * cy + sr * sp * sy;
+        q[1] = sr * cp * cy - cr * sp * sy;
+        q[2] = cr * sp * cy + sr * cp * sy;
+        q[3] = cr * cp * sy - sr * sp * cy;
+        dims[0] = dims[1] = dims[2] = rad;
+    } else {
+        const double ht = fmax(rad / cfg->aspect, cfg->min_half_thickness);
+        const double tilt = -0.17 + 0.34 * rng_u(rng);
+        double ax[3] = {-1.0 + 2.0 * rng_u(rng), -1.0 + 2.0 * rng_u(rng), 0.0};
+        normalize3(ax);
+        const double sh = sin(tilt * 0.5), ch = cos(tilt * 0.5);
+        q[0] = ch;
+        q[1] = ax[0] * sh;
+        q[2] = ax[1] * sh;
+        q[3] = ax[2] * sh;
+        dims[0] = rad;
+        dims[1] = rad;
+        dims[2] = ht;
+    }
+    *morph = cfg->morph;
+    *sens = s;
+}
+
 /* sample_poisson_3d (poisson.rs:14-132) followed by the per-crystal draws of
  * VirtualEmulsionLayer::synthesize (emulsion.rs:155-211). */
 int64_t orc_synthesize(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, double *pos,
@@ -1309,40 +1365,9 @@ int64_t orc_synthesize(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, dou
         free(pts);
         return -np;
     }
-    const double mu = log(cfg->mean_radius);
     for (int64_t i = 0; i < np; i++) {
-        double rad = exp(normal_sample(&rng, mu, cfg->size_sigma));
-        rad = fmin(fmax(rad, cfg->min_radius), r * 0.48);
-        const double s = fmax(normal_sample(&rng, cfg->base_sens, cfg->sens_sigma), 0.1);
         memcpy(pos + 3 * i, pts + 3 * i, 3 * sizeof(double));
-        double *q = quat + 4 * i;
-        if (cfg->morph == 0 || cfg->morph == 2) {
-            const double u1 = rng_u(&rng), u2 = rng_u(&rng), u3 = rng_u(&rng);
-            /* UnitQuaternion::from_euler_angles(roll, pitch, yaw) */
-            const double ro = u1 * TAU * 0.5, pi_ = u2 * PI_ * 0.5, ya = u3 * TAU * 0.5;
-            const double sr = sin(ro), cr = cos(ro), sp = sin(pi_), cp = cos(pi_),
-                         sy = sin(ya), cy = cos(ya);
-            q[0] = cr * cp * cy + sr * sp * sy;
-            q[1] = sr * cp * cy - cr * sp * sy;
-            q[2] = cr * sp * cy + sr * cp * sy;
-            q[3] = cr * cp * sy - sr * sp * cy;
-            dims[3 * i] = dims[3 * i + 1] = dims[3 * i + 2] = rad;
-        } else {
-            const double ht = fmax(rad / cfg->aspect, cfg->min_half_thickness);
-            const double tilt = -0.17 + 0.34 * rng_u(&rng);
-            double ax[3] = {-1.0 + 2.0 * rng_u(&rng), -1.0 + 2.0 * rng_u(&rng), 0.0};
-            normalize3(ax);
-            const double sh = sin(tilt * 0.5), ch = cos(tilt * 0.5);
-            q[0] = ch;
-            q[1] = ax[0] * sh;
-            q[2] = ax[1] * sh;
-            q[3] = ax[2] * sh;
-            dims[3 * i] = rad;
-            dims[3 * i + 1] = rad;
-            dims[3 * i + 2] = ht;
-        }
-        morph[i] = cfg->morph;
-        sens[i] = s;
+        crystal_attrs(cfg, &rng, dims + 3 * i, quat + 4 * i, morph + i, sens + i);
     }
     free(pts);
     return np;
@@ -1982,4 +2007,82 @@ int64_t orc_touch_graph(const orc_crystals *cs, double fe, double pf, int64_t *p
     free(nb);
     bvh_free(&b);
     return k <= cap ? k : -k;
+}
+
+/* Device synthesis twin (no reference counterpart): the library's parallel dart throwing
+ * evaluated sequentially in float with the kernel's operation order (cells of one phase
+ * do not interact, so their order is free), then per-crystal streams for the
+ * attributes.  Returns the number of crystals, or minus it when cap is too small. */
+int64_t orc_darts(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, double *pos, double *dims,
+                  double *quat, int32_t *morph, double *sens)
+{
+    const double W = cfg->W, H = cfg->H, D = cfg->D, r = cfg->r_min;
+    if (W <= 0 || H <= 0 || D <= 0 || r <= 0) return 0;
+    const float cell = (float)(r / sqrt(3.0));
+    const uint32_t gx = (uint32_t)ceil(W / (double)cell), gy = (uint32_t)ceil(H / (double)cell),
+                   gz = (uint32_t)ceil(D / (double)cell);
+    const float r2 = (float)(r * r), Wf = (float)W, Hf = (float)H, Df = (float)D;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const size_t nc = (size_t)gx * gy * gz;
+    float *cx = (float *)calloc(nc * 4, sizeof(float));
+    const int rounds = cfg->k_samples > 0 ? cfg->k_samples : 30;
+    for (int rd = 0; rd < rounds; rd++)
+        for (uint32_t k0 = 0; k0 < 3; k0++)
+            for (uint32_t j0 = 0; j0 < 3; j0++)
+                for (uint32_t i0 = 0; i0 < 3; i0++)
+                    for (uint32_t k = k0; k < gz; k += 3)
+                        for (uint32_t j = j0; j < gy; j += 3)
+                            for (uint32_t i = i0; i < gx; i += 3) {
+                                const uint32_t c = i + gx * (j + gy * k);
+                                if (cx[4 * (size_t)c + 3] != 0.0f) continue;
+                                const uint32_t ctr[4] = {c, (uint32_t)rd, TAG_DART, 0u};
+                                uint32_t w[4];
+                                orc_philox4x32_10(ctr, key, w);
+                                const float x = ((float)i + (float)(w[0] >> 8) * 0x1p-24f) * cell;
+                                const float y = ((float)j + (float)(w[1] >> 8) * 0x1p-24f) * cell;
+                                const float z = ((float)k + (float)(w[2] >> 8) * 0x1p-24f) * cell;
+                                if (!(x < Wf && y < Hf && z < Df)) continue;
+                                int ok = 1;
+                                for (int dz = -2; dz <= 2 && ok; dz++) {
+                                    const int kk = (int)k + dz;
+                                    if (kk < 0 || kk >= (int)gz) continue;
+                                    for (int dy = -2; dy <= 2 && ok; dy++) {
+                                        const int jj = (int)j + dy;
+                                        if (jj < 0 || jj >= (int)gy) continue;
+                                        for (int dx = -2; dx <= 2; dx++) {
+                                            const int ii = (int)i + dx;
+                                            if (ii < 0 || ii >= (int)gx) continue;
+                                            const float *q = cx + 4 * ((size_t)ii + (size_t)gx * ((size_t)jj + (size_t)gy * (size_t)kk));
+                                            if (q[3] == 0.0f) continue;
+                                            const float ex = q[0] - x, ey = q[1] - y, ez = q[2] - z;
+                                            if ((ex * ex + ey * ey) + ez * ez < r2) {
+                                                ok = 0;
+                                                break;
+                                            }
+                                        }
+                                    }
+                                }
+                                if (!ok) continue;
+                                cx[4 * (size_t)c] = x;
+                                cx[4 * (size_t)c + 1] = y;
+                                cx[4 * (size_t)c + 2] = z;
+                                cx[4 * (size_t)c + 3] = 1.0f;
+                            }
+    int64_t np = 0;
+    for (size_t c = 0; c < nc; c++) np += cx[4 * c + 3] != 0.0f;
+    if (np > cap) {
+        free(cx);
+        return -np;
+    }
+    int64_t p = 0;
+    for (size_t c = 0; c < nc; c++) {
+        if (cx[4 * c + 3] == 0.0f) continue;
+        for (int a = 0; a < 3; a++) pos[3 * p + a] = (double)cx[4 * c + a];
+        orng rng;
+        rng_point(&rng, seed, (uint64_t)p);
+        crystal_attrs(cfg, &rng, dims + 3 * p, quat + 4 * p, morph + p, sens + p);
+        p++;
+    }
+    free(cx);
+    return np;
 }

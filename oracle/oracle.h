@@ -118,6 +118,10 @@ double orc_fresnel(double n, double cos_i);
 int orc_antihalation(double r_ahu, double od, uint64_t seed, double *energy, double d[3]);
 int64_t orc_synthesize(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, double *pos,
                        double *dims, double *quat, int32_t *morph, double *sens);
+/* twin of the library's device synthesis (parallel dart throwing, per-crystal streams);
+ * no reference counterpart */
+int64_t orc_darts(const orc_synth_cfg *cfg, uint64_t seed, int64_t cap, double *pos, double *dims,
+                  double *quat, int32_t *morph, double *sens);
 
 /* --- expected (differentiable) model: no reference counterpart ---------------------- */
 typedef struct {

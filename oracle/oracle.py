@@ -112,6 +112,9 @@ def lib():
         L.orc_synthesize.argtypes = [C.POINTER(OrcSynthCfg), C.c_uint64, C.c_int64, _f64p, _f64p,
                                      _f64p, _i32p, _f64p]
         L.orc_synthesize.restype = C.c_int64
+        L.orc_darts.argtypes = [C.POINTER(OrcSynthCfg), C.c_uint64, C.c_int64, _f64p, _f64p,
+                                _f64p, _i32p, _f64p]
+        L.orc_darts.restype = C.c_int64
         L.orc_free_path.argtypes = [C.c_double, C.c_double]
         L.orc_free_path.restype = C.c_double
         L.orc_hg.argtypes = [_f64p, C.c_double, C.c_double, C.c_double, _f64p]
@@ -298,20 +301,29 @@ def otf3(img):
     return out
 
 
-def synthesize(*, W, H, D, r_min, mean_radius, size_sigma, morph, aspect=1.0, base_sens=1.2,
+def _synth(fn, *, W, H, D, r_min, mean_radius, size_sigma, morph, aspect=1.0, base_sens=1.2,
                sens_sigma=0.15, min_radius=0.15, min_half_thickness=0.05, k_samples=30, seed=0):
-    """VirtualEmulsionLayer::synthesize -> Crystals."""
     cfg = OrcSynthCfg(W, H, D, r_min, mean_radius, size_sigma, aspect, base_sens, sens_sigma,
                       min_radius, min_half_thickness, morph, k_samples)
     cap = int(W * H * D / (r_min ** 3) * 1.2) + 64
     while True:
         pos = np.zeros((cap, 3)); dims = np.zeros((cap, 3)); quat = np.zeros((cap, 4))
         mo = np.zeros(cap, np.int32); sens = np.zeros(cap)
-        n = lib().orc_synthesize(C.byref(cfg), seed, cap, _p(pos, _f64p), _p(dims, _f64p),
-                                 _p(quat, _f64p), _p(mo, _i32p), _p(sens, _f64p))
+        n = fn(C.byref(cfg), seed, cap, _p(pos, _f64p), _p(dims, _f64p), _p(quat, _f64p),
+               _p(mo, _i32p), _p(sens, _f64p))
         if n >= 0:
             return Crystals(pos[:n], dims[:n], quat[:n], mo[:n], sens[:n])
         cap = -n
+
+
+def synthesize(**kw):
+    """VirtualEmulsionLayer::synthesize -> Crystals (Bridson, one sequential stream)."""
+    return _synth(lib().orc_synthesize, **kw)
+
+
+def synthesize_darts(**kw):
+    """The library's device synthesis (dart throwing) evaluated on the CPU -> Crystals."""
+    return _synth(lib().orc_darts, **kw)
 
 
 def free_path(mu_t, xi):
