@@ -101,7 +101,7 @@ class FrameDescC(C.Structure):
                 ("emulsion", EmulsionConfigC), ("exposure", ExposureConfigC),
                 ("development", DevelopmentConfigC), ("scan", ScanConfigC),
                 ("seed", C.c_uint64), ("exposure_map", _f32p), ("density_mult", _f32p),
-                ("synthesis", C.c_uint32), ("pad_", C.c_uint32)]
+                ("synthesis", C.c_uint32), ("overlap", C.c_uint32)]
 
 
 class FrameStatsC(C.Structure):
@@ -127,6 +127,36 @@ class ParamGradsC(C.Structure):
 class AdjointStatsC(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("packets", "bounces", "events")] + [
         (n, C.c_float) for n in ("ms_replay", "ms_scan", "ms_total", "pad_")]
+
+
+class FilmOptionsC(C.Structure):
+    _fields_ = [("preset", C.c_char * 32), ("grain_size_px", C.c_double), ("grain_strength", C.c_double),
+                ("exposure", C.c_double), ("scale", C.c_double), ("push", C.c_double), ("seed", C.c_uint64),
+                ("tile_size", C.c_uint32), ("reference_res", C.c_uint32), ("auto_scale", C.c_uint32),
+                ("scanner_otf", C.c_uint32), ("synthesis", C.c_uint32), ("frame_mode", C.c_uint32),
+                ("threads", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class FilmPlanC(C.Structure):
+    _fields_ = [("grain_size_px", C.c_double), ("strength", C.c_double), ("push", C.c_double),
+                ("pixel_size_um", C.c_double), ("tile_step", C.c_uint32), ("pad", C.c_uint32),
+                ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("filter_sigma", C.c_float),
+                ("k_rad", C.c_uint32), ("base_nominal", C.c_float), ("pad_", C.c_uint32)]
+
+
+class FilmReportC(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("in_w", "in_h", "out_w", "out_h")] + [
+        (n, C.c_uint64) for n in ("total_tiles", "total_crystals", "developed_crystals", "total_clumps",
+                                  "photons", "rays")] + [("elapsed_seconds", C.c_double)] + [
+        (n, C.c_float) for n in ("high_mean", "high_std", "norm_factor", "pad_", "ms_luma_optics",
+                                 "ms_synthesis", "ms_render", "ms_composite", "ms_kernel_optics",
+                                 "ms_kernel_composite", "ms_kernel_transport", "ms_kernel_scan")]
+
+
+class CompositeStatsC(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("high_mean", "high_std", "norm_factor", "filter_sigma")] + [
+        (n, C.c_uint32) for n in ("k_rad", "tiles_x", "tiles_y", "pad_")] + [
+        ("ms_kernels", C.c_float), ("pad2_", C.c_float)]
 
 
 # every entry point of include/fgrain_hip.h: name -> (restype, argtypes)
@@ -167,6 +197,7 @@ SIGNATURES = {
     "fg_frame_destroy": (None, [_vp]),
     "fg_frame_render": (_st, [_vp, _vp, C.c_uint32, C.POINTER(FrameStatsC)]),
     "fg_frame_read": (_st, [_vp, _vp, _f32p]),
+    "fg_frame_read_tiles": (_st, [_vp, _vp, C.c_int64, _f32p, _i64p]),
     "fg_frame_read_counts": (_st, [_vp, _vp, C.c_int64, _u32p]),
     "fg_frame_read_development": (_st, [_vp, _vp, C.c_int64, _u8p, _f64p, _i64p]),
     "fg_frame_crystal_count": (C.c_int64, [_vp]),
@@ -175,6 +206,14 @@ SIGNATURES = {
     "fg_vjp": (_st, [_vp, _vp, C.POINTER(ExpectedModelC), _f32p, C.c_uint32, C.c_uint32,
                      _f32p, _f32p, _f32p, C.POINTER(ParamGradsC), _f32p,
                      C.POINTER(AdjointStatsC)]),
+    "fg_film_options_default": (_st, [C.POINTER(FilmOptionsC)]),
+    "fg_film_plan_for": (_st, [C.POINTER(FilmOptionsC), C.c_uint32, C.c_uint32, C.POINTER(FilmPlanC)]),
+    "fg_render_film_bw": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(FilmOptionsC),
+                                _u8p, C.POINTER(FilmReportC)]),
+    "fg_film_luma": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, C.c_char_p, _u8p]),
+    "fg_film_optics": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, _f32p]),
+    "fg_film_composite": (_st, [_vp, C.POINTER(FilmOptionsC), C.c_uint32, C.c_uint32, _u8p, _f32p, _f32p,
+                                C.c_int64, _u8p, _f32p, _f32p, _f32p, C.POINTER(CompositeStatsC)]),
 }
 
 _lib = None

@@ -23,8 +23,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-I", os.path.join(HERE, "..", "include")]
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-munsafe-fp-atomics"]
-HOST_SOURCES = ["host.cpp", "engine.cpp", "adjoint_host.cpp"]
-DEVICE_SOURCES = ["kernels.hip", "batched.hip", "adjoint.hip", "synth.hip"]
+HOST_SOURCES = ["host.cpp", "engine.cpp", "adjoint_host.cpp", "film_host.cpp"]
+DEVICE_SOURCES = ["kernels.hip", "batched.hip", "adjoint.hip", "synth.hip", "film.hip"]
 
 
 def _run(cmd):

@@ -497,8 +497,7 @@ __global__ void __launch_bounds__(256) k_dev_records(Bufs B, const Crystal *recs
     B.drecs[e] = r;
 }
 
-__global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, uint32_t frame_w,
-                                              uint32_t n_tiles)
+__global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, uint32_t n_tiles)
 {
     const uint32_t t = tile_of_block(B.tiles, n_tiles, blockIdx.x);
     const TileDev &T = B.tiles[t];
@@ -511,7 +510,7 @@ __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, 
     const float Tm = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, S.g, S.ss, T.pw, T.ph, S.z0,
                                 S.sigma, T.skey0, T.skey1, nodes, B.node_count[t], recs);
     if (valid && threadIdx.x % L == 0u)
-        out[(size_t)(T.y0 + py) * frame_w + T.x0 + px] = scan_value(Tm, S.mode);
+        out[T.out_base + (size_t)py * T.out_stride + px] = scan_value(Tm, S.mode);
 }
 
 }  // namespace
@@ -669,8 +668,7 @@ static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
     hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, 2 * n);
     hipLaunchKernelGGL(k_dev_records, dim3(nb), dim3(256), 0, s, B, in.recs, n);
     TRY(mark(b, 2, s));
-    hipLaunchKernelGGL(k_scan, dim3(b->total_blocks), dim3(256), 0, s, B, in.scan, in.dneg, in.frame_w,
-                       b->n_tiles);
+    hipLaunchKernelGGL(k_scan, dim3(b->total_blocks), dim3(256), 0, s, B, in.scan, in.dneg, b->n_tiles);
     TRY(mark(b, 3, s));
     return hipGetLastError();
 }

@@ -70,7 +70,9 @@ extern "C" void fg_engine_destroy(fg_engine *E)
                       &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
                       &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
                       &E->gmap, &E->partials, &E->pgrad, &E->gout, &E->boost, &E->dboost,
-                      &E->gboost, &E->synth_cells})
+                      &E->gboost, &E->synth_cells, &E->film_rgb, &E->film_gray, &E->film_heff, &E->film_bbox,
+                      &E->film_tiles, &E->film_dnorm, &E->film_tmp, &E->film_dhigh, &E->film_partials,
+                      &E->film_fstats, &E->film_kern, &E->film_layout, &E->film_grain, &E->film_out})
         b->release();
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
@@ -743,7 +745,7 @@ extern "C" fg_status fg_tile_params_init(fg_tile_params *p, const char *preset, 
     return FG_OK;
 }
 
-static fg_emulsion_config tile_emulsion_config(const fg_tile_params &p)
+fg_emulsion_config tile_emulsion_config(const fg_tile_params &p)
 {
     fg_emulsion_config c{};
     c.width = p.film_w;
@@ -818,17 +820,22 @@ extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img,
 
 // -------------------------------------------------------------------- frames -----
 struct TileRec {
-    uint32_t x0, y0, tw, th;
+    uint32_t x0, y0, tw, th;   // first pixel and clipped extent
     uint64_t seed;
     int64_t crystal_base, node_base;
     uint32_t n_nodes;
     int64_t n;
+    int64_t out_base;          // first D_neg value of the tile
+    uint32_t out_stride;       // D_neg row stride
 };
 
 struct fg_frame {
     fg_frame_desc desc{};
     std::vector<float> emap;
     std::vector<TileRec> tiles;
+    int64_t dneg_values = 0;   // width * height, or the sum of tile areas with overlap
+    fg::FilmLayout layout{};   // tile spans and Hann ramps of an overlapped frame
+    DevBuf d_layout;
     std::vector<Emulsion> ems;
     int64_t n_crystals = 0, n_nodes = 0;
     DevBuf d_nodes, d_recs, d_counts, d_emap, d_dneg, d_dneg2, d_stats, d_scan_nodes, d_scan_recs;
@@ -848,6 +855,10 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
         return fail(FG_ERR_INVALID_ARG, "fg_frame_create: null argument");
     if (desc->width == 0 || desc->height == 0 || desc->tile == 0 || !(desc->pixel_size > 0))
         return fail(FG_ERR_INVALID_ARG, "fg_frame_create: empty frame or tile");
+    if (desc->overlap && (2ull * desc->overlap > desc->tile || desc->scan.otf))
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: overlap needs 2 * overlap <= tile and scan.otf = 0");
+    if ((uint64_t)desc->width * desc->height > (1ull << 31))
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: frame above 2^31 pixels");
     std::string msg;
     if (!valid_exposure(&desc->exposure, msg) || !valid_scan(&desc->scan, msg))
         return fail(FG_ERR_INVALID_ARG, "fg_frame_create: " + msg);
@@ -861,18 +872,27 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
     std::vector<double> mult((size_t)nx * ny, 1.0);
     if (desc->density_mult)
         for (size_t k = 0; k < mult.size(); k++) mult[k] = desc->density_mult[k];
+    const uint32_t ov = desc->overlap;
     for (uint32_t ty = 0; ty < ny; ty++)
         for (uint32_t tx = 0; tx < nx; tx++) {
             TileRec t{};
-            t.x0 = tx * desc->tile;
-            t.y0 = ty * desc->tile;
-            t.tw = std::min(desc->tile, desc->width - t.x0);
-            t.th = std::min(desc->tile, desc->height - t.y0);
+            // tile span (image_pipeline.rs:400-410); abutting tiles when the overlap is 0
+            fg::film_span(tx, desc->tile, ov, desc->width, t.x0, t.tw);
+            fg::film_span(ty, desc->tile, ov, desc->height, t.y0, t.th);
             const uint64_t idx = (uint64_t)ty * nx + tx;
             // tile seed (image_pipeline.rs:430-434)
             t.seed = desc->seed + idx * 73856093ull + (uint64_t)tx * 19349663ull + (uint64_t)ty * 83492791ull;
+            if (ov) {
+                t.out_base = F->dneg_values;
+                t.out_stride = t.tw;
+                F->dneg_values += (int64_t)t.tw * t.th;
+            } else {
+                t.out_base = (int64_t)t.y0 * desc->width + t.x0;
+                t.out_stride = desc->width;
+            }
             F->tiles.push_back(t);
         }
+    if (!ov) F->dneg_values = (int64_t)desc->width * desc->height;
     F->ems.resize(F->tiles.size());
     auto tile_cfg = [&](size_t k) {
         const TileRec &t = F->tiles[k];
@@ -961,8 +981,10 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
     if (e == hipSuccess) e = F->d_recs.ensure((size_t)F->n_crystals * sizeof(Crystal));
     if (e == hipSuccess) e = F->d_counts.ensure((size_t)F->n_crystals * 4);
     if (e == hipSuccess) e = F->d_emap.ensure(F->emap.size() * 4);
-    if (e == hipSuccess) e = F->d_dneg.ensure(F->emap.size() * 4);
+    if (e == hipSuccess) e = F->d_dneg.ensure((size_t)F->dneg_values * 4);
     if (e == hipSuccess) e = F->d_dneg2.ensure(F->emap.size() * 4);
+    if (e == hipSuccess && ov)
+        e = fg::film_layout_upload(desc->width, desc->height, desc->tile, ov, F->d_layout, F->layout, E->stream);
     if (e == hipSuccess) e = F->d_stats.ensure(64);
     if (e != hipSuccess) return cleanup(fail(FG_ERR_BUFFER_RANGE, std::string("frame allocation: ") + hipGetErrorString(e)));
     for (size_t k = 0; k < F->tiles.size() && e == hipSuccess; k++) {
@@ -976,6 +998,9 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
     }
     if (e == hipSuccess)
         e = hipMemcpy(F->d_emap.p, F->emap.data(), F->emap.size() * 4, hipMemcpyHostToDevice);
+    // the uploads above run on the null stream, which does not order the engine's
+    // non-blocking stream: wait for them before any kernel reads the frame
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return cleanup(fail(FG_ERR_BUFFER_MAP, std::string("frame upload: ") + hipGetErrorString(e)));
     *out = F;
     return FG_OK;
@@ -986,7 +1011,7 @@ extern "C" void fg_frame_destroy(fg_frame *F)
     if (!F) return;
     if (F->batched) batched_destroy(F->batched);
     for (DevBuf *b : {&F->d_nodes, &F->d_recs, &F->d_counts, &F->d_emap, &F->d_dneg, &F->d_dneg2,
-                      &F->d_stats, &F->d_scan_nodes, &F->d_scan_recs})
+                      &F->d_stats, &F->d_scan_nodes, &F->d_scan_recs, &F->d_layout})
         b->release();
     delete F;
 }
@@ -1063,8 +1088,8 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
         s.sigma = (float)D.scan.silver_opacity;
         s.key0 = (uint32_t)(t.seed + 2);
         s.key1 = (uint32_t)((t.seed + 2) >> 32);
-        s.out = F->d_dneg.as<float>() + (size_t)t.y0 * D.width + t.x0;
-        s.ostride = D.width;
+        s.out = F->d_dneg.as<float>() + t.out_base;
+        s.ostride = t.out_stride;
         s.rays_per_lane = E->tuning.scan_rays_per_thread;
         HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
         HIP_TRY(launch_scan(s, E->stream), FG_ERR_PIPELINE);
@@ -1143,6 +1168,8 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
             td.key1 = a.key1;
             td.skey0 = (uint32_t)(t.seed + 2);
             td.skey1 = (uint32_t)((t.seed + 2) >> 32);
+            td.out_base = (uint64_t)t.out_base;
+            td.out_stride = t.out_stride;
             td.node_base = (uint32_t)t.node_base;
             td.n_nodes = t.n_nodes;
             td.crystal_base = (uint32_t)t.crystal_base;
@@ -1242,9 +1269,49 @@ extern "C" fg_status fg_frame_read(fg_engine *E, const fg_frame *F, float *out)
     std::lock_guard<std::mutex> lk(E->mu);
     HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
     const void *src = F->otf_applied ? F->d_dneg2.p : F->d_dneg.p;
-    HIP_TRY(hipMemcpy(out, src, F->emap.size() * 4, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+    if (F->desc.overlap) {
+        // the Hann blend (image_pipeline.rs:593-654) is one of the compositing kernels,
+        // which are switched off in this build (device memory fault under investigation);
+        // fg_frame_read_tiles returns the tiles
+        if (!fg::kFilmBlendEnabled)
+            return fail(FG_ERR_PIPELINE, "fg_frame_read: the Hann blend of overlapping tiles is switched off "
+                                         "in this build; read the tiles with fg_frame_read_tiles");
+        HIP_TRY(fg::launch_film_blend(F->layout, F->d_dneg.as<float>(), F->d_dneg2.as<float>(), E->stream),
+                FG_ERR_PIPELINE);
+        src = F->d_dneg2.p;
+    }
+    HIP_TRY(hipMemcpyAsync(out, src, F->emap.size() * 4, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
     return FG_OK;
 }
+
+extern "C" fg_status fg_frame_read_tiles(fg_engine *E, const fg_frame *F, int64_t n, float *out,
+                                         int64_t *n_total)
+{
+    if (!E || !F) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_tiles: null argument");
+    int64_t total = 0;
+    for (const TileRec &t : F->tiles) total += (int64_t)t.tw * t.th;
+    if (n_total) *n_total = total;
+    if (!out) return FG_OK;
+    if (n != total) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_tiles: n must equal the tile values");
+    if (!F->rendered) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_tiles: frame not rendered");
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    if (F->desc.overlap) {
+        HIP_TRY(hipMemcpy(out, F->d_dneg.p, (size_t)total * 4, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+        return FG_OK;
+    }
+    // abutting tiles share the frame image: gather each tile's rectangle
+    for (const TileRec &t : F->tiles) {
+        HIP_TRY(hipMemcpy2D(out, (size_t)t.tw * 4, F->d_dneg.as<float>() + t.out_base, (size_t)F->desc.width * 4,
+                            (size_t)t.tw * 4, t.th, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+        out += (size_t)t.tw * t.th;
+    }
+    return FG_OK;
+}
+
+const float *frame_tile_dneg(const fg_frame *F) { return F->d_dneg.as<float>(); }
+const fg::FilmLayout *frame_layout(const fg_frame *F) { return F->desc.overlap ? &F->layout : nullptr; }
 
 extern "C" fg_status fg_frame_read_counts(fg_engine *E, const fg_frame *F, int64_t n, uint32_t *counts)
 {

@@ -7,6 +7,7 @@
 #include <mutex>
 #include <string>
 
+#include "film.h"
 #include "internal.h"
 #include "kernels.h"
 
@@ -78,6 +79,11 @@ struct fg_engine {
     DevBuf dose64, dose32, prob, dprob, gprob, gdose, gmap, partials, pgrad, gout, boost, dboost,
         gboost;
     DevBuf synth_cells;        // device synthesis grid
+    // full-frame film compositing; film_mu (taken before mu) serialises the film calls,
+    // which hold these buffers across several engine calls
+    std::mutex film_mu;
+    DevBuf film_rgb, film_gray, film_heff, film_bbox, film_tiles, film_dnorm, film_tmp, film_dhigh,
+        film_partials, film_fstats, film_kern, film_layout, film_grain, film_out;
 };
 
 struct fg_emulsion {
@@ -85,6 +91,17 @@ struct fg_emulsion {
 };
 
 fg_status ensure_device(fg_engine *E, fg::Emulsion &em);
+namespace fg {
+// Tables of an overlapped layout (film_host.cpp) uploaded into buf on stream (complete
+// on return); L points into buf.
+hipError_t film_layout_upload(uint32_t W, uint32_t H, uint32_t step, uint32_t pad, DevBuf &buf,
+                              FilmLayout &L, hipStream_t stream);
+}  // namespace fg
+// EmulsionConfig of a tile render (render_tile_cpu_full's preset, image_pipeline.rs:925-938)
+fg_emulsion_config tile_emulsion_config(const fg_tile_params &p);
+// the frame's tile D_neg buffer and its layout (null for abutting tiles)
+const float *frame_tile_dneg(const fg_frame *F);
+const fg::FilmLayout *frame_layout(const fg_frame *F);
 void fill_transport(fg::TransportArgs &a, const fg_emulsion_config &ec, const fg_exposure_config &x,
                     uint32_t tw, uint32_t th, uint64_t seed, double reach, const fg_tuning &tn);
 bool valid_exposure(const fg_exposure_config *c, std::string &msg);

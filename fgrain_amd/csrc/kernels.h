@@ -67,7 +67,8 @@ struct TileDev {
     uint64_t n_photons;
     uint64_t photon_base;           // first packet id of the tile in the frame
     uint32_t block_base, blocks_x;  // scan blocks of 16 x 16 pixels
-    uint32_t blocks, pad0;
+    uint32_t blocks, out_stride;    // scan output row stride
+    uint64_t out_base;              // first output value of the tile
     double maxb;                    // largest semi-axis among the tile's crystals
     // results of the last render
     uint32_t n_dev, clumps, max_clump, dev_nodes;

@@ -148,7 +148,7 @@ class Frame:
 
     def __init__(self, engine: GpuEngine, cfg: dict, exposure: np.ndarray | None = None,
                  density_mult: np.ndarray | None = None, threads: int = 0,
-                 frame: tuple | None = None, synthesis: str | None = None):
+                 frame: tuple | None = None, synthesis: str | None = None, overlap: int = 0):
         fr = cfg["frame"]
         w, h = (frame or (fr["width"], fr["height"]))
         self.width, self.height, self.tile = w, h, fr["tile"]
@@ -172,6 +172,9 @@ class Frame:
         if self.synthesis not in ("host", "device"):
             raise ValueError("synthesis must be 'host' or 'device'")
         d.synthesis = 1 if self.synthesis == "device" else 0
+        # overlapping tiles as the reference's tiled renderer lays them out
+        self.overlap = int(overlap)
+        d.overlap = self.overlap
         if density_mult is None:
             n_tiles = -(-w // fr["tile"]) * -(-h // fr["tile"])
             density_mult = density_multipliers(cfg, n_tiles)
@@ -208,16 +211,27 @@ class Frame:
         return d, f, c
 
     def tile_ranges(self):
-        """(x0, y0, tw, th, seed) per tile, in frame order."""
+        """(x0, y0, tw, th, seed) per tile, in frame order (image_pipeline.rs:397-434)."""
         out = []
         nx = -(-self.width // self.tile)
         ny = -(-self.height // self.tile)
+        ov = getattr(self, "overlap", 0)
         for ty in range(ny):
             for tx in range(nx):
                 idx = ty * nx + tx
                 seed = (self.seed + idx * 73856093 + tx * 19349663 + ty * 83492791) % (1 << 64)
-                x0, y0 = tx * self.tile, ty * self.tile
-                out.append((x0, y0, min(self.tile, self.width - x0), min(self.tile, self.height - y0), seed))
+                x0, y0 = max(tx * self.tile - ov, 0), max(ty * self.tile - ov, 0)
+                x1 = min(tx * self.tile + self.tile + ov, self.width)
+                y1 = min(ty * self.tile + self.tile + ov, self.height)
+                out.append((x0, y0, x1 - x0, y1 - y0, seed))
+        return out
+
+    def read_tiles(self) -> np.ndarray:
+        """D_neg of every tile of the last render, concatenated in frame order."""
+        n = C.c_int64()
+        check(L.lib().fg_frame_read_tiles(self.engine._h, self._h, 0, None, C.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        check(L.lib().fg_frame_read_tiles(self.engine._h, self._h, n.value, ptr(out, L._f32p), None))
         return out
 
     def counts(self) -> np.ndarray:

@@ -262,7 +262,12 @@ fg_status fg_render_tile_all_gpu(fg_engine *engine, const float *tile_img,
 
 /* A frame of width x height pixels cut into tile x tile tiles (edge tiles partial),
  * one emulsion per tile (seed per image_pipeline.rs:430-434), pixel units scaled by
- * pixel_size.  density_mult (per tile, NULL = 1) scales crystal density. */
+ * pixel_size.  density_mult (per tile, NULL = 1) scales crystal density.
+ * overlap > 0 lays the tiles out as the reference's tiled renderer does
+ * (image_pipeline.rs:397-421): tile (tx, ty) spans [tx*tile - overlap, tx*tile + tile +
+ * overlap) clipped to the frame, each with its own D_neg (fg_frame_read_tiles), blended
+ * by fg_frame_read or fg_film_composite; it needs 2 * overlap <= tile and scan.otf = 0
+ * (the scanner OTF then applies to the composited grain). */
 typedef struct fg_frame_desc {
     uint32_t width, height, tile;
     uint32_t threads;             /* host threads for synthesis; 0 = all */
@@ -276,7 +281,7 @@ typedef struct fg_frame_desc {
     const float *density_mult;    /* tiles, or NULL */
     uint32_t synthesis;           /* 0 host Bridson (reference algorithm), 1 device dart
                                      throwing (fg_emulsion_synthesize_device) */
-    uint32_t pad_;
+    uint32_t overlap;             /* tile overlap margin in pixels; 0 = abutting tiles */
 } fg_frame_desc;
 
 typedef struct fg_frame_stats {
@@ -291,8 +296,14 @@ fg_status fg_frame_create(fg_engine *engine, const fg_frame_desc *desc, fg_frame
 void fg_frame_destroy(fg_frame *frame);
 fg_status fg_frame_render(fg_engine *engine, fg_frame *frame, uint32_t mode,
                           fg_frame_stats *stats);
-/* Frame D_neg (width * height, row-major) of the last render. */
+/* Frame D_neg (width * height, row-major) of the last render; with overlapping tiles
+ * the Hann-weighted blend d_accum / max(weight, 1e-5) (image_pipeline.rs:593-654). */
 fg_status fg_frame_read(fg_engine *engine, const fg_frame *frame, float *out);
+/* D_neg of every tile of the last render, tiles in frame order, each tw * th row-major
+ * (tw, th the tile's clipped extent); n = the total.  Returns the total in *n_total when
+ * out is NULL. */
+fg_status fg_frame_read_tiles(fg_engine *engine, const fg_frame *frame, int64_t n, float *out,
+                              int64_t *n_total);
 /* Per-tile latent counts of the last render, crystals in tile order then the tile's
  * crystal order; n must equal the frame's crystal count. */
 fg_status fg_frame_read_counts(fg_engine *engine, const fg_frame *frame, int64_t n,
@@ -358,6 +369,93 @@ fg_status fg_vjp(fg_engine *engine, const fg_emulsion *em, const fg_expected_mod
                  const float *exposure_map, uint32_t tw, uint32_t th, const float *dose,
                  const float *grad_out, float *grad_exposure, fg_param_grads *grads,
                  float *grad_dose, fg_adjoint_stats *stats);
+
+/* ---- full-frame black-and-white film render ---------------------------------------
+ * render_film_black_and_white_tiled (image_pipeline.rs:299-904) with every stage on the
+ * GPU: panchromatic luma (:259-296), camera and gelatin optics (:133-252), overlapping
+ * tiles rendered as one batched frame (:380-592, render_tile_all_gpu per tile), Hann
+ * accumulation (:593-649), low-frequency separation (:651-706), sensitometric
+ * compositing (:708-820), scanner OTF (:822-855) and the 8-bit output (:857-891).
+ * Resampling to another output resolution and supersampling (:306-343) are the
+ * image crate's filters and stay with the caller: the input is at output resolution. */
+typedef struct fg_film_options {   /* FilmRenderOptions (image_pipeline.rs:20-92) */
+    char preset[32];               /* "tri-x" (default), "tri-x-1600", "t-max", "hp5", ... */
+    double grain_size_px;          /* 2.2 */
+    double grain_strength;         /* 1.0 */
+    double exposure;               /* 1.0 */
+    double scale;                  /* 1.0 */
+    double push;                   /* 0.0 */
+    uint64_t seed;                 /* 42 */
+    uint32_t tile_size;            /* 320 (>= 48) */
+    uint32_t reference_res;        /* 6000 */
+    uint32_t auto_scale;           /* 0 */
+    uint32_t scanner_otf;          /* 1 */
+    uint32_t synthesis;            /* 0 host Bridson, 1 device dart throwing */
+    uint32_t frame_mode;           /* FG_FRAME_BATCHED (default) or FG_FRAME_PLAIN */
+    uint32_t threads;              /* host threads for synthesis; 0 = all */
+    uint32_t pad_;
+} fg_film_options;
+
+/* Sizes render_film_black_and_white_tiled derives (image_pipeline.rs:346-385, :651-658). */
+typedef struct fg_film_plan {
+    double grain_size_px;          /* after manual / automatic scaling, in [0.05, 15] */
+    double strength;               /* effective grain strength */
+    double push;                   /* effective push */
+    double pixel_size_um;          /* 1 / grain_size_px */
+    uint32_t tile_step, pad, tiles_x, tiles_y;
+    float filter_sigma;            /* low-frequency Gaussian */
+    uint32_t k_rad;
+    float base_nominal;            /* per-preset grain amplitude before push */
+    uint32_t pad_;
+} fg_film_plan;
+
+/* ImageProcessReport (image_pipeline.rs:95-105) with the real development counts of
+ * every tile, the compositing statistics and per-stage times. */
+typedef struct fg_film_report {
+    uint32_t in_w, in_h, out_w, out_h;
+    uint64_t total_tiles, total_crystals, developed_crystals, total_clumps;
+    uint64_t photons, rays;
+    double elapsed_seconds;
+    float high_mean, high_std, norm_factor, pad_;
+    float ms_luma_optics, ms_synthesis, ms_render, ms_composite;
+    float ms_kernel_optics, ms_kernel_composite;   /* HIP events around the kernels */
+    float ms_kernel_transport, ms_kernel_scan;
+} fg_film_report;
+
+/* Compositing statistics of fg_film_composite. */
+typedef struct fg_composite_stats {
+    float high_mean, high_std, norm_factor, filter_sigma;
+    uint32_t k_rad, tiles_x, tiles_y, pad_;
+    float ms_kernels;              /* HIP events around the compositing kernels */
+    float pad2_;
+} fg_composite_stats;
+
+/* FilmRenderOptions::default() (image_pipeline.rs:72-92). */
+fg_status fg_film_options_default(fg_film_options *opts);
+/* Derived sizes for a width x height frame; host only, no device work. */
+fg_status fg_film_plan_for(const fg_film_options *opts, uint32_t width, uint32_t height,
+                           fg_film_plan *plan);
+/* The whole render: pixels is width x height gray (channels 1) or interleaved RGB
+ * (channels 3); out receives width x height 8-bit values. */
+fg_status fg_render_film_bw(fg_engine *engine, const uint8_t *pixels, uint32_t width, uint32_t height,
+                            uint32_t channels, const fg_film_options *opts, uint8_t *out,
+                            fg_film_report *report);
+/* Stage entry points (host buffers in and out): */
+/* convert_to_panchromatic_luma of interleaved RGB for a preset's spectral weights */
+fg_status fg_film_luma(fg_engine *engine, const uint8_t *rgb, uint32_t width, uint32_t height,
+                       const char *preset, uint8_t *gray);
+/* apply_camera_and_gelatin_optics: h_effective (width * height f32) of a gray image */
+fg_status fg_film_optics(fg_engine *engine, const uint8_t *gray, uint32_t width, uint32_t height,
+                         float *h_effective);
+/* Compositing of rendered tiles for plan's layout: tiles = every tile's D_neg in frame
+ * order (fg_frame_read_tiles' layout, n_tile_values in total), gray and h_effective at
+ * width * height.  Optional outputs (NULL to skip), width * height f32 each: d_norm
+ * (Hann blend), d_high (d_norm minus its Gaussian low-pass), grain (the grain field
+ * after the scanner OTF when the plan's options enable it). */
+fg_status fg_film_composite(fg_engine *engine, const fg_film_options *opts, uint32_t width,
+                            uint32_t height, const uint8_t *gray, const float *h_effective,
+                            const float *tiles, int64_t n_tile_values, uint8_t *out,
+                            float *d_norm, float *d_high, float *grain, fg_composite_stats *stats);
 
 #ifdef __cplusplus
 }

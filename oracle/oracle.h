@@ -155,6 +155,38 @@ int orc_replay_grad(const orc_crystals *cs, const float *map, const orc_expected
                     const double *gdose, double *gmap, double grads[3]);
 double orc_hg_score(double g, double mu);
 
+/* --- full-frame black-and-white compositing (film.c, image_pipeline.rs:133-891) ------ */
+typedef struct {
+    double grain_size_px;   /* after manual and automatic scaling */
+    double strength;        /* effective grain strength */
+    uint32_t tile_step, pad, tiles_x, tiles_y;
+} orc_film_plan_out;
+
+typedef struct {
+    int32_t width, height, tile_step, pad;
+    double grain_size_px;
+    float strength;         /* effective grain strength */
+    float base_nominal;     /* per preset: t-max 0.024, hp5 0.032, tri-x-1600 0.042, else 0.038 */
+    float push;             /* effective push */
+    int32_t scanner_otf;
+    int32_t fold;           /* 0 sequential f32 statistics (reference), 1 f64 sums */
+} orc_film_cfg;
+
+typedef struct {
+    float high_mean, high_std, norm_factor, filter_sigma;
+    int32_t k_rad, tiles_x, tiles_y, pad_;
+} orc_film_stats;
+
+void orc_film_luma(const uint8_t *rgb, int64_t n, float rw, float gw, float bw, uint8_t *out);
+int orc_film_plan(double grain_size_px, double grain_strength, double scale, int32_t auto_scale,
+                  uint32_t reference_res, uint32_t tile_size, uint32_t out_w, uint32_t out_h,
+                  orc_film_plan_out *p);
+void orc_film_optics(const uint8_t *src, int32_t w, int32_t h, float *h_eff, int32_t margins[4]);
+int64_t orc_film_tile_values(int32_t w, int32_t h, int32_t step, int32_t pad);
+int orc_film_composite(const orc_film_cfg *c, const uint8_t *src, const float *h_eff, const float *tiles,
+                       uint8_t *out, float *d_norm, float *d_low, float *grain, float *scanned,
+                       float *tone, orc_film_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,23 @@ class OrcExpectedCfg(C.Structure):
                 ("spp", C.c_int32), ("ss", C.c_int32), ("scan_seed", C.c_uint64)]
 
 
+class OrcFilmPlan(C.Structure):
+    _fields_ = [("grain_size_px", C.c_double), ("strength", C.c_double), ("tile_step", C.c_uint32),
+                ("pad", C.c_uint32), ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32)]
+
+
+class OrcFilmCfg(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("tile_step", C.c_int32), ("pad", C.c_int32),
+                ("grain_size_px", C.c_double), ("strength", C.c_float), ("base_nominal", C.c_float),
+                ("push", C.c_float), ("scanner_otf", C.c_int32), ("fold", C.c_int32)]
+
+
+class OrcFilmStats(C.Structure):
+    _fields_ = [("high_mean", C.c_float), ("high_std", C.c_float), ("norm_factor", C.c_float),
+                ("filter_sigma", C.c_float), ("k_rad", C.c_int32), ("tiles_x", C.c_int32),
+                ("tiles_y", C.c_int32), ("pad_", C.c_int32)]
+
+
 def build() -> str:
     """Compile liboracle.so with the committed Makefile (gcc, -ffp-contract=off)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -134,6 +151,14 @@ def lib():
                                       _f64p, _f64p, _f64p]
         L.orc_hg_score.argtypes = [C.c_double, C.c_double]
         L.orc_hg_score.restype = C.c_double
+        L.orc_film_luma.argtypes = [_u8p, C.c_int64, C.c_float, C.c_float, C.c_float, _u8p]
+        L.orc_film_plan.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_uint32, C.POINTER(OrcFilmPlan)]
+        L.orc_film_optics.argtypes = [_u8p, C.c_int32, C.c_int32, _f32p, _i32p]
+        L.orc_film_tile_values.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_film_tile_values.restype = C.c_int64
+        L.orc_film_composite.argtypes = [C.POINTER(OrcFilmCfg), _u8p, _f32p, _f32p, _u8p, _f32p, _f32p,
+                                         _f32p, _f32p, _f32p, C.POINTER(OrcFilmStats)]
         _lib = L
     return _lib
 
@@ -495,3 +520,100 @@ def expected_vjp(cs: Crystals, exposure_map, cfg: OrcExpectedCfg, threshold, mod
 
 def hg_score(g, mu):
     return lib().orc_hg_score(g, mu)
+
+
+# ------------------------------------------------------------- film compositing -----
+# Per-preset constants of render_film_black_and_white_tiled (image_pipeline.rs:264-283,
+# :726-731).
+FILM_LUMA_WEIGHTS = {
+    "tri-x": (0.42, 0.46, 0.12), "trix": (0.42, 0.46, 0.12), "tri-x-400": (0.42, 0.46, 0.12),
+    "tri-x-1600": (0.42, 0.46, 0.12), "trix-1600": (0.42, 0.46, 0.12), "caffenol": (0.42, 0.46, 0.12),
+    "hp5": (0.36, 0.48, 0.16), "hp5-plus": (0.36, 0.48, 0.16),
+    "t-max": (0.33, 0.52, 0.15), "tmax": (0.33, 0.52, 0.15),
+}
+
+
+def film_luma_weights(preset: str):
+    return FILM_LUMA_WEIGHTS.get(preset.lower(), (0.38, 0.48, 0.14))
+
+
+def film_base_nominal(preset: str) -> float:
+    p = preset.lower()
+    if p in ("t-max", "tmax"):
+        return 0.024
+    if p in ("hp5", "hp5-plus"):
+        return 0.032
+    if p in ("tri-x-1600", "trix-1600", "caffenol"):
+        return 0.042
+    return 0.038
+
+
+def film_effective_push(preset: str, push: float) -> float:
+    p = preset.lower()
+    return max(push, 2.0) if ("1600" in p or p == "caffenol") else max(push, 0.0)
+
+
+def film_luma(rgb, preset: str):
+    """convert_to_panchromatic_luma of an (h, w, 3) u8 image."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    out = np.zeros(rgb.shape[:2], np.uint8)
+    rw, gw, bw = film_luma_weights(preset)
+    lib().orc_film_luma(_p(rgb, _u8p), out.size, rw, gw, bw, _p(out, _u8p))
+    return out
+
+
+def film_plan(w, h, grain_size_px=2.2, grain_strength=1.0, scale=1.0, auto_scale=False,
+              reference_res=6000, tile_size=320) -> dict:
+    p = OrcFilmPlan()
+    if lib().orc_film_plan(grain_size_px, grain_strength, scale, int(auto_scale), reference_res,
+                           tile_size, w, h, C.byref(p)) != 0:
+        raise ValueError("tile_size below 48")
+    return {k: getattr(p, k) for k, _ in OrcFilmPlan._fields_}
+
+
+def film_optics(src):
+    """apply_camera_and_gelatin_optics: (h_effective f32, (top, bottom, left, right))."""
+    src = np.ascontiguousarray(src, np.uint8)
+    h, w = src.shape
+    out = np.zeros((h, w), np.float32)
+    m = np.zeros(4, np.int32)
+    lib().orc_film_optics(_p(src, _u8p), w, h, _p(out, _f32p), _p(m, _i32p))
+    return out, tuple(int(v) for v in m)
+
+
+def film_tile_spans(w, h, step, pad):
+    """(x_start, y_start, tw, th) per tile in tile order (image_pipeline.rs:397-421)."""
+    out = []
+    for ty in range(-(-h // step)):
+        for tx in range(-(-w // step)):
+            x0, y0 = tx * step, ty * step
+            xs, ys = max(x0 - pad, 0), max(y0 - pad, 0)
+            out.append((xs, ys, min(x0 + step + pad, w) - xs, min(y0 + step + pad, h) - ys))
+    return out
+
+
+def film_composite(src, h_eff, tiles, *, tile_step, pad, grain_size_px, strength=1.0,
+                   preset="tri-x", push=0.0, scanner_otf=True, fold=0, intermediates=False):
+    """Compositing of the rendered tiles (image_pipeline.rs:593-891).  tiles: concatenated
+    per-tile D_neg in tile order.  Returns (u8 image, stats dict[, intermediates dict])."""
+    src = np.ascontiguousarray(src, np.uint8)
+    h, w = src.shape
+    h_eff = np.ascontiguousarray(h_eff, np.float32)
+    tiles = np.ascontiguousarray(tiles, np.float32).reshape(-1)
+    need = lib().orc_film_tile_values(w, h, tile_step, pad)
+    if tiles.size != need:
+        raise ValueError(f"tiles hold {tiles.size} values, layout needs {need}")
+    c = OrcFilmCfg(w, h, tile_step, pad, grain_size_px, strength, film_base_nominal(preset),
+                   film_effective_push(preset, push), int(bool(scanner_otf)), fold)
+    out = np.zeros((h, w), np.uint8)
+    bufs = {k: np.zeros((h, w), np.float32) for k in ("d_norm", "d_low", "grain", "scanned", "tone")} \
+        if intermediates else {}
+    st = OrcFilmStats()
+    r = lib().orc_film_composite(C.byref(c), _p(src, _u8p), _p(h_eff, _f32p), _p(tiles, _f32p),
+                                 _p(out, _u8p), *[_p(bufs.get(k), _f32p) for k in
+                                                  ("d_norm", "d_low", "grain", "scanned", "tone")],
+                                 C.byref(st))
+    if r != 0:
+        raise ValueError("bad compositing layout")
+    stats = {k: getattr(st, k) for k, _ in OrcFilmStats._fields_ if k != "pad_"}
+    return (out, stats, bufs) if intermediates else (out, stats)

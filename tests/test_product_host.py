@@ -41,6 +41,8 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         "fg_tile_stats": _lib.TileStatsC, "fg_frame_desc": _lib.FrameDescC,
         "fg_frame_stats": _lib.FrameStatsC, "fg_expected_model": _lib.ExpectedModelC,
         "fg_param_grads": _lib.ParamGradsC, "fg_adjoint_stats": _lib.AdjointStatsC,
+        "fg_film_options": _lib.FilmOptionsC, "fg_film_plan": _lib.FilmPlanC,
+        "fg_film_report": _lib.FilmReportC, "fg_composite_stats": _lib.CompositeStatsC,
     }
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "fgrain_hip.h"\nint main(void){\n' +
@@ -252,3 +254,44 @@ def test_smooth_field_range_and_packet_weight():
     cfg = W.load_config("tile512_trix_adjoint")
     assert W.packet_weight(cfg) == pytest.approx(0.10417 / 256)
     assert W.packet_weight(W.load_config("tile320_trix_cubic")) == 1.0
+
+
+@pytest.mark.parametrize("kw", [
+    dict(), dict(grain_size_px=0.5), dict(grain_size_px=0.05, tile_size=1000), dict(grain_size_px=15.0),
+    dict(grain_size_px=2.2, auto_scale=True, reference_res=3000), dict(scale=2.0, grain_strength=0.7),
+    dict(preset="tri-x-1600", push=0.5), dict(preset="t-max", push=1.0), dict(preset="hp5"),
+])
+def test_film_plan_follows_the_reference(kw):
+    """fg_film_plan_for (host logic) against the oracle's restatement of
+    image_pipeline.rs:346-385 and :651-658."""
+    from fgrain_amd import film as FM
+    opts = FM.FilmRenderOptions(**kw)
+    for w, h in [(6000, 4000), (640, 480), (97, 61)]:
+        p = FM.film_plan(opts, w, h)
+        o = O.film_plan(w, h, grain_size_px=opts.grain_size_px, grain_strength=opts.grain_strength,
+                        scale=opts.scale, auto_scale=opts.auto_scale, reference_res=opts.reference_res,
+                        tile_size=opts.tile_size)
+        assert p["grain_size_px"] == o["grain_size_px"] and p["strength"] == o["strength"]
+        assert (p["tile_step"], p["pad"], p["tiles_x"], p["tiles_y"]) == \
+            (o["tile_step"], o["pad"], o["tiles_x"], o["tiles_y"])
+        assert p["push"] == O.film_effective_push(opts.preset, opts.push)
+        assert p["base_nominal"] == pytest.approx(O.film_base_nominal(opts.preset))
+        # the Gaussian of the low-frequency separation, as the oracle composites it
+        _, st = O.film_composite(np.zeros((8, 8), np.uint8), np.zeros((8, 8), np.float32),
+                                 np.zeros(64, np.float32), tile_step=48, pad=12,
+                                 grain_size_px=p["grain_size_px"])
+        assert p["filter_sigma"] == st["filter_sigma"] and p["k_rad"] == st["k_rad"]
+
+
+def test_film_options_validation():
+    from fgrain_amd import film as FM
+    with pytest.raises(F.GpuError):
+        FM.film_plan(FM.FilmRenderOptions(tile_size=40), 100, 100)
+    with pytest.raises(F.GpuError):
+        FM.film_plan(FM.FilmRenderOptions(grain_size_px=0.0), 100, 100)
+    with pytest.raises(F.GpuError):
+        FM.film_plan(FM.FilmRenderOptions(), 0, 100)
+    with pytest.raises(NotImplementedError):
+        FM.render_film_black_and_white_tiled(np.zeros((8, 8), np.uint8), FM.FilmRenderOptions(color=True))
+    with pytest.raises(NotImplementedError):
+        FM.render_film_black_and_white_tiled(np.zeros((8, 8), np.uint8), FM.FilmRenderOptions(supersample=2))
