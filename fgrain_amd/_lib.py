@@ -210,8 +210,8 @@ SIGNATURES = {
     "fg_film_plan_for": (_st, [C.POINTER(FilmOptionsC), C.c_uint32, C.c_uint32, C.POINTER(FilmPlanC)]),
     "fg_render_film_bw": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(FilmOptionsC),
                                 _u8p, C.POINTER(FilmReportC)]),
-    "fg_film_luma": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, C.c_char_p, _u8p]),
-    "fg_film_optics": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, _f32p]),
+    "fg_film_luma": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, C.c_char_p, _u8p, _f32p]),
+    "fg_film_optics": (_st, [_vp, _u8p, C.c_uint32, C.c_uint32, _f32p, _f32p]),
     "fg_film_composite": (_st, [_vp, C.POINTER(FilmOptionsC), C.c_uint32, C.c_uint32, _u8p, _f32p, _f32p,
                                 C.c_int64, _u8p, _f32p, _f32p, _f32p, C.POINTER(CompositeStatsC)]),
 }

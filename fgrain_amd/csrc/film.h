@@ -46,8 +46,10 @@ inline void film_span(uint32_t t, uint32_t step, uint32_t pad, uint32_t size, ui
     len = (uint32_t)(e - s);
 }
 
-// bounding box of the nonzero pixels (top, bottom, left, right) into bbox[4] and the
-// optics into h_eff (image_pipeline.rs:137-249)
+// bounding box of the nonzero pixels (first row, last row, first column, last column)
+// into bbox[0..3] and the optics into h_eff (image_pipeline.rs:137-249); bbox holds
+// film_bbox_size(W, H) ints (the per-block boxes follow the first four); W * H <= 2^31
+size_t film_bbox_size(uint32_t W, uint32_t H);
 hipError_t launch_film_optics(const uint8_t *gray, uint32_t W, uint32_t H, int32_t *bbox, float *h_eff,
                               hipStream_t s);
 // convert_to_panchromatic_luma of interleaved RGB (image_pipeline.rs:284-293)

@@ -23,22 +23,13 @@ namespace fg {
 namespace {
 
 constexpr float kPi = 3.14159265358979323846f;
-constexpr int kOX = 64, kOY = 16;     // optics / final output tile
+constexpr int kOX = 64, kOY = 16;     // final output tile
+constexpr int kPY = 32;               // optics output tile rows (64 x 32)
 constexpr int kVX = 64, kVY = 64;     // vertical Gaussian tile
 constexpr int kHX = 256;              // horizontal Gaussian segment
 constexpr int kMaxRad = 48;
 
 // ------------------------------------------------------------------ gate and optics
-__global__ void k_bbox_init(int32_t *bbox, uint32_t W, uint32_t H)
-{
-    if (threadIdx.x == 0) {
-        bbox[0] = (int32_t)H;
-        bbox[1] = -1;
-        bbox[2] = (int32_t)W;
-        bbox[3] = -1;
-    }
-}
-
 __device__ __forceinline__ int wave_min(int v)
 {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -51,39 +42,94 @@ __device__ __forceinline__ int wave_max(int v)
 }
 
 // bounding box of the nonzero pixels: rows and columns outside it are the camera gate's
-// all-zero margins (image_pipeline.rs:137-153).  16 pixels per lane.
-__global__ void __launch_bounds__(256) k_bbox(const uint8_t *gray, uint32_t W, uint64_t n, int32_t *bbox)
+// all-zero margins (image_pipeline.rs:137-153).  16 pixels per lane (one 16-byte load),
+// wave then block min/max, one partial box per block (part[4 b .. 4 b + 3]); no atomics.
+__global__ void __launch_bounds__(256) k_bbox(const uint8_t *gray, uint32_t W, uint32_t n, int32_t *part)
 {
-    const uint64_t i0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 16;
+    __shared__ int red[4][4];
+    const uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 16u;
     int rmin = INT_MAX, rmax = -1, cmin = INT_MAX, cmax = -1;
     if (i0 < n) {
         uint8_t v[16];
-        if (i0 + 16 <= n) {
+        if (i0 + 16u <= n) {
             const uint4 q = *reinterpret_cast<const uint4 *>(gray + i0);
             const uint32_t w[4] = {q.x, q.y, q.z, q.w};
             for (int k = 0; k < 16; k++) v[k] = (uint8_t)(w[k >> 2] >> (8 * (k & 3)));
         } else {
             for (int k = 0; k < 16; k++) v[k] = i0 + k < n ? gray[i0 + k] : 0;
         }
-        for (int k = 0; k < 16; k++)
+        uint32_t r = i0 / W, c = i0 - r * W;
+        for (int k = 0; k < 16; k++) {
             if (v[k]) {
-                const uint64_t i = i0 + k;
-                const int r = (int)(i / W), c = (int)(i % W);
-                rmin = min(rmin, r);
-                rmax = max(rmax, r);
-                cmin = min(cmin, c);
-                cmax = max(cmax, c);
+                rmin = min(rmin, (int)r);
+                rmax = max(rmax, (int)r);
+                cmin = min(cmin, (int)c);
+                cmax = max(cmax, (int)c);
             }
+            if (++c == W) {
+                c = 0;
+                r++;
+            }
+        }
     }
     rmin = wave_min(rmin);
     rmax = wave_max(rmax);
     cmin = wave_min(cmin);
     cmax = wave_max(cmax);
-    if ((threadIdx.x & 63) == 0 && rmax >= 0) {
-        atomicMin(bbox + 0, rmin);
-        atomicMax(bbox + 1, rmax);
-        atomicMin(bbox + 2, cmin);
-        atomicMax(bbox + 3, cmax);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wv] = rmin;
+        red[1][wv] = rmax;
+        red[2][wv] = cmin;
+        red[3][wv] = cmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t *o = part + 4u * blockIdx.x;
+        o[0] = min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3]));
+        o[1] = max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3]));
+        o[2] = min(min(red[2][0], red[2][1]), min(red[2][2], red[2][3]));
+        o[3] = max(max(red[3][0], red[3][1]), max(red[3][2], red[3][3]));
+    }
+}
+
+// the frame's box from the per-block boxes: bbox = (first row, last row, first column,
+// last column) of the nonzero pixels, (H, -1, W, -1) when there are none
+__global__ void __launch_bounds__(1024) k_bbox_reduce(const int32_t *part, uint32_t nb, uint32_t W, uint32_t H,
+                                                      int32_t *bbox)
+{
+    __shared__ int red[4][16];
+    int rmin = INT_MAX, rmax = -1, cmin = INT_MAX, cmax = -1;
+    for (uint32_t b = threadIdx.x; b < nb; b += 1024u) {
+        rmin = min(rmin, part[4 * b]);
+        rmax = max(rmax, part[4 * b + 1]);
+        cmin = min(cmin, part[4 * b + 2]);
+        cmax = max(cmax, part[4 * b + 3]);
+    }
+    rmin = wave_min(rmin);
+    rmax = wave_max(rmax);
+    cmin = wave_min(cmin);
+    cmax = wave_max(cmax);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wv] = rmin;
+        red[1][wv] = rmax;
+        red[2][wv] = cmin;
+        red[3][wv] = cmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 16; k++) {
+            red[0][0] = min(red[0][0], red[0][k]);
+            red[1][0] = max(red[1][0], red[1][k]);
+            red[2][0] = min(red[2][0], red[2][k]);
+            red[3][0] = max(red[3][0], red[3][k]);
+        }
+        const bool any = red[1][0] >= 0;
+        bbox[0] = any ? red[0][0] : (int32_t)H;
+        bbox[1] = red[1][0];
+        bbox[2] = any ? red[2][0] : (int32_t)W;
+        bbox[3] = red[3][0];
     }
 }
 
@@ -95,21 +141,33 @@ __global__ void __launch_bounds__(256) k_film_optics(const uint8_t *gray, uint32
 {
     constexpr float KN[3] = {0.15f, 0.70f, 0.15f};
     constexpr float KW[7] = {0.03f, 0.12f, 0.20f, 0.30f, 0.20f, 0.12f, 0.03f};
-    __shared__ float hl[kOY + 6][kOX + 6];
-    __shared__ float tn[kOY + 2][kOX];
-    __shared__ float tw[kOY + 6][kOX];
+    __shared__ float hl[kPY + 6][kOX + 6];
+    __shared__ float tn[kPY + 2][kOX];
+    __shared__ float tw[kPY + 6][kOX];
+    __shared__ float lut[256];   // v / 255 for every 8-bit value, correctly rounded
     const int top = bbox[0], bottom = bbox[1] + 1, left = bbox[2], right = bbox[3] + 1;
-    const int x0 = blockIdx.x * kOX, y0 = blockIdx.y * kOY;
+    const int x0 = blockIdx.x * kOX, y0 = blockIdx.y * kPY;
     const int t = threadIdx.x;
-    for (int i = t; i < (kOY + 6) * (kOX + 6); i += 256) {
+    lut[t] = (float)t / 255.0f;
+    // all of a lane's tile loads are issued before the first is used
+    constexpr int kTile = (kPY + 6) * (kOX + 6), kPer = (kTile + 255) / 256;
+    uint32_t g8[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int i = t + 256 * j;
         const int r = i / (kOX + 6), c = i % (kOX + 6);
         const int y = y0 - 3 + r, x = x0 - 3 + c;
-        float v = 0.0f;
-        if (y >= top && y < bottom && x >= left && x < right) v = (float)gray[(size_t)y * W + x] / 255.0f;
-        hl[r][c] = v;
+        g8[j] = 0xFFFFFFFFu;   // outside the gate
+        if (i < kTile && y >= top && y < bottom && x >= left && x < right) g8[j] = gray[(size_t)y * W + x];
     }
     __syncthreads();
-    for (int i = t; i < (kOY + 2) * kOX; i += 256) {
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int i = t + 256 * j;
+        if (i < kTile) hl[i / (kOX + 6)][i % (kOX + 6)] = g8[j] == 0xFFFFFFFFu ? 0.0f : lut[g8[j]];
+    }
+    __syncthreads();
+    for (int i = t; i < (kPY + 2) * kOX; i += 256) {
         const int r = i / kOX, c = i % kOX;
         const int y = y0 - 1 + r, x = x0 + c;
         float v = 0.0f;
@@ -126,7 +184,7 @@ __global__ void __launch_bounds__(256) k_film_optics(const uint8_t *gray, uint32
         }
         tn[r][c] = v;
     }
-    for (int i = t; i < (kOY + 6) * kOX; i += 256) {
+    for (int i = t; i < (kPY + 6) * kOX; i += 256) {
         const int r = i / kOX, c = i % kOX;
         const int y = y0 - 3 + r, x = x0 + c;
         float v = 0.0f;
@@ -144,7 +202,7 @@ __global__ void __launch_bounds__(256) k_film_optics(const uint8_t *gray, uint32
         tw[r][c] = v;
     }
     __syncthreads();
-    for (int i = t; i < kOY * kOX; i += 256) {
+    for (int i = t; i < kPY * kOX; i += 256) {
         const int r = i / kOX, c = i % kOX;
         const int y = y0 + r, x = x0 + c;
         if (y >= (int)H || x >= (int)W) continue;
@@ -173,16 +231,33 @@ __global__ void __launch_bounds__(256) k_film_optics(const uint8_t *gray, uint32
     }
 }
 
-// convert_to_panchromatic_luma (image_pipeline.rs:284-293)
+__device__ __forceinline__ uint32_t luma_of(uint32_t r8, uint32_t g8, uint32_t b8, float rw, float gw, float bw)
+{
+    float l = (float)r8 * rw + (float)g8 * gw + (float)b8 * bw;
+    l = fminf(fmaxf(l, 0.0f), 255.0f);
+    return (uint32_t)roundf(l);
+}
+
+// convert_to_panchromatic_luma (image_pipeline.rs:284-293), 4 pixels per lane: three
+// 4-byte loads of interleaved RGB, one 4-byte store; the last n % 4 pixels one by one
 __global__ void __launch_bounds__(256) k_film_luma(const uint8_t *rgb, uint64_t n, float rw, float gw, float bw,
                                                    uint8_t *gray)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float r = (float)rgb[3 * i], g = (float)rgb[3 * i + 1], b = (float)rgb[3 * i + 2];
-    float l = r * rw + g * gw + b * bw;
-    l = fminf(fmaxf(l, 0.0f), 255.0f);
-    gray[i] = (uint8_t)roundf(l);
+    const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t i = 4 * q;
+    if (i + 4 <= n) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(rgb + 3 * i);
+        const uint32_t w0 = src[0], w1 = src[1], w2 = src[2];
+        const uint32_t b[12] = {w0 & 255u, (w0 >> 8) & 255u, (w0 >> 16) & 255u, w0 >> 24,
+                                w1 & 255u, (w1 >> 8) & 255u, (w1 >> 16) & 255u, w1 >> 24,
+                                w2 & 255u, (w2 >> 8) & 255u, (w2 >> 16) & 255u, w2 >> 24};
+        uint32_t o = 0;
+        for (int k = 0; k < 4; k++) o |= luma_of(b[3 * k], b[3 * k + 1], b[3 * k + 2], rw, gw, bw) << (8 * k);
+        *reinterpret_cast<uint32_t *>(gray + i) = o;
+    } else {
+        for (uint64_t p = i; p < n; p++)
+            gray[p] = (uint8_t)luma_of(rgb[3 * p], rgb[3 * p + 1], rgb[3 * p + 2], rw, gw, bw);
+    }
 }
 
 // ------------------------------------------------------------------ Hann blend
@@ -467,14 +542,18 @@ inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 
 }  // namespace
 
+size_t film_bbox_size(uint32_t W, uint32_t H) { return 4u + 4u * (size_t)(((uint64_t)W * H + 4095) / 4096); }
+
 hipError_t launch_film_optics(const uint8_t *gray, uint32_t W, uint32_t H, int32_t *bbox, float *h_eff,
                               hipStream_t s)
 {
     const uint64_t n = (uint64_t)W * H;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, bbox, W, H);
-    hipLaunchKernelGGL(k_bbox, dim3((uint32_t)((n + 4095) / 4096)), dim3(256), 0, s, gray, W, n, bbox);
-    hipLaunchKernelGGL(k_film_optics, dim3(cdiv(W, kOX), cdiv(H, kOY)), dim3(256), 0, s, gray, W, H, bbox, h_eff);
+    if (n > 0x80000000ull) return hipErrorInvalidValue;
+    const uint32_t nb = (uint32_t)((n + 4095) / 4096);
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, s, gray, W, (uint32_t)n, bbox + 4);
+    hipLaunchKernelGGL(k_bbox_reduce, dim3(1), dim3(1024), 0, s, bbox + 4, nb, W, H, bbox);
+    hipLaunchKernelGGL(k_film_optics, dim3(cdiv(W, kOX), cdiv(H, kPY)), dim3(256), 0, s, gray, W, H, bbox, h_eff);
     return hipGetLastError();
 }
 
@@ -482,7 +561,8 @@ hipError_t launch_film_luma(const uint8_t *rgb, uint64_t n, float rw, float gw, 
                             hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_film_luma, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, rgb, n, rw, gw, bw, gray);
+    const uint64_t quads = (n + 3) / 4;
+    hipLaunchKernelGGL(k_film_luma, dim3((uint32_t)((quads + 255) / 256)), dim3(256), 0, s, rgb, n, rw, gw, bw, gray);
     return hipGetLastError();
 }
 

@@ -176,6 +176,7 @@ fg_status composite_device(fg_engine *E, const fg_film_plan &plan, uint32_t otf,
     if (want_grain) HIP_TRY(E->film_grain.ensure(n * 4), FG_ERR_BUFFER_RANGE);
     HIP_TRY(hipMemcpyAsync(E->film_kern.p, k.data(), k.size() * 4, hipMemcpyHostToDevice, E->stream),
             FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);   // start event on an idle stream
     fg::FilmParams P{};
     P.strength = (float)plan.strength;
     P.base_amp = plan.base_nominal * (1.0f + 0.15f * (float)plan.push);
@@ -312,7 +313,7 @@ extern "C" fg_status fg_film_plan_for(const fg_film_options *o, uint32_t W, uint
 }
 
 extern "C" fg_status fg_film_luma(fg_engine *E, const uint8_t *rgb, uint32_t W, uint32_t H, const char *preset,
-                                  uint8_t *gray)
+                                  uint8_t *gray, float *ms_kernel)
 {
     std::string msg;
     if (!E || !rgb || !gray || !preset) return fail(FG_ERR_INVALID_ARG, "fg_film_luma: null argument");
@@ -320,14 +321,27 @@ extern "C" fg_status fg_film_luma(fg_engine *E, const uint8_t *rgb, uint32_t W, 
     std::lock_guard<std::mutex> fl(E->film_mu);
     std::lock_guard<std::mutex> lk(E->mu);
     HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
-    fg_status s = upload_gray(E, rgb, W, H, 3, lower(preset));
-    if (s != FG_OK) return s;
-    HIP_TRY(hipMemcpyAsync(gray, E->film_gray.p, (size_t)W * H, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    const size_t n = (size_t)W * H;
+    float rw, gw, bw;
+    luma_weights(lower(preset), rw, gw, bw);
+    HIP_TRY(E->film_gray.ensure(n), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->film_rgb.ensure(3 * n), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipMemcpyAsync(E->film_rgb.p, rgb, 3 * n, hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    // the kernel's start event on an idle stream: after a staged (pageable) upload the
+    // next event would not carry a device timestamp of the kernel's start
     HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(fg::launch_film_luma(E->film_rgb.as<uint8_t>(), n, rw, gw, bw, E->film_gray.as<uint8_t>(), E->stream),
+            FG_ERR_PIPELINE);
+    HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(hipMemcpyAsync(gray, E->film_gray.p, n, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    if (ms_kernel) HIP_TRY(hipEventElapsedTime(ms_kernel, E->ev0, E->ev1), FG_ERR_CHANNEL);
     return FG_OK;
 }
 
-extern "C" fg_status fg_film_optics(fg_engine *E, const uint8_t *gray, uint32_t W, uint32_t H, float *h_eff)
+extern "C" fg_status fg_film_optics(fg_engine *E, const uint8_t *gray, uint32_t W, uint32_t H, float *h_eff,
+                                    float *ms_kernel)
 {
     std::string msg;
     if (!E || !gray || !h_eff) return fail(FG_ERR_INVALID_ARG, "fg_film_optics: null argument");
@@ -339,12 +353,16 @@ extern "C" fg_status fg_film_optics(fg_engine *E, const uint8_t *gray, uint32_t 
     fg_status s = upload_gray(E, gray, W, H, 1, "");
     if (s != FG_OK) return s;
     HIP_TRY(E->film_heff.ensure(n * 4), FG_ERR_BUFFER_RANGE);
-    HIP_TRY(E->film_bbox.ensure(16), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->film_bbox.ensure(fg::film_bbox_size(W, H) * 4), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);   // see fg_film_luma
+    HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
     HIP_TRY(fg::launch_film_optics(E->film_gray.as<uint8_t>(), W, H, E->film_bbox.as<int32_t>(),
                                    E->film_heff.as<float>(), E->stream),
             FG_ERR_PIPELINE);
+    HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
     HIP_TRY(hipMemcpyAsync(h_eff, E->film_heff.p, n * 4, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
     HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    if (ms_kernel) HIP_TRY(hipEventElapsedTime(ms_kernel, E->ev0, E->ev1), FG_ERR_CHANNEL);
     return FG_OK;
 }
 
@@ -423,7 +441,7 @@ extern "C" fg_status fg_render_film_bw(fg_engine *E, const uint8_t *pixels, uint
         fg_status s = upload_gray(E, pixels, W, H, channels, preset);
         if (s != FG_OK) return s;
         HIP_TRY(E->film_heff.ensure(n * 4), FG_ERR_BUFFER_RANGE);
-        HIP_TRY(E->film_bbox.ensure(16), FG_ERR_BUFFER_RANGE);
+        HIP_TRY(E->film_bbox.ensure(fg::film_bbox_size(W, H) * 4), FG_ERR_BUFFER_RANGE);
         HIP_TRY(fg::launch_film_optics(E->film_gray.as<uint8_t>(), W, H, E->film_bbox.as<int32_t>(),
                                        E->film_heff.as<float>(), E->stream),
                 FG_ERR_PIPELINE);

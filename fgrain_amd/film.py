@@ -144,24 +144,32 @@ def render_film_black_and_white_tiled(image, options: FilmRenderOptions | None =
     return rep, out
 
 
-def film_luma(rgb, preset="tri-x", engine: GpuEngine | None = None) -> np.ndarray:
-    """convert_to_panchromatic_luma of an (h, w, 3) uint8 image on the device."""
+def film_luma(rgb, preset="tri-x", engine: GpuEngine | None = None, timing: dict | None = None) -> np.ndarray:
+    """convert_to_panchromatic_luma of an (h, w, 3) uint8 image on the device;
+    timing["ms_kernel"] receives the kernel time when a dict is given."""
     rgb = np.ascontiguousarray(rgb, np.uint8)
     if rgb.ndim != 3 or rgb.shape[2] != 3:
         raise ValueError("rgb must be (h, w, 3) uint8")
     h, w = rgb.shape[:2]
     out = np.zeros((h, w), np.uint8)
+    ms = C.c_float()
     check(L.lib().fg_film_luma(_engine(engine)._h, ptr(rgb, L._u8p), w, h, preset.encode(),
-                               ptr(out, L._u8p)))
+                               ptr(out, L._u8p), C.byref(ms)))
+    if timing is not None:
+        timing["ms_kernel"] = ms.value
     return out
 
 
-def film_optics(gray, engine: GpuEngine | None = None) -> np.ndarray:
-    """apply_camera_and_gelatin_optics: h_effective (f32) of an (h, w) uint8 image."""
+def film_optics(gray, engine: GpuEngine | None = None, timing: dict | None = None) -> np.ndarray:
+    """apply_camera_and_gelatin_optics: h_effective (f32) of an (h, w) uint8 image;
+    timing["ms_kernel"] receives the gate + optics kernel time when a dict is given."""
     g = np.ascontiguousarray(gray, np.uint8)
     h, w = g.shape
     out = np.zeros((h, w), np.float32)
-    check(L.lib().fg_film_optics(_engine(engine)._h, ptr(g, L._u8p), w, h, ptr(out, L._f32p)))
+    ms = C.c_float()
+    check(L.lib().fg_film_optics(_engine(engine)._h, ptr(g, L._u8p), w, h, ptr(out, L._f32p), C.byref(ms)))
+    if timing is not None:
+        timing["ms_kernel"] = ms.value
     return out
 
 

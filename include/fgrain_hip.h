@@ -441,12 +441,14 @@ fg_status fg_render_film_bw(fg_engine *engine, const uint8_t *pixels, uint32_t w
                             uint32_t channels, const fg_film_options *opts, uint8_t *out,
                             fg_film_report *report);
 /* Stage entry points (host buffers in and out): */
-/* convert_to_panchromatic_luma of interleaved RGB for a preset's spectral weights */
+/* convert_to_panchromatic_luma of interleaved RGB for a preset's spectral weights;
+ * ms_kernel (optional) receives the kernel's time (HIP events) */
 fg_status fg_film_luma(fg_engine *engine, const uint8_t *rgb, uint32_t width, uint32_t height,
-                       const char *preset, uint8_t *gray);
-/* apply_camera_and_gelatin_optics: h_effective (width * height f32) of a gray image */
+                       const char *preset, uint8_t *gray, float *ms_kernel);
+/* apply_camera_and_gelatin_optics: h_effective (width * height f32) of a gray image;
+ * ms_kernel (optional) receives the time of the gate and optics kernels (HIP events) */
 fg_status fg_film_optics(fg_engine *engine, const uint8_t *gray, uint32_t width, uint32_t height,
-                         float *h_effective);
+                         float *h_effective, float *ms_kernel);
 /* Compositing of rendered tiles for plan's layout: tiles = every tile's D_neg in frame
  * order (fg_frame_read_tiles' layout, n_tile_values in total), gray and h_effective at
  * width * height.  Optional outputs (NULL to skip), width * height f32 each: d_norm

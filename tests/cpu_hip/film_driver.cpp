@@ -51,7 +51,7 @@ int main(int argc, char **argv)
     std::vector<float> heff(n), dn(n), tmp(n), dh(n), grain(n), fs(4);
     std::vector<double> part(film_partials_size(W, H));
     std::vector<uint8_t> out(n);
-    std::vector<int32_t> bbox(4);
+    std::vector<int32_t> bbox(film_bbox_size(W, H));
     if (launch_film_optics(gray.data(), W, H, bbox.data(), heff.data(), nullptr)) return 3;
     if (launch_film_composite(L, P, gray.data(), heff.data(), tiles.data(), dn.data(), tmp.data(), dh.data(),
                               part.data(), fs.data(), grain.data(), out.data(), nullptr))
