@@ -302,7 +302,13 @@ struct MinD { __device__ double operator()(double a, double b) const { return fm
 struct MaxD { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 struct SumU { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; } };
 
-// One level of the median-split build (bvh.rs:166-295), one block per segment.
+// Segments of at most kLaneSegment crystals are split by one lane (k_bvh_level_tiny),
+// of at most kWaveSegment by one wave (k_bvh_level_small), larger ones by a block.
+constexpr uint32_t kLaneSegment = 16u;
+constexpr uint32_t kWaveSegment = 64u;
+
+// One level of the median-split build (bvh.rs:166-295), one block per segment of more
+// than kWaveSegment crystals.
 __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
 {
     __shared__ float lf[4];
@@ -319,6 +325,7 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t si = blockIdx.x; si < nseg; si += gridDim.x) {
         const Seg sg = segs[si];
+        if (sg.count <= kWaveSegment) continue;   // k_bvh_level_small
         float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
         double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256) {
@@ -416,6 +423,167 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
             out[k + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
         }
         __syncthreads();
+    }
+}
+
+template <class V, class Op>
+__device__ __forceinline__ V wave_reduce(V v, Op op)
+{
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// The same level step for segments of at most kWaveSegment crystals, one wave per
+// segment (a lane per crystal): the box, centroid extent, split count and the stable
+// left/right partition are those of k_bvh_level, without block barriers.
+__global__ void __launch_bounds__(256) k_bvh_level_small(Bufs B, int level)
+{
+    const int in = level % 3, outc = (level + 1) % 3;
+    const Seg *segs = B.seg[level & 1];
+    Seg *out = B.seg[(level + 1) & 1];
+    const uint32_t *src = B.elem[level & 1];
+    uint32_t *dst = B.elem[(level + 1) & 1];
+    const uint32_t nseg = B.seg_count[in];
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t si = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); si < nseg; si += waves) {
+        const Seg sg = segs[si];
+        if (sg.count <= kLaneSegment || sg.count > kWaveSegment) continue;
+        const bool valid = lane < sg.count;
+        const uint32_t e = sg.start + lane;
+        const uint32_t s = valid ? src[e] : 0u;
+        float blo[3], bhi[3];
+        double cmn[3], cmx[3], cen[3];
+        for (int a = 0; a < 3; a++) {
+            cen[a] = valid ? B.cen[3 * s + a] : 0.0;
+            blo[a] = wave_reduce(valid ? B.elo[3 * s + a] : INFINITY, MinF());
+            bhi[a] = wave_reduce(valid ? B.ehi[3 * s + a] : -INFINITY, MaxF());
+            cmn[a] = wave_reduce(valid ? cen[a] : INFINITY, MinD());
+            cmx[a] = wave_reduce(valid ? cen[a] : -INFINITY, MaxD());
+        }
+        BNode &nd = B.bn[sg.node];
+        const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
+        const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+        const double ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
+        const bool leaf = sg.count <= 4u || ext <= 1e-7;
+        if (lane == 0) {
+            for (int a = 0; a < 3; a++) {
+                nd.lo[a] = blo[a];
+                nd.hi[a] = bhi[a];
+            }
+            nd.depth = sg.depth;
+            nd.start = sg.start;
+            nd.count = sg.count;
+        }
+        if (leaf) {
+            if (valid) B.final_elem[e] = s;
+            if (lane == 0) {
+                nd.left = nd.right = kNone;
+                if (sg.count > kMaxLeafCount) atomicOr(&B.err[1], 1u);
+            }
+            continue;
+        }
+        const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
+                                  (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
+        const double c = axis == 0 ? cen[0] : (axis == 1 ? cen[1] : cen[2]);
+        const bool left = valid && c < mid;
+        const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
+        uint32_t nl = (uint32_t)__popcll(ml);
+        if (nl == 0u || nl == sg.count) {
+            // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
+            if (valid) dst[e] = s;
+            nl = sg.count / 2;
+        } else if (valid) {
+            if (left) dst[sg.start + (uint32_t)__popcll(ml & below)] = s;
+            else dst[sg.start + nl + (uint32_t)__popcll(mr & below)] = s;
+        }
+        if (lane == 0) {
+            const TileDev &T = B.tiles[sg.tile];
+            const uint32_t id = atomicAdd(&B.node_count[sg.tile], 2u);
+            const uint32_t l = 2 * T.crystal_base + id, r = l + 1;
+            nd.left = l;
+            nd.right = r;
+            B.bn[l].parent = sg.node;
+            B.bn[r].parent = sg.node;
+            if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
+            const uint32_t k = atomicAdd(&B.seg_count[outc], 2u);
+            out[k] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
+            out[k + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
+        }
+    }
+}
+
+// The level step for segments of at most kLaneSegment crystals, one lane per segment:
+// sequential box and extent (min/max are exact in any order), then the same stable
+// left/right partition as the block and wave forms.
+__global__ void __launch_bounds__(256) k_bvh_level_tiny(Bufs B, int level)
+{
+    const int in = level % 3, outc = (level + 1) % 3;
+    const Seg *segs = B.seg[level & 1];
+    Seg *out = B.seg[(level + 1) & 1];
+    const uint32_t *src = B.elem[level & 1];
+    uint32_t *dst = B.elem[(level + 1) & 1];
+    const uint32_t nseg = B.seg_count[in];
+    for (uint32_t si = blockIdx.x * blockDim.x + threadIdx.x; si < nseg; si += gridDim.x * blockDim.x) {
+        const Seg sg = segs[si];
+        if (sg.count > kLaneSegment) continue;
+        float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t e = sg.start; e < sg.start + sg.count; e++) {
+            const uint32_t s = src[e];
+            for (int a = 0; a < 3; a++) {
+                blo[a] = fminf(blo[a], B.elo[3 * s + a]);
+                bhi[a] = fmaxf(bhi[a], B.ehi[3 * s + a]);
+                cmn[a] = fmin(cmn[a], B.cen[3 * s + a]);
+                cmx[a] = fmax(cmx[a], B.cen[3 * s + a]);
+            }
+        }
+        BNode &nd = B.bn[sg.node];
+        const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
+        const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+        const double ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
+        const bool leaf = sg.count <= 4u || ext <= 1e-7;
+        for (int a = 0; a < 3; a++) {
+            nd.lo[a] = blo[a];
+            nd.hi[a] = bhi[a];
+        }
+        nd.depth = sg.depth;
+        nd.start = sg.start;
+        nd.count = sg.count;
+        if (leaf) {
+            for (uint32_t e = sg.start; e < sg.start + sg.count; e++) B.final_elem[e] = src[e];
+            nd.left = nd.right = kNone;
+            if (sg.count > kMaxLeafCount) atomicOr(&B.err[1], 1u);
+            continue;
+        }
+        const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
+                                  (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
+        uint32_t nl = 0;
+        for (uint32_t e = sg.start; e < sg.start + sg.count; e++) nl += (B.cen[3 * src[e] + axis] < mid) ? 1u : 0u;
+        if (nl == 0u || nl == sg.count) {
+            // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
+            for (uint32_t e = sg.start; e < sg.start + sg.count; e++) dst[e] = src[e];
+            nl = sg.count / 2;
+        } else {
+            uint32_t il = sg.start, ir = sg.start + nl;
+            for (uint32_t e = sg.start; e < sg.start + sg.count; e++) {
+                const uint32_t s = src[e];
+                if (B.cen[3 * s + axis] < mid) dst[il++] = s;
+                else dst[ir++] = s;
+            }
+        }
+        const TileDev &T = B.tiles[sg.tile];
+        const uint32_t id = atomicAdd(&B.node_count[sg.tile], 2u);
+        const uint32_t l = 2 * T.crystal_base + id, r = l + 1;
+        nd.left = l;
+        nd.right = r;
+        B.bn[l].parent = sg.node;
+        B.bn[r].parent = sg.node;
+        if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
+        const uint32_t k = atomicAdd(&B.seg_count[outc], 2u);
+        out[k] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
+        out[k + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
     }
 }
 
@@ -661,8 +829,11 @@ static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
     hipLaunchKernelGGL(k_tile_scans, dim3(b->n_tiles), dim3(256), 0, s, B);
     hipLaunchKernelGGL(k_boost, dim3(nb), dim3(256), 0, s, B, n, in.develop);
     hipLaunchKernelGGL(k_root_segments, dim3((b->n_tiles + 255) / 256), dim3(256), 0, s, B, b->n_tiles);
-    for (int level = 0; level < kMaxLevels; level++)
+    for (int level = 0; level < kMaxLevels; level++) {
         hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(256), 0, s, B, level);
+        hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
+        hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
+    }
     hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B);
     hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, 2 * n);
     hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, 2 * n);
