@@ -397,9 +397,14 @@ enum : uint32_t { kNeed = 0, kStart = 1, kWalk = 2, kDone = 3 };
 // Hit leaves of one walk are queued, up to kLeafQueue, and processed together in the
 // order found.  A bounce's ray is fixed and the walk does not read the packet weight, so
 // deferring a leaf changes no result; lanes holding a leaf keep stepping instead of
-// idling until their neighbours find one.  Four measured best (Tri-X frame: transport
-// 1961 ms with one leaf, 1626 with two, 1514 with three, 1460 with four, 1491 with six).
-constexpr int kLeafQueue = 4;
+// idling until their neighbours find one.  The queue lives in LDS, one column per lane
+// (slot q of lane t at [q][t]: 64 consecutive words per wave, no bank conflicts), so
+// its depth costs no registers.  Measured on the Tri-X frame (transport ms): register
+// queue 1961 / 1626 / 1514 / 1460 with depth 1 / 2 / 3 / 4 (30 VGPRs spilled at
+// depth 4); LDS queue 1415 / 1378 / 1363 / 1366 / 1369 with depth 4 / 8 / 12 / 16 / 24
+// (T-Max frame: 4027 / 3848 / 3809 / 3842 / 3865).  Every launch is 256 lanes per block.
+constexpr int kLeafQueue = 12;
+constexpr uint32_t kQueueLanes = 256;
 
 // kMode 0: latent counts (the transport); 1: expected-dose replay; 2: gradient replay
 // (single tile only), with R the replay outputs and sums the lane's parameter sums.
@@ -431,7 +436,9 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f, w = 0.f;
     float step = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, h0 = 0.f, h1 = 0.f, h2 = 0.f;
-    uint32_t b = 0, i = 0, leaf = 0, leaf2 = 0, leaf3 = 0, leaf4 = 0;   // the leaf queue
+    uint32_t b = 0, i = 0, nq = 0;   // nq: leaves queued in this lane's column of lq
+    __shared__ uint32_t lq_all[kLeafQueue][kQueueLanes];
+    uint32_t *lq = &lq_all[0][threadIdx.x];
     // replay: exposure of the packet's pixel, relative weight tangents, HG score and the
     // packet's gradient sums (flushed when the lane takes its next packet)
     float E = 0.f, w_mu = 0.f, w_r = 0.f, score = 0.f, a_e = 0.f, a_mu = 0.f, a_g = 0.f, a_r = 0.f;
@@ -548,7 +555,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 s2 = fminf(oz, ez) - mg;
                 h2 = fmaxf(oz, ez) + mg;
                 i = 0;
-                leaf = 0;
+                nq = 0u;
                 phase = kWalk;
             }
         }
@@ -558,7 +565,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             const uint32_t walking = (uint32_t)__popcll(__ballot(phase == kWalk));
             const uint32_t stop = (walking * (100u - min(C.wait_pct, 100u))) / 100u;
             for (uint32_t it = 0;; it++) {
-                const bool stepping = phase == kWalk && leaf4 == 0u && i < T.n_nodes;
+                const bool stepping = phase == kWalk && nq < (uint32_t)kLeafQueue && i < T.n_nodes;
                 const uint32_t ns = (uint32_t)__popcll(__ballot(stepping));
                 if (ns == 0u || (it > 0u && ns <= stop)) break;
                 cnt.nodes += ns;
@@ -567,12 +574,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     const bool near = !prune || (nd.lo[0] <= h0 && nd.hi[0] >= s0 && nd.lo[1] <= h1 &&
                                                  nd.hi[1] >= s1 && nd.lo[2] <= h2 && nd.hi[2] >= s2);
                     const bool hit = near && box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
-                    if (hit && nd.prim != kInterior) {
-                        if (leaf == 0u) leaf = nd.prim;
-                        else if (leaf2 == 0u) leaf2 = nd.prim;
-                        else if (leaf3 == 0u) leaf3 = nd.prim;
-                        else leaf4 = nd.prim;
-                    }
+                    if (hit && nd.prim != kInterior) lq[kQueueLanes * nq++] = nd.prim;
                     i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
                 }
             }
@@ -581,8 +583,8 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
         // before the step end
 #pragma unroll 1
         for (int qd = 0; qd < kLeafQueue; qd++) {
-            const uint32_t lf = qd == 0 ? leaf : (qd == 1 ? leaf2 : (qd == 2 ? leaf3 : leaf4));
-            if (!(phase == kWalk && lf != 0u)) continue;
+            if (!(phase == kWalk && (uint32_t)qd < nq)) continue;
+            const uint32_t lf = lq[kQueueLanes * qd];
             const uint32_t first = lf >> 4, nprim = lf & 15u;
             for (uint32_t k = first; k < first + nprim; k++) {
                 cnt.prims++;
@@ -625,9 +627,9 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 }
             }
         }
-        leaf = leaf2 = leaf3 = leaf4 = 0u;
+        nq = 0u;
         // ---- end of the bounce: move, boundaries, halation or scattering, roulette
-        if (phase == kWalk && leaf == 0u && i >= T.n_nodes) {
+        if (phase == kWalk && nq == 0u && i >= T.n_nodes) {
             const float qx = ox + dx * step, qy = oy + dy * step, qz = oz + dz * step;
             if (qx < 0.0f || qx >= T.W || qy < 0.0f || qy >= T.H || qz <= 0.0f) {
                 phase = kNeed;
