@@ -71,7 +71,8 @@ class ScanConfigC(C.Structure):
 
 class TuningC(C.Structure):
     _fields_ = [("transport_kernel", C.c_uint32), ("transport_chunk", C.c_uint32),
-                ("transport_wait_pct", C.c_uint32), ("scan_rays_per_thread", C.c_uint32)]
+                ("transport_wait_pct", C.c_uint32), ("scan_rays_per_thread", C.c_uint32),
+                ("transport_waves", C.c_uint32)]
 
 
 class TileParamsC(C.Structure):

@@ -293,9 +293,10 @@ class GpuEngine:
         return L.lib().fg_engine_adapter_name(self._h).decode()
 
     def set_tuning(self, transport_kernel=0, transport_chunk=64, transport_wait_pct=80,
-                   scan_rays_per_thread=0):
+                   scan_rays_per_thread=0, transport_waves=6):
         """Launch parameters (speed only; results do not depend on them)."""
-        t = L.TuningC(transport_kernel, transport_chunk, transport_wait_pct, scan_rays_per_thread)
+        t = L.TuningC(transport_kernel, transport_chunk, transport_wait_pct, scan_rays_per_thread,
+                      transport_waves)
         check(L.lib().fg_engine_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

@@ -4,7 +4,8 @@ Knobs (include/fgrain_hip.h, fg_tuning) that the batched frame path uses (it alw
 the persistent lane-regenerating transport; the kernel choice applies to per-tile calls):
 * photon-packet fetch size: packets a wave takes from the work counter at a time (chunk);
 * leaf-batch threshold: BVH node steps continue until this percentage of the walking
-  lanes hold a pending leaf (wait_pct);
+  lanes hold a full leaf queue (wait_pct);
+* transport occupancy: 6 or 7 waves per SIMD (register budget against latency hiding);
 * scan tile shape: rays of a pixel per lane (the pixel's rays / that many adjacent lanes).
 The BVH walk is stackless (threaded skip links), so there is no traversal-stack depth to
 tune.  Transport and scan are separate kernels, so their knobs are searched separately:
@@ -31,7 +32,8 @@ TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs",
 
 TRANSPORT_GRID = {
     "transport_chunk": [64, 128, 256, 512],
-    "transport_wait_pct": [50, 65, 80, 90, 100],
+    "transport_wait_pct": [65, 80, 90, 95, 100],
+    "transport_waves": [6, 7],
 }
 SCAN_RAYS_PER_LANE = [1, 2, 4, 8, 16]
 
@@ -64,20 +66,23 @@ def tune(engine, workload: str, tiles=(4, 3), reps=3, log=print) -> dict:
     exposure = np.ascontiguousarray(W.workload_exposure(cfg)[:h, :w])
     frame = W.Frame(engine, cfg, exposure, frame=(w, h))
     log(f"autotune {workload}: crop {w}x{h} px, {len(frame.tile_ranges())} tiles, {frame.crystals} crystals")
-    default = {"transport_kernel": 0, "transport_chunk": 64, "transport_wait_pct": 80, "scan_rays_per_thread": 0}
+    default = {"transport_kernel": 0, "transport_chunk": 64, "transport_wait_pct": 80, "scan_rays_per_thread": 0,
+               "transport_waves": 6}
     engine.set_tuning(**default)
     frame.render(W.Frame.BATCHED)
     ref_counts = frame.counts().copy()
     results = []
     best_t, best_ms = None, float("inf")
-    for chunk, wait in itertools.product(TRANSPORT_GRID["transport_chunk"], TRANSPORT_GRID["transport_wait_pct"]):
-        cand = dict(default, transport_chunk=chunk, transport_wait_pct=wait)
+    keys = list(TRANSPORT_GRID)
+    for values in itertools.product(*(TRANSPORT_GRID[k] for k in keys)):
+        cand = dict(default, **dict(zip(keys, values)))
         engine.set_tuning(**cand)
         ms = _median_ms(frame, "ms_kernel_transport", reps)
         if not np.array_equal(frame.counts(), ref_counts):
             raise RuntimeError(f"tuning {cand} changed the latent counts")
         results.append({"knobs": cand, "ms_kernel_transport": ms})
-        log(f"  transport chunk {chunk} wait {wait}: {ms:.2f} ms")
+        log("  transport " + " ".join(f"{k[len('transport_'):]} {v}" for k, v in zip(keys, values))
+            + f": {ms:.2f} ms")
         if ms < best_ms:
             best_t, best_ms = cand, ms
     best_s, best_sms = None, float("inf")

@@ -37,6 +37,7 @@ struct BatchedInput {
     std::vector<double> pos64, dims64;
     std::vector<uint32_t> orig;
     TransportArgs transport;        // optics and shared fields; per-tile fields overwritten
+    uint32_t transport_waves = 6;   // occupancy of the transport kernel: 6 or 7 waves per SIMD
     DevelopParams develop;
     ScanCommon scan;
 };

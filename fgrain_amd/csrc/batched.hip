@@ -136,15 +136,32 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
     }
 }
 
-// persistent transport over every tile's packets
+// persistent transport over every tile's packets, built for 6 waves per SIMD (80 VGPRs)
+// and for 7 (72 VGPRs, more spills): which is faster depends on the frame (Tri-X frame
+// 1363 / 1285 ms, T-Max frame 3806 / 3877 ms), so the tuning chooses
+__device__ __forceinline__ void transport_frame(const TransportArgs &C, const Bufs &B, const Node *nodes,
+                                                const float4 *recs, uint32_t *counts, const float *emap,
+                                                uint32_t n_tiles, uint64_t total_photons)
+{
+    LaneCounters cnt;
+    transport_persistent<true>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work, total_photons, cnt);
+    flush_lane_counters(cnt, C.stats);
+}
+
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) k_transport(TransportArgs C, Bufs B, const Node *nodes,
                                                    const float4 *recs, uint32_t *counts,
                                                    const float *emap, uint32_t n_tiles,
                                                    uint64_t total_photons)
 {
-    LaneCounters cnt;
-    transport_persistent<true>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work, total_photons, cnt);
-    flush_lane_counters(cnt, C.stats);
+    transport_frame(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))) k_transport_w7(TransportArgs C, Bufs B, const Node *nodes,
+                                                   const float4 *recs, uint32_t *counts,
+                                                   const float *emap, uint32_t n_tiles,
+                                                   uint64_t total_photons)
+{
+    transport_frame(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
 }
 
 __global__ void __launch_bounds__(256) k_threshold(Bufs B, const uint32_t *counts, uint64_t n,
@@ -796,7 +813,10 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     int dev = 0, cus = 0, per_cu = 0;
     TRY(hipGetDevice(&dev));
     TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_transport), 256, 0));
+    if (in.transport_waves != 6u && in.transport_waves != 7u) return hipErrorInvalidValue;
+    const void *tk = in.transport_waves == 7u ? reinterpret_cast<const void *>(k_transport_w7)
+                                              : reinterpret_cast<const void *>(k_transport);
+    TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tk, 256, 0));
     b->transport_grid = std::max(1, cus * std::max(per_cu, 1));
     for (auto &e : b->ev) TRY(hipEventCreate(&e));
     return hipSuccess;
@@ -819,9 +839,9 @@ static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
     C.stats = in.stats;
     C.estride = in.frame_w;
     TRY(mark(b, 0, s));
-    hipLaunchKernelGGL(k_transport, dim3(b->transport_grid), dim3(256), 0, s, C, B, in.nodes,
-                       reinterpret_cast<const float4 *>(in.recs), in.counts, in.emap, b->n_tiles,
-                       b->total_photons);
+    hipLaunchKernelGGL(in.transport_waves == 7u ? k_transport_w7 : k_transport, dim3(b->transport_grid),
+                       dim3(256), 0, s, C, B, in.nodes, reinterpret_cast<const float4 *>(in.recs),
+                       in.counts, in.emap, b->n_tiles, b->total_photons);
     TRY(mark(b, 1, s));
     hipLaunchKernelGGL(k_threshold, dim3(nb), dim3(256), 0, s, B, in.counts, n, in.develop);
     hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, n, in.develop);

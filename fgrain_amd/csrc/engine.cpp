@@ -87,14 +87,17 @@ extern "C" const char *fg_engine_adapter_name(const fg_engine *E) { return E ? E
 extern "C" fg_status fg_engine_set_tuning(fg_engine *E, const fg_tuning *t)
 {
     if (!E || !t) return fail(FG_ERR_INVALID_ARG, "fg_engine_set_tuning: null argument");
-    if (t->transport_kernel > 1 || t->transport_wait_pct > 100 || t->scan_rays_per_thread > 64)
+    if (t->transport_kernel > 1 || t->transport_wait_pct > 100 || t->scan_rays_per_thread > 64 ||
+        (t->transport_waves != 0 && t->transport_waves != 6 && t->transport_waves != 7))
         return fail(FG_ERR_INVALID_ARG,
-                    "fg_engine_set_tuning: transport_kernel must be 0 or 1, wait_pct <= 100, scan_rays_per_thread <= 64");
+                    "fg_engine_set_tuning: transport_kernel must be 0 or 1, wait_pct <= 100, "
+                    "scan_rays_per_thread <= 64, transport_waves 0, 6 or 7");
     std::lock_guard<std::mutex> lk(E->mu);
     E->tuning.transport_kernel = t->transport_kernel;
     E->tuning.transport_chunk = t->transport_chunk ? std::max(t->transport_chunk, 64u) : 64u;
     E->tuning.transport_wait_pct = t->transport_wait_pct ? t->transport_wait_pct : 80u;
     E->tuning.scan_rays_per_thread = t->scan_rays_per_thread;
+    E->tuning.transport_waves = t->transport_waves ? t->transport_waves : 6u;
     return FG_OK;
 }
 
@@ -1184,6 +1187,7 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
         in.scan.ss = std::max(D.scan.supersample, 1u);
         in.scan.mode = D.scan.mode;
         in.scan.rays_per_lane = E->tuning.scan_rays_per_thread;
+        in.transport_waves = E->tuning.transport_waves;
         in.scan.z0 = (float)(D.emulsion.depth + 0.1);
         in.scan.sigma = (float)D.scan.silver_opacity;
         if ((uint64_t)F->n_crystals * 2 >= 0xFFFFFFFFull)

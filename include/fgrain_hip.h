@@ -177,6 +177,8 @@ typedef struct fg_tuning {
     uint32_t transport_wait_pct; /* BVH steps continue until this % of lanes hold a leaf (80) */
     uint32_t scan_rays_per_thread; /* scan rays per lane: 0 or 1 one lane per ray (default);
                                       r > 1 gives a pixel rays / r adjacent lanes (<= 64) */
+    uint32_t transport_waves;    /* waves per SIMD of the batched frame transport: 6 (80
+                                    VGPRs) or 7 (72 VGPRs, more spills); 0 means 6 */
 } fg_tuning;
 fg_status fg_engine_set_tuning(fg_engine *engine, const fg_tuning *tuning);
 fg_status fg_engine_get_tuning(const fg_engine *engine, fg_tuning *tuning);

@@ -9,6 +9,7 @@ import pytest
 
 import fgrain_amd as F
 from fgrain_amd import workloads as W
+from fgrain_amd._lib import GpuError
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -82,6 +83,27 @@ def test_batched_graph_replay_is_deterministic(engine, frame):
     for D, c in outs[1:]:
         assert (D.view(np.uint32) == outs[0][0].view(np.uint32)).all()
         assert np.array_equal(c, outs[0][1])
+
+
+def test_batched_transport_occupancy_gives_identical_frames(engine, frame):
+    """The 7-waves-per-SIMD build of the frame transport (fewer VGPRs, more spills) and
+    the 6-wave build deposit the same photons and give the same frame bit for bit."""
+    cfg, fr = frame
+    old = engine.tuning()
+    outs = []
+    try:
+        for waves in (6, 7):
+            engine.set_tuning(**dict(old, transport_waves=waves, transport_wait_pct=90))
+            assert engine.tuning()["transport_waves"] == waves
+            st = fr.render(W.Frame.BATCHED)
+            outs.append((fr.read(), fr.counts(), st["nodes_visited"], st["prims_tested"]))
+        with pytest.raises(GpuError, match="transport_waves"):
+            engine.set_tuning(**dict(old, transport_waves=5))
+    finally:
+        engine.set_tuning(**old)
+    (D6, c6, n6, p6), (D7, c7, n7, p7) = outs
+    assert np.array_equal(c6, c7) and n6 == n7 and p6 == p7
+    assert (D6.view(np.uint32) == D7.view(np.uint32)).all()
 
 
 def test_batched_development_matches_oracle(engine):
