@@ -220,7 +220,16 @@ __global__ void __launch_bounds__(256) k_roots(Bufs B, uint64_t n)
     if (s >= n || !B.devflag[s]) return;
     const uint32_t r = uf_find(B.parent, (uint32_t)s);
     B.root[s] = r;
-    atomicAdd(&B.csize[r], 1u);
+    // one atomic per distinct root among the wave's lanes (neighbouring slots share clumps)
+    for (;;) {
+        const uint32_t rl = __builtin_amdgcn_readfirstlane(r);
+        const unsigned long long m = __ballot(r == rl);
+        if (r == rl) {
+            if ((threadIdx.x & 63u) == (uint32_t)__ffsll((long long)m) - 1u)
+                atomicAdd(&B.csize[rl], (uint32_t)__popcll(m));
+            break;
+        }
+    }
     if (r == s) B.root_flag[B.tiles[B.tile_of[s]].crystal_base + B.orig[s]] = 1u;
 }
 
@@ -266,25 +275,41 @@ __global__ void __launch_bounds__(256) k_tile_scans(Bufs B)
 __global__ void __launch_bounds__(256) k_boost(Bufs B, uint64_t n, DevelopParams P)
 {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n || !B.devflag[s]) return;
-    const uint32_t t = B.tile_of[s];
-    TileDev &T = B.tiles[t];
-    const uint32_t r = B.root[s];
-    const uint32_t size = B.csize[r];
-    double fe = P.fe;
-    if (size > 1u) fe *= 1.15 + 0.18 * (double)(min(size, 6u) - 1u);
-    B.fe[s] = fe;
-    B.clump[s] = (int32_t)B.prefix[T.crystal_base + B.orig[r]];
-    if (r == s) atomicMax(&T.max_clump, size);
-    // developed-crystal box (crystal.rs:106-133 with the boosted expansion) and centroid
-    const uint32_t e = T.crystal_base + B.devpos[s];
-    B.elem[0][e] = (uint32_t)s;
-    const double m = B.mdim[s] * fe;
-    for (int a = 0; a < 3; a++) {
-        const double lo = B.pos64[3 * s + a] - m, hi = B.pos64[3 * s + a] + m;
-        B.elo[3 * s + a] = __double2float_rd(lo);
-        B.ehi[3 * s + a] = __double2float_ru(hi);
-        B.cen[3 * s + a] = 0.5 * (lo + hi);
+    const bool live = s < n && B.devflag[s];
+    uint32_t key = kNone, size = 0u;   // key: the tile of a clump root
+    if (live) {
+        const uint32_t t = B.tile_of[s];
+        const uint32_t crystal_base = B.tiles[t].crystal_base;
+        const uint32_t r = B.root[s];
+        size = B.csize[r];
+        double fe = P.fe;
+        if (size > 1u) fe *= 1.15 + 0.18 * (double)(min(size, 6u) - 1u);
+        B.fe[s] = fe;
+        B.clump[s] = (int32_t)B.prefix[crystal_base + B.orig[r]];
+        if (r == s) key = t;
+        // developed-crystal box (crystal.rs:106-133 with the boosted expansion) and centroid
+        const uint32_t e = crystal_base + B.devpos[s];
+        B.elem[0][e] = (uint32_t)s;
+        const double m = B.mdim[s] * fe;
+        for (int a = 0; a < 3; a++) {
+            const double lo = B.pos64[3 * s + a] - m, hi = B.pos64[3 * s + a] + m;
+            B.elo[3 * s + a] = __double2float_rd(lo);
+            B.ehi[3 * s + a] = __double2float_ru(hi);
+            B.cen[3 * s + a] = 0.5 * (lo + hi);
+        }
+    }
+    // largest clump per tile: every lane of the wave takes part, one atomic per tile
+    // among the wave's roots
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        const unsigned long long pend = __ballot(key != kNone);
+        if (pend == 0ull) break;
+        const int leader = __ffsll((long long)pend) - 1;
+        const uint32_t tl = (uint32_t)__shfl((int)key, leader, 64);
+        uint32_t v = key == tl ? size : 0u;
+        for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+        if ((int)lane == leader) atomicMax(&B.tiles[tl].max_clump, v);
+        if (key == tl) key = kNone;
     }
 }
 
@@ -616,16 +641,25 @@ __device__ __forceinline__ bool node_valid(const Bufs &B, uint64_t g, uint32_t &
     return g - 2ull * T.crystal_base < B.node_count[t];
 }
 
+// Leaves under every node, bottom-up from the leaves: each child adds its count plus
+// kArrive to its parent's (zeroed) counter; the first child to arrive stops, the second
+// reads the sibling's count from the atomic's return value and carries the sum upward.
+// Both additions of kArrive wrap to zero, so every counter ends at its leaf count.  One
+// uncontended atomic per child, instead of one per leaf on every ancestor.
+constexpr uint32_t kArrive = 0x80000000u;
+
 __global__ void __launch_bounds__(256) k_leafcount(Bufs B, uint64_t cap)
 {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t t;
     if (g >= cap || !node_valid(B, g, t)) return;
     if (B.bn[g].left != kNone) return;
-    uint32_t x = (uint32_t)g;
-    while (x != kNone) {
-        atomicAdd(&B.leafcnt[x], 1u);
-        x = B.bn[x].parent;
+    B.leafcnt[g] = 1u;
+    uint32_t c = 1u;
+    for (uint32_t x = B.bn[g].parent; x != kNone; x = B.bn[x].parent) {
+        const uint32_t old = atomicAdd(&B.leafcnt[x], c + kArrive);
+        if (!(old & kArrive)) break;
+        c += old - kArrive;
     }
 }
 
