@@ -33,6 +33,18 @@ struct Seg {
     uint32_t tile, start, count, node, depth;
 };
 
+// Segments of at most kLaneSegment crystals are split by one lane (k_bvh_level_tiny),
+// of at most kWaveSegment by one wave (k_bvh_level_small), larger ones by a block
+// (k_bvh_level).  Each size class has its own segment list per level, so each kernel
+// reads only the segments it splits.
+constexpr uint32_t kLaneSegment = 16u;
+constexpr uint32_t kWaveSegment = 64u;
+enum : uint32_t { kSegBlock = 0, kSegWave = 1, kSegLane = 2, kSegClasses = 3 };
+__host__ __device__ __forceinline__ uint32_t seg_class(uint32_t count)
+{
+    return count > kWaveSegment ? kSegBlock : (count > kLaneSegment ? kSegWave : kSegLane);
+}
+
 struct BNode {
     float lo[3], hi[3];
     uint32_t left, right, parent;
@@ -51,8 +63,8 @@ struct Bufs {
     uint32_t *leafcnt;
     Node *dnodes;
     Crystal *drecs;
-    Seg *seg[2];
-    uint32_t *seg_count;   // 3 rotating counters
+    Seg *seg[2][kSegClasses];   // [level parity][size class]
+    uint32_t *seg_count;        // [3 rotating levels][size class]
     uint32_t *node_count;  // per tile
     unsigned long long *work;  // transport work counter
     uint32_t *err;         // [0] depth, [1] leaf size, [2] unfinished build
@@ -115,6 +127,11 @@ __device__ __forceinline__ uint32_t tile_of_block(const TileDev *tiles, uint32_t
 __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigned long long *stats,
                                                uint64_t n, uint32_t n_tiles)
 {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 3 * kSegClasses) B.seg_count[threadIdx.x] = 0u;
+        if (threadIdx.x < 4) B.err[threadIdx.x] = 0u;
+        if (threadIdx.x == 0) *B.work = 0ull;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += stride) {
         B.leafcnt[i] = 0u;
@@ -128,11 +145,6 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
             B.tiles[i].n_dev = B.tiles[i].clumps = B.tiles[i].max_clump = B.tiles[i].dev_nodes = 0u;
         }
         if (i < 8) stats[i] = 0ull;
-        if (i < 4) {
-            B.err[i] = 0u;
-            if (i < 3) B.seg_count[i] = 0u;
-            if (i == 3) *B.work = 0ull;
-        }
     }
 }
 
@@ -319,8 +331,9 @@ __global__ void k_root_segments(Bufs B, uint32_t n_tiles)
     if (t >= n_tiles) return;
     const TileDev T = B.tiles[t];
     if (T.n_dev == 0) return;
-    const uint32_t k = atomicAdd(&B.seg_count[0], 1u);
-    B.seg[0][k] = Seg{t, T.crystal_base, T.n_dev, 2 * T.crystal_base, 0u};
+    const uint32_t c = seg_class(T.n_dev);
+    const uint32_t k = atomicAdd(&B.seg_count[c], 1u);
+    B.seg[0][c][k] = Seg{t, T.crystal_base, T.n_dev, 2 * T.crystal_base, 0u};
     B.node_count[t] = 1u;
     B.bn[2 * T.crystal_base].parent = kNone;
 }
@@ -344,10 +357,32 @@ struct MinD { __device__ double operator()(double a, double b) const { return fm
 struct MaxD { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 struct SumU { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; } };
 
-// Segments of at most kLaneSegment crystals are split by one lane (k_bvh_level_tiny),
-// of at most kWaveSegment by one wave (k_bvh_level_small), larger ones by a block.
-constexpr uint32_t kLaneSegment = 16u;
-constexpr uint32_t kWaveSegment = 64u;
+// Node ids of a split and its two child segments.  One lane does this per split (a block's
+// thread 0, a wave's lane 0); k_bvh_level_tiny aggregates the counters over its wave.
+__device__ __forceinline__ void emit_split(Bufs &B, int level, const Seg &sg, uint32_t nl, BNode &nd)
+{
+    const int outc = (level + 1) % 3;
+    const TileDev &T = B.tiles[sg.tile];
+    const uint32_t id = atomicAdd(&B.node_count[sg.tile], 2u);
+    const uint32_t l = 2 * T.crystal_base + id, r = l + 1;
+    nd.left = l;
+    nd.right = r;
+    B.bn[l].parent = sg.node;
+    B.bn[r].parent = sg.node;
+    if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
+    const Seg kids[2] = {Seg{sg.tile, sg.start, nl, l, sg.depth + 1},
+                         Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1}};
+    const uint32_t c0 = seg_class(kids[0].count), c1 = seg_class(kids[1].count);
+    Seg *const *out = B.seg[(level + 1) & 1];
+    if (c0 == c1) {
+        const uint32_t k = atomicAdd(&B.seg_count[3 * outc + c0], 2u);
+        out[c0][k] = kids[0];
+        out[c0][k + 1] = kids[1];
+    } else {
+        out[c0][atomicAdd(&B.seg_count[3 * outc + c0], 1u)] = kids[0];
+        out[c1][atomicAdd(&B.seg_count[3 * outc + c1], 1u)] = kids[1];
+    }
+}
 
 // One level of the median-split build (bvh.rs:166-295), one block per segment of more
 // than kWaveSegment crystals.
@@ -357,17 +392,15 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
     __shared__ double ld[4];
     __shared__ uint32_t lu[4];
     __shared__ uint32_t wl[4], wr[4];
-    const int in = level % 3, outc = (level + 1) % 3, zc = (level + 2) % 3;
-    const Seg *segs = B.seg[level & 1];
-    Seg *out = B.seg[(level + 1) & 1];
+    const int in = level % 3, zc = (level + 2) % 3;
+    const Seg *segs = B.seg[level & 1][kSegBlock];
     const uint32_t *src = B.elem[level & 1];
     uint32_t *dst = B.elem[(level + 1) & 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) B.seg_count[zc] = 0u;
-    const uint32_t nseg = B.seg_count[in];
+    if (blockIdx.x == 0 && threadIdx.x < kSegClasses) B.seg_count[3 * zc + threadIdx.x] = 0u;
+    const uint32_t nseg = B.seg_count[3 * in + kSegBlock];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t si = blockIdx.x; si < nseg; si += gridDim.x) {
         const Seg sg = segs[si];
-        if (sg.count <= kWaveSegment) continue;   // k_bvh_level_small
         float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
         double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256) {
@@ -451,19 +484,7 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
                 __syncthreads();
             }
         }
-        if (threadIdx.x == 0) {
-            const TileDev &T = B.tiles[sg.tile];
-            const uint32_t id = atomicAdd(&B.node_count[sg.tile], 2u);
-            const uint32_t l = 2 * T.crystal_base + id, r = l + 1;
-            nd.left = l;
-            nd.right = r;
-            B.bn[l].parent = sg.node;
-            B.bn[r].parent = sg.node;
-            if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
-            const uint32_t k = atomicAdd(&B.seg_count[outc], 2u);
-            out[k] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
-            out[k + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
-        }
+        if (threadIdx.x == 0) emit_split(B, level, sg, nl, nd);
         __syncthreads();
     }
 }
@@ -475,41 +496,175 @@ __device__ __forceinline__ V wave_reduce(V v, Op op)
     return v;
 }
 
-// The same level step for segments of at most kWaveSegment crystals, one wave per
-// segment (a lane per crystal): the box, centroid extent, split count and the stable
-// left/right partition are those of k_bvh_level, without block barriers.
+// The same level step for segments of kLaneSegment + 1 .. kWaveSegment crystals, one
+// wave per segment (a lane per crystal): the box, centroid extent, split count and the
+// stable left/right partition are those of k_bvh_level.  A block takes four segments per
+// step (block-uniform trip count) and reserves their node ids and child segments with one
+// atomic per tile and per size class.
 __global__ void __launch_bounds__(256) k_bvh_level_small(Bufs B, int level)
 {
+    __shared__ uint32_t w_tile[4], w_nl[4], w_id[4], w_k[4][2];
+    __shared__ Seg w_seg[4];
     const int in = level % 3, outc = (level + 1) % 3;
-    const Seg *segs = B.seg[level & 1];
-    Seg *out = B.seg[(level + 1) & 1];
+    const Seg *segs = B.seg[level & 1][kSegWave];
+    Seg *const *out = B.seg[(level + 1) & 1];
     const uint32_t *src = B.elem[level & 1];
     uint32_t *dst = B.elem[(level + 1) & 1];
-    const uint32_t nseg = B.seg_count[in];
+    const uint32_t nseg = B.seg_count[3 * in + kSegWave];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t base = blockIdx.x * 4u; base < nseg; base += gridDim.x * 4u) {
+        const uint32_t si = base + wave;
+        bool split = false;
+        if (si < nseg) {
+            const Seg sg = segs[si];
+            const bool valid = lane < sg.count;
+            const uint32_t e = sg.start + lane;
+            const uint32_t s = valid ? src[e] : 0u;
+            float blo[3], bhi[3];
+            double cmn[3], cmx[3], cen[3];
+            for (int a = 0; a < 3; a++) {
+                cen[a] = valid ? B.cen[3 * s + a] : 0.0;
+                blo[a] = wave_reduce(valid ? B.elo[3 * s + a] : INFINITY, MinF());
+                bhi[a] = wave_reduce(valid ? B.ehi[3 * s + a] : -INFINITY, MaxF());
+                cmn[a] = wave_reduce(valid ? cen[a] : INFINITY, MinD());
+                cmx[a] = wave_reduce(valid ? cen[a] : -INFINITY, MaxD());
+            }
+            BNode &nd = B.bn[sg.node];
+            const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
+            const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+            const double ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
+            const bool leaf = sg.count <= 4u || ext <= 1e-7;
+            if (lane == 0) {
+                for (int a = 0; a < 3; a++) {
+                    nd.lo[a] = blo[a];
+                    nd.hi[a] = bhi[a];
+                }
+                nd.depth = sg.depth;
+                nd.start = sg.start;
+                nd.count = sg.count;
+            }
+            if (leaf) {
+                if (valid) B.final_elem[e] = s;
+                if (lane == 0) {
+                    nd.left = nd.right = kNone;
+                    if (sg.count > kMaxLeafCount) atomicOr(&B.err[1], 1u);
+                }
+            } else {
+                split = true;
+                const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
+                                          (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
+                const double c = axis == 0 ? cen[0] : (axis == 1 ? cen[1] : cen[2]);
+                const bool left = valid && c < mid;
+                const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
+                uint32_t nl = (uint32_t)__popcll(ml);
+                if (nl == 0u || nl == sg.count) {
+                    // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
+                    if (valid) dst[e] = s;
+                    nl = sg.count / 2;
+                } else if (valid) {
+                    if (left) dst[sg.start + (uint32_t)__popcll(ml & below)] = s;
+                    else dst[sg.start + nl + (uint32_t)__popcll(mr & below)] = s;
+                }
+                if (lane == 0) {
+                    w_seg[wave] = sg;
+                    w_nl[wave] = nl;
+                    if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
+                }
+            }
+        }
+        if (lane == 0) w_tile[wave] = split ? w_seg[wave].tile : kNone;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // node ids per tile, child segments per class, in wave order
+            uint32_t need[kSegClasses] = {0u, 0u, 0u};
+            uint32_t cls[4][2];
+            for (int w = 0; w < 4; w++) {
+                if (w_tile[w] == kNone) continue;
+                cls[w][0] = seg_class(w_nl[w]);
+                cls[w][1] = seg_class(w_seg[w].count - w_nl[w]);
+                need[cls[w][0]]++;
+                need[cls[w][1]]++;
+            }
+            uint32_t kb[kSegClasses];
+            for (uint32_t c = 0; c < kSegClasses; c++)
+                kb[c] = need[c] ? atomicAdd(&B.seg_count[3 * outc + c], need[c]) : 0u;
+            bool done[4] = {false, false, false, false};
+            for (int w = 0; w < 4; w++) {
+                if (w_tile[w] == kNone) continue;
+                w_k[w][0] = kb[cls[w][0]]++;
+                w_k[w][1] = kb[cls[w][1]]++;
+                if (done[w]) continue;
+                uint32_t cnt = 0;
+                for (int v = w; v < 4; v++)
+                    if (w_tile[v] == w_tile[w]) cnt++;
+                uint32_t id = atomicAdd(&B.node_count[w_tile[w]], 2u * cnt);
+                for (int v = w; v < 4; v++)
+                    if (w_tile[v] == w_tile[w]) {
+                        w_id[v] = id;
+                        id += 2u;
+                        done[v] = true;
+                    }
+            }
+        }
+        __syncthreads();
+        if (split && lane == 0) {
+            const Seg sg = w_seg[wave];
+            const uint32_t nl = w_nl[wave];
+            const uint32_t l = 2 * B.tiles[sg.tile].crystal_base + w_id[wave], r = l + 1;
+            BNode &nd = B.bn[sg.node];
+            nd.left = l;
+            nd.right = r;
+            B.bn[l].parent = sg.node;
+            B.bn[r].parent = sg.node;
+            out[seg_class(nl)][w_k[wave][0]] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
+            out[seg_class(sg.count - nl)][w_k[wave][1]] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
+        }
+        __syncthreads();
+    }
+}
+
+// The level step for segments of at most kLaneSegment crystals, one lane per segment:
+// sequential box and extent (min/max are exact in any order), then the same stable
+// left/right partition as the block and wave forms.  The children of such a segment are
+// again at most kLaneSegment long; a wave reserves its splits' node ids (per tile) and
+// child segments with one atomic each.
+__global__ void __launch_bounds__(256) k_bvh_level_tiny(Bufs B, int level)
+{
+    const int in = level % 3, outc = (level + 1) % 3;
+    const Seg *segs = B.seg[level & 1][kSegLane];
+    Seg *out = B.seg[(level + 1) & 1][kSegLane];
+    const uint32_t *src = B.elem[level & 1];
+    uint32_t *dst = B.elem[(level + 1) & 1];
+    const uint32_t nseg = B.seg_count[3 * in + kSegLane];
     const uint32_t lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t si = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); si < nseg; si += waves) {
-        const Seg sg = segs[si];
-        if (sg.count <= kLaneSegment || sg.count > kWaveSegment) continue;
-        const bool valid = lane < sg.count;
-        const uint32_t e = sg.start + lane;
-        const uint32_t s = valid ? src[e] : 0u;
-        float blo[3], bhi[3];
-        double cmn[3], cmx[3], cen[3];
-        for (int a = 0; a < 3; a++) {
-            cen[a] = valid ? B.cen[3 * s + a] : 0.0;
-            blo[a] = wave_reduce(valid ? B.elo[3 * s + a] : INFINITY, MinF());
-            bhi[a] = wave_reduce(valid ? B.ehi[3 * s + a] : -INFINITY, MaxF());
-            cmn[a] = wave_reduce(valid ? cen[a] : INFINITY, MinD());
-            cmx[a] = wave_reduce(valid ? cen[a] : -INFINITY, MaxD());
-        }
-        BNode &nd = B.bn[sg.node];
-        const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
-        const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
-        const double ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
-        const bool leaf = sg.count <= 4u || ext <= 1e-7;
-        if (lane == 0) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // wave-uniform trip count, so that the ballots below see the whole wave
+    const uint32_t first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t sw = first; sw < nseg; sw += stride) {
+        const uint32_t si = sw + lane;
+        bool split = false;
+        Seg sg{};
+        uint32_t nl = 0;
+        if (si < nseg) {
+            sg = segs[si];
+            float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t e = sg.start; e < sg.start + sg.count; e++) {
+                const uint32_t s = src[e];
+                for (int a = 0; a < 3; a++) {
+                    blo[a] = fminf(blo[a], B.elo[3 * s + a]);
+                    bhi[a] = fmaxf(bhi[a], B.ehi[3 * s + a]);
+                    cmn[a] = fmin(cmn[a], B.cen[3 * s + a]);
+                    cmx[a] = fmax(cmx[a], B.cen[3 * s + a]);
+                }
+            }
+            BNode &nd = B.bn[sg.node];
+            const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
+            const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+            const double ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
+            const bool leaf = sg.count <= 4u || ext <= 1e-7;
             for (int a = 0; a < 3; a++) {
                 nd.lo[a] = blo[a];
                 nd.hi[a] = bhi[a];
@@ -517,121 +672,71 @@ __global__ void __launch_bounds__(256) k_bvh_level_small(Bufs B, int level)
             nd.depth = sg.depth;
             nd.start = sg.start;
             nd.count = sg.count;
-        }
-        if (leaf) {
-            if (valid) B.final_elem[e] = s;
-            if (lane == 0) {
+            if (leaf) {
+                for (uint32_t e = sg.start; e < sg.start + sg.count; e++) B.final_elem[e] = src[e];
                 nd.left = nd.right = kNone;
                 if (sg.count > kMaxLeafCount) atomicOr(&B.err[1], 1u);
+            } else {
+                split = true;
+                const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
+                                          (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
+                for (uint32_t e = sg.start; e < sg.start + sg.count; e++)
+                    nl += (B.cen[3 * src[e] + axis] < mid) ? 1u : 0u;
+                if (nl == 0u || nl == sg.count) {
+                    // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
+                    for (uint32_t e = sg.start; e < sg.start + sg.count; e++) dst[e] = src[e];
+                    nl = sg.count / 2;
+                } else {
+                    uint32_t il = sg.start, ir = sg.start + nl;
+                    for (uint32_t e = sg.start; e < sg.start + sg.count; e++) {
+                        const uint32_t s = src[e];
+                        if (B.cen[3 * s + axis] < mid) dst[il++] = s;
+                        else dst[ir++] = s;
+                    }
+                }
+                if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
             }
-            continue;
         }
-        const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
-                                  (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
-        const double c = axis == 0 ? cen[0] : (axis == 1 ? cen[1] : cen[2]);
-        const bool left = valid && c < mid;
-        const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
-        uint32_t nl = (uint32_t)__popcll(ml);
-        if (nl == 0u || nl == sg.count) {
-            // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
-            if (valid) dst[e] = s;
-            nl = sg.count / 2;
-        } else if (valid) {
-            if (left) dst[sg.start + (uint32_t)__popcll(ml & below)] = s;
-            else dst[sg.start + nl + (uint32_t)__popcll(mr & below)] = s;
+        // the wave's splits: two child segments each, one reservation
+        const unsigned long long ms = __ballot(split);
+        if (ms == 0ull) continue;
+        const int leader = __ffsll((long long)ms) - 1;
+        uint32_t kbase = 0;
+        if ((int)lane == leader) kbase = atomicAdd(&B.seg_count[3 * outc + kSegLane], 2u * (uint32_t)__popcll(ms));
+        kbase = (uint32_t)__shfl((int)kbase, leader, 64) + 2u * (uint32_t)__popcll(ms & below);
+        // node ids: one reservation per tile among the wave's splits
+        uint32_t key = split ? sg.tile : kNone, id = 0;
+        for (;;) {
+            const unsigned long long pend = __ballot(key != kNone);
+            if (pend == 0ull) break;
+            const int lt = __ffsll((long long)pend) - 1;
+            const uint32_t tl = (uint32_t)__shfl((int)key, lt, 64);
+            const unsigned long long mt = __ballot(key == tl);
+            uint32_t b0 = 0;
+            if ((int)lane == lt) b0 = atomicAdd(&B.node_count[tl], 2u * (uint32_t)__popcll(mt));
+            b0 = (uint32_t)__shfl((int)b0, lt, 64);
+            if (key == tl) {
+                id = b0 + 2u * (uint32_t)__popcll(mt & below);
+                key = kNone;
+            }
         }
-        if (lane == 0) {
-            const TileDev &T = B.tiles[sg.tile];
-            const uint32_t id = atomicAdd(&B.node_count[sg.tile], 2u);
-            const uint32_t l = 2 * T.crystal_base + id, r = l + 1;
+        if (split) {
+            const uint32_t l = 2 * B.tiles[sg.tile].crystal_base + id, r = l + 1;
+            BNode &nd = B.bn[sg.node];
             nd.left = l;
             nd.right = r;
             B.bn[l].parent = sg.node;
             B.bn[r].parent = sg.node;
-            if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
-            const uint32_t k = atomicAdd(&B.seg_count[outc], 2u);
-            out[k] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
-            out[k + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
+            out[kbase] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
+            out[kbase + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
         }
-    }
-}
-
-// The level step for segments of at most kLaneSegment crystals, one lane per segment:
-// sequential box and extent (min/max are exact in any order), then the same stable
-// left/right partition as the block and wave forms.
-__global__ void __launch_bounds__(256) k_bvh_level_tiny(Bufs B, int level)
-{
-    const int in = level % 3, outc = (level + 1) % 3;
-    const Seg *segs = B.seg[level & 1];
-    Seg *out = B.seg[(level + 1) & 1];
-    const uint32_t *src = B.elem[level & 1];
-    uint32_t *dst = B.elem[(level + 1) & 1];
-    const uint32_t nseg = B.seg_count[in];
-    for (uint32_t si = blockIdx.x * blockDim.x + threadIdx.x; si < nseg; si += gridDim.x * blockDim.x) {
-        const Seg sg = segs[si];
-        if (sg.count > kLaneSegment) continue;
-        float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t e = sg.start; e < sg.start + sg.count; e++) {
-            const uint32_t s = src[e];
-            for (int a = 0; a < 3; a++) {
-                blo[a] = fminf(blo[a], B.elo[3 * s + a]);
-                bhi[a] = fmaxf(bhi[a], B.ehi[3 * s + a]);
-                cmn[a] = fmin(cmn[a], B.cen[3 * s + a]);
-                cmx[a] = fmax(cmx[a], B.cen[3 * s + a]);
-            }
-        }
-        BNode &nd = B.bn[sg.node];
-        const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
-        const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
-        const double ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
-        const bool leaf = sg.count <= 4u || ext <= 1e-7;
-        for (int a = 0; a < 3; a++) {
-            nd.lo[a] = blo[a];
-            nd.hi[a] = bhi[a];
-        }
-        nd.depth = sg.depth;
-        nd.start = sg.start;
-        nd.count = sg.count;
-        if (leaf) {
-            for (uint32_t e = sg.start; e < sg.start + sg.count; e++) B.final_elem[e] = src[e];
-            nd.left = nd.right = kNone;
-            if (sg.count > kMaxLeafCount) atomicOr(&B.err[1], 1u);
-            continue;
-        }
-        const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
-                                  (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
-        uint32_t nl = 0;
-        for (uint32_t e = sg.start; e < sg.start + sg.count; e++) nl += (B.cen[3 * src[e] + axis] < mid) ? 1u : 0u;
-        if (nl == 0u || nl == sg.count) {
-            // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
-            for (uint32_t e = sg.start; e < sg.start + sg.count; e++) dst[e] = src[e];
-            nl = sg.count / 2;
-        } else {
-            uint32_t il = sg.start, ir = sg.start + nl;
-            for (uint32_t e = sg.start; e < sg.start + sg.count; e++) {
-                const uint32_t s = src[e];
-                if (B.cen[3 * s + axis] < mid) dst[il++] = s;
-                else dst[ir++] = s;
-            }
-        }
-        const TileDev &T = B.tiles[sg.tile];
-        const uint32_t id = atomicAdd(&B.node_count[sg.tile], 2u);
-        const uint32_t l = 2 * T.crystal_base + id, r = l + 1;
-        nd.left = l;
-        nd.right = r;
-        B.bn[l].parent = sg.node;
-        B.bn[r].parent = sg.node;
-        if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
-        const uint32_t k = atomicAdd(&B.seg_count[outc], 2u);
-        out[k] = Seg{sg.tile, sg.start, nl, l, sg.depth + 1};
-        out[k + 1] = Seg{sg.tile, sg.start + nl, sg.count - nl, r, sg.depth + 1};
     }
 }
 
 __global__ void k_build_check(Bufs B)
 {
-    if (threadIdx.x == 0 && B.seg_count[kMaxLevels % 3] != 0u) B.err[2] = 1u;
+    const uint32_t *left = B.seg_count + 3u * (kMaxLevels % 3);
+    if (threadIdx.x == 0 && (left[0] | left[1] | left[2]) != 0u) B.err[2] = 1u;
 }
 
 __device__ __forceinline__ bool node_valid(const Bufs &B, uint64_t g, uint32_t &t)
@@ -799,9 +904,14 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.leafcnt, 2 * n));
     TRY(alloc(b, B.dnodes, 2 * n));
     TRY(alloc(b, B.drecs, n));
-    TRY(alloc(b, B.seg[0], n));
-    TRY(alloc(b, B.seg[1], n));
-    TRY(alloc(b, B.seg_count, 3));
+    // a level's segments partition its developed crystals: at most n / 65 have more
+    // than kWaveSegment crystals, n / 17 more than kLaneSegment
+    for (int p = 0; p < 2; p++) {
+        TRY(alloc(b, B.seg[p][kSegBlock], n / (kWaveSegment + 1) + 1));
+        TRY(alloc(b, B.seg[p][kSegWave], n / (kLaneSegment + 1) + 1));
+        TRY(alloc(b, B.seg[p][kSegLane], n));
+    }
+    TRY(alloc(b, B.seg_count, 3 * kSegClasses));
     TRY(alloc(b, B.node_count, b->n_tiles));
     TRY(alloc(b, B.work, 1));  // 64-bit packet counter
     TRY(alloc(b, B.err, 4));
