@@ -68,6 +68,8 @@ struct Bufs {
     uint32_t *node_count;  // per tile
     unsigned long long *work;  // transport work counter
     uint32_t *err;         // [0] depth, [1] leaf size, [2] unfinished build
+    uint32_t *devlist;     // developed slots (any order) and their number
+    uint32_t *dev_count;
     TileDev *tiles;
 };
 
@@ -130,6 +132,7 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
     if (blockIdx.x == 0) {
         if (threadIdx.x < 3 * kSegClasses) B.seg_count[threadIdx.x] = 0u;
         if (threadIdx.x < 4) B.err[threadIdx.x] = 0u;
+        if (threadIdx.x == 0) *B.dev_count = 0u;
         if (threadIdx.x == 0) *B.work = 0ull;
     }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -179,21 +182,48 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))
 __global__ void __launch_bounds__(256) k_threshold(Bufs B, const uint32_t *counts, uint64_t n,
                                                    DevelopParams P)
 {
+    __shared__ uint32_t wcnt[4], base;
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t d = counts[s] >= P.threshold ? 1u : 0u;
-    B.devflag[s] = d;
-    B.fe[s] = d ? P.fe : 0.0;
-    B.parent[s] = (uint32_t)s;
-    B.clump[s] = -1;
+    const uint32_t d = (s < n && counts[s] >= P.threshold) ? 1u : 0u;
+    if (s < n) {
+        B.devflag[s] = d;
+        B.fe[s] = d ? P.fe : 0.0;
+        B.parent[s] = (uint32_t)s;
+        B.clump[s] = -1;
+    }
+    // append the block's developed slots to devlist (one atomic per block)
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(d != 0u);
+    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        base = tot ? atomicAdd(B.dev_count, tot) : 0u;
+    }
+    __syncthreads();
+    if (d) {
+        uint32_t off = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        for (uint32_t w = 0; w < wave; w++) off += wcnt[w];
+        B.devlist[off] = (uint32_t)s;
+    }
 }
 
-// touching developed pairs within a tile, found by a box query on the tile's
-// transport BVH (inclusive overlap as bvh.rs:373-379), decided in double precision
-__global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, uint64_t n, DevelopParams P)
+// Touching developed pairs within a tile, found by a box query on the tile's transport
+// BVH (inclusive overlap as bvh.rs:373-379), decided in double precision; one lane per
+// developed slot of devlist.  The walk queues hit leaves in an LDS column per lane and
+// tests their crystals after the node steps, as the transport does; the union-find
+// components do not depend on the order of the unions.
+constexpr uint32_t kEdgeQueue = 8;
+
+__global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, DevelopParams P)
 {
-    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n || !B.devflag[s]) return;
+    __shared__ uint32_t lq_all[kEdgeQueue][256];
+    uint32_t *lq = &lq_all[0][threadIdx.x];
+    const uint32_t nd = *B.dev_count;
+    if (blockIdx.x * blockDim.x >= nd) return;   // whole block past the list
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = q < nd;
+    const uint64_t s = live ? B.devlist[q] : 0u;
     const TileDev T = B.tiles[B.tile_of[s]];
     const double px = B.pos64[3 * s], py = B.pos64[3 * s + 1], pz = B.pos64[3 * s + 2];
     const double ri = B.mdim[s] * P.fe;
@@ -201,13 +231,29 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, uint64
     const float qlo[3] = {__double2float_rd(px - R), __double2float_rd(py - R), __double2float_rd(pz - R)};
     const float qhi[3] = {__double2float_ru(px + R), __double2float_ru(py + R), __double2float_ru(pz + R)};
     const Node *nd0 = nodes + T.node_base;
-    uint32_t i = 0;
-    while (i < T.n_nodes) {
-        const NodeView nd = load_node(nd0, i);
-        const bool ov = nd.lo[0] <= qhi[0] && nd.hi[0] >= qlo[0] && nd.lo[1] <= qhi[1] &&
-                        nd.hi[1] >= qlo[1] && nd.lo[2] <= qhi[2] && nd.hi[2] >= qlo[2];
-        if (ov && nd.prim != kInterior) {
-            const uint32_t first = nd.prim >> 4, cnt = nd.prim & 15u;
+    const uint32_t n_nodes = live ? T.n_nodes : 0u;
+    uint32_t i = 0, nq = 0;
+    for (;;) {
+        const uint32_t walking = (uint32_t)__popcll(__ballot(i < n_nodes));
+        if (walking == 0u) break;
+        const uint32_t stop = walking / 10u;
+        for (uint32_t it = 0;; it++) {
+            const bool stepping = nq < kEdgeQueue && i < n_nodes;
+            const uint32_t ns = (uint32_t)__popcll(__ballot(stepping));
+            if (ns == 0u || (it > 0u && ns <= stop)) break;
+            if (stepping) {
+                const NodeView v = load_node(nd0, i);
+                const bool ov = v.lo[0] <= qhi[0] && v.hi[0] >= qlo[0] && v.lo[1] <= qhi[1] &&
+                                v.hi[1] >= qlo[1] && v.lo[2] <= qhi[2] && v.hi[2] >= qlo[2];
+                if (ov && v.prim != kInterior) lq[256u * nq++] = v.prim;
+                i = (ov && v.prim == kInterior) ? i + 1 : v.skip;
+            }
+        }
+#pragma unroll 1
+        for (uint32_t qd = 0; qd < kEdgeQueue; qd++) {
+            if (qd >= nq) continue;
+            const uint32_t lf = lq[256u * qd];
+            const uint32_t first = lf >> 4, cnt = lf & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
                 const uint64_t j = (uint64_t)T.crystal_base + k;
                 if (j <= s || !B.devflag[j]) continue;
@@ -219,7 +265,7 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, uint64
                     uf_unite(B.parent, B.orig, (uint32_t)s, (uint32_t)j);
             }
         }
-        i = (ov && nd.prim == kInterior) ? i + 1 : nd.skip;
+        nq = 0u;
     }
 }
 
@@ -338,7 +384,7 @@ __global__ void k_root_segments(Bufs B, uint32_t n_tiles)
     B.bn[2 * T.crystal_base].parent = kNone;
 }
 
-template <class V, class Op>
+template <int kWaves, class V, class Op>
 __device__ __forceinline__ V block_reduce(V v, V *lds, Op op)
 {
     for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
@@ -347,7 +393,7 @@ __device__ __forceinline__ V block_reduce(V v, V *lds, Op op)
     if (lane == 0) lds[wave] = v;
     __syncthreads();
     V r = lds[0];
-    for (int w = 1; w < 4; w++) r = op(r, lds[w]);
+    for (int w = 1; w < kWaves; w++) r = op(r, lds[w]);
     return r;
 }
 
@@ -384,14 +430,17 @@ __device__ __forceinline__ void emit_split(Bufs &B, int level, const Seg &sg, ui
     }
 }
 
-// One level of the median-split build (bvh.rs:166-295), one block per segment of more
-// than kWaveSegment crystals.
-__global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
+// One level of the median-split build (bvh.rs:166-295), one block of kLevelThreads per
+// segment of more than kWaveSegment crystals.  Most such segments hold a few hundred
+// crystals: 1024-lane blocks made the T-Max frame's levels 27.8 -> 265.6 ms.
+constexpr uint32_t kLevelThreads = 256u, kLevelWaves = kLevelThreads / 64u;
+
+__global__ void __launch_bounds__(kLevelThreads) k_bvh_level(Bufs B, int level)
 {
-    __shared__ float lf[4];
-    __shared__ double ld[4];
-    __shared__ uint32_t lu[4];
-    __shared__ uint32_t wl[4], wr[4];
+    __shared__ float lf[kLevelWaves];
+    __shared__ double ld[kLevelWaves];
+    __shared__ uint32_t lu[kLevelWaves];
+    __shared__ uint32_t wl[kLevelWaves], wr[kLevelWaves];
     const int in = level % 3, zc = (level + 2) % 3;
     const Seg *segs = B.seg[level & 1][kSegBlock];
     const uint32_t *src = B.elem[level & 1];
@@ -403,7 +452,7 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
         const Seg sg = segs[si];
         float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
         double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256) {
+        for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += kLevelThreads) {
             const uint32_t s = src[e];
             for (int a = 0; a < 3; a++) {
                 blo[a] = fminf(blo[a], B.elo[3 * s + a]);
@@ -413,10 +462,10 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
             }
         }
         for (int a = 0; a < 3; a++) {
-            blo[a] = block_reduce(blo[a], lf, MinF());
-            bhi[a] = block_reduce(bhi[a], lf, MaxF());
-            cmn[a] = block_reduce(cmn[a], ld, MinD());
-            cmx[a] = block_reduce(cmx[a], ld, MaxD());
+            blo[a] = block_reduce<kLevelWaves>(blo[a], lf, MinF());
+            bhi[a] = block_reduce<kLevelWaves>(bhi[a], lf, MaxF());
+            cmn[a] = block_reduce<kLevelWaves>(cmn[a], ld, MinD());
+            cmx[a] = block_reduce<kLevelWaves>(cmx[a], ld, MaxD());
         }
         BNode &nd = B.bn[sg.node];
         const double ex = cmx[0] - cmn[0], ey = cmx[1] - cmn[1], ez = cmx[2] - cmn[2];
@@ -433,7 +482,7 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
             nd.count = sg.count;
         }
         if (leaf) {
-            for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256)
+            for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += kLevelThreads)
                 B.final_elem[e] = src[e];
             if (threadIdx.x == 0) {
                 nd.left = nd.right = kNone;
@@ -445,16 +494,16 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
         const double mid = 0.5 * ((axis == 0 ? cmn[0] : axis == 1 ? cmn[1] : cmn[2]) +
                                   (axis == 0 ? cmx[0] : axis == 1 ? cmx[1] : cmx[2]));
         uint32_t nl = 0;
-        for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256)
+        for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += kLevelThreads)
             nl += (B.cen[3 * src[e] + axis] < mid) ? 1u : 0u;
-        nl = block_reduce(nl, lu, SumU());
+        nl = block_reduce<kLevelWaves>(nl, lu, SumU());
         if (nl == 0u || nl == sg.count) {
             // bvh.rs:262-266 fallback; unreachable while the centroid extent exceeds 1e-7
-            for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += 256) dst[e] = src[e];
+            for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += kLevelThreads) dst[e] = src[e];
             nl = sg.count / 2;
         } else {
             uint32_t base_l = 0, base_r = 0;
-            for (uint32_t c = 0; c < sg.count; c += 256) {
+            for (uint32_t c = 0; c < sg.count; c += kLevelThreads) {
                 const uint32_t e = sg.start + c + threadIdx.x;
                 const bool valid = c + threadIdx.x < sg.count;
                 const uint32_t s = valid ? src[e] : 0u;
@@ -467,7 +516,7 @@ __global__ void __launch_bounds__(256) k_bvh_level(Bufs B, int level)
                 }
                 __syncthreads();
                 uint32_t ol = 0, orr = 0, tl = 0, tr = 0;
-                for (uint32_t w = 0; w < 4; w++) {
+                for (uint32_t w = 0; w < kLevelWaves; w++) {
                     if (w < wave) {
                         ol += wl[w];
                         orr += wr[w];
@@ -915,6 +964,8 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.node_count, b->n_tiles));
     TRY(alloc(b, B.work, 1));  // 64-bit packet counter
     TRY(alloc(b, B.err, 4));
+    TRY(alloc(b, B.devlist, n));
+    TRY(alloc(b, B.dev_count, 1));
     TRY(alloc(b, B.tiles, b->n_tiles));
     // host-side per-slot arrays
     std::vector<double> mdim(n);
@@ -988,13 +1039,13 @@ static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
                        in.counts, in.emap, b->n_tiles, b->total_photons);
     TRY(mark(b, 1, s));
     hipLaunchKernelGGL(k_threshold, dim3(nb), dim3(256), 0, s, B, in.counts, n, in.develop);
-    hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, n, in.develop);
+    hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, in.develop);
     hipLaunchKernelGGL(k_roots, dim3(nb), dim3(256), 0, s, B, n);
     hipLaunchKernelGGL(k_tile_scans, dim3(b->n_tiles), dim3(256), 0, s, B);
     hipLaunchKernelGGL(k_boost, dim3(nb), dim3(256), 0, s, B, n, in.develop);
     hipLaunchKernelGGL(k_root_segments, dim3((b->n_tiles + 255) / 256), dim3(256), 0, s, B, b->n_tiles);
     for (int level = 0; level < kMaxLevels; level++) {
-        hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(256), 0, s, B, level);
+        hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(kLevelThreads), 0, s, B, level);
         hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
         hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
     }
