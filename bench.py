@@ -349,11 +349,13 @@ def main():
     alg_bytes = per_path["bytes"] * photons
     dev_bytes = 4 * photons + 32 * agg["nodes_visited"] + 48 * agg["prims_tested"] + 8 * agg["abs_events"]
     traffic = None
+    prof = {}
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath) and args.mode == "batched":
         # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs,
         # copy-calibrated (profiles/transport_scan_frame24mp_batched.md)
-        traffic = json.load(open(tpath))["transport_traffic_bytes"]
+        prof = json.load(open(tpath))
+        traffic = prof["transport_traffic_bytes"]
     roof = None
     if agg["ms_kernel_transport"] > 0:
         ach = alg_bytes * args.steps / (agg["ms_kernel_transport"] * 1e-3) / 1e9
@@ -361,6 +363,13 @@ def main():
                 "frac": ach / MI355X_HBM_GBPS, "traffic": traffic,
                 "bytes_per_path": per_path["bytes"],
                 "launches_per_step": 1 if args.mode == "batched" else agg["tiles"]}
+        if prof:
+            # what bounds the kernel instead: the algorithmic bytes are served from L2 and
+            # the MALL (frac above 1); lane utilisation and L2 hit rate from the same
+            # committed rocprofv3 runs
+            roof["profiled"] = {"valu_lane_util": prof.get("transport_valu_lane_util"),
+                                "l2_hit_rate": prof.get("transport_l2_hit_rate"),
+                                "kernel": prof.get("kernel")}
     out = {
         "metric": "Mphoton-paths/s + scanned Mpix/s, 24MP Tri-X, 1 GPU; adjoint/forward time ratio",
         "value": photons / (ms_step * 1e-3) / 1e6,
