@@ -130,6 +130,28 @@ def test_batched_development_matches_oracle(engine):
         base += n
 
 
+@pytest.mark.parametrize("threshold", [0, 1 << 30])
+def test_batched_development_extremes_equal_plain(engine, threshold):
+    """Every crystal developed (the wave-aggregated clump sizes and per-tile largest clump,
+    every BVH size class and the bottom-up leaf counts at full load) and none developed
+    (empty development lists, T = 1): the batched device path equals the per-tile path."""
+    cfg = _cfg(ppp=2, seed=21)
+    cfg["development"]["threshold_photons"] = threshold
+    w, h = 360, 200
+    fr = W.Frame(engine, cfg, W.blob_exposure(w, h, 3, sigma=(30, 90)), frame=(w, h))
+    sp = fr.render(W.Frame.PLAIN)
+    Dp, (dp, fp, kp) = fr.read(), fr.development()
+    sb = fr.render(W.Frame.BATCHED)
+    Db, (db, fb, kb) = fr.read(), fr.development()
+    assert np.array_equal(dp, db) and np.array_equal(kp, kb) and np.array_equal(fp, fb)
+    assert sp["developed"] == sb["developed"] and sp["clumps"] == sb["clumps"]
+    if threshold == 0:
+        assert sb["developed"] == fr.crystals and 0 < sb["clumps"] <= sb["developed"]
+    else:
+        assert sb["developed"] == 0 and (Db == np.float32(0.05)).all()
+    assert np.abs(Dp - Db).max() <= 1e-5
+
+
 def test_batched_frame_with_otf_matches_oracle_filter(engine):
     cfg = _cfg(ppp=4, otf=1, ss=1, seed=3)
     w, h = 352, 90
