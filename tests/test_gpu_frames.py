@@ -210,3 +210,35 @@ def test_autotuner_on_a_small_crop(engine, monkeypatch):
     assert r["ms_kernel_transport"] == best
     assert r["tuning"]["scan_rays_per_thread"] in (1, 4)
     assert engine.tuning()["transport_chunk"] == 64   # restored
+
+
+def test_full_frame_properties(engine):
+    """The benchmark's full frame (6000 x 4000 px, 247 Tri-X tiles, 3.07e9 photon paths) on
+    the batched path, through properties that do not depend on its size: graph replays
+    are bit-identical, another launch tuning (packet chunk, leaf wait, transport
+    occupancy) deposits the same photons, every latent photon belongs to a counted
+    absorption event, and D_neg stays inside its mapped range."""
+    cfg = W.load_config("frame24mp_trix_cubic")
+    fr = W.Frame(engine, cfg, W.workload_exposure(cfg))
+    old = engine.tuning()
+    try:
+        engine.set_tuning(**dict(old, transport_chunk=256, transport_wait_pct=90, transport_waves=7))
+        s1 = fr.render(W.Frame.BATCHED)
+        D1, c1 = fr.read(), fr.counts()
+        fr.render(W.Frame.BATCHED)
+        D2, c2 = fr.read(), fr.counts()
+        engine.set_tuning(**dict(old, transport_chunk=64, transport_wait_pct=80, transport_waves=6))
+        s3 = fr.render(W.Frame.BATCHED)
+        c3 = fr.counts()
+    finally:
+        engine.set_tuning(**old)
+    assert s1["tiles"] == 247 and s1["photons"] == 128 * 6000 * 4000 and s1["rays"] == 16 * 6000 * 4000
+    assert np.array_equal(c1, c2) and (D1.view(np.uint32) == D2.view(np.uint32)).all()
+    assert np.array_equal(c1, c3)
+    assert (s3["bounces"], s3["nodes_visited"], s3["prims_tested"], s3["abs_events"]) == \
+        (s1["bounces"], s1["nodes_visited"], s1["prims_tested"], s1["abs_events"])
+    assert s1["developed"] == s3["developed"] and 0 < s1["developed"] < s1["crystals"]
+    assert int(c1.sum(dtype=np.uint64)) >= s1["abs_events"] > 0
+    assert np.isfinite(D1).all()
+    # the scanner OTF is a convex 3-tap blend, so it keeps the range up to rounding
+    assert D1.min() >= 0.05 - 1e-5 and D1.max() <= 4.05 + 1e-5
