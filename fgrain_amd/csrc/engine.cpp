@@ -562,13 +562,12 @@ bool valid_scan(const fg_scan_config *c, std::string &msg)
     return true;
 }
 
-static fg_status scan_locked(fg_engine *E, const Emulsion &em, const fg_scan_config &cfg, uint64_t seed,
+// The scan of an emulsion whose developed-crystal BVH (nodes, recs) was built on the
+// host by build_scan_bvh; the caller holds E->mu (the build needs no lock).
+static fg_status scan_locked(fg_engine *E, const Emulsion &em, const std::vector<Node> &nodes,
+                             const std::vector<Crystal> &recs, const fg_scan_config &cfg, uint64_t seed,
                              float *out, float *kernel_ms)
 {
-    std::vector<Node> nodes;
-    std::vector<Crystal> recs;
-    std::string msg;
-    if (!build_scan_bvh(em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "scan: " + msg);
     HIP_TRY(E->scan_nodes.ensure(nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(E->scan_recs.ensure(recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
     if (!nodes.empty())
@@ -621,9 +620,12 @@ extern "C" fg_status fg_scan(fg_engine *E, const fg_emulsion *em, const fg_scan_
     if (!E || !em || !out) return fail(FG_ERR_INVALID_ARG, "fg_scan: null argument");
     if (!valid_scan(cfg, msg)) return fail(FG_ERR_INVALID_ARG, "fg_scan: " + msg);
     if (cfg->width_px == 0 || cfg->height_px == 0) return FG_OK;
+    std::vector<Node> nodes;
+    std::vector<Crystal> recs;
+    if (!build_scan_bvh(em->em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "fg_scan: " + msg);
     std::lock_guard<std::mutex> lk(E->mu);
     HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
-    return scan_locked(E, em->em, *cfg, seed, out, nullptr);
+    return scan_locked(E, em->em, nodes, recs, *cfg, seed, out, nullptr);
 }
 
 static fg_status traverse(fg_engine *E, const fg_emulsion *emw, int64_t n_rays, const float *o,
@@ -783,8 +785,9 @@ extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img,
     const fg_emulsion_config ec = tile_emulsion_config(*p);
     std::string msg;
     if (!valid_cfg(&ec, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: " + msg);
-    std::lock_guard<std::mutex> lk(E->mu);
-    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    // The engine lock covers the GPU phases only (the reference's dispatch_lock,
+    // gpu.rs:169, 503, serialises GPU work): synthesis, development and the scan's BVH
+    // build run on the calling thread, so callers on several threads overlap them.
     const auto t0 = Clock::now();
     fg_emulsion em;
     synthesize(ec, p->seed, em.em);
@@ -797,8 +800,13 @@ extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img,
     xc.photons_per_pixel = 0;
     fg_expose_stats xs;
     const auto t1 = Clock::now();
-    fg_status s = expose_locked(E, em.em, tile_img, p->width_px, p->height_px, xc, p->seed + 1, &xs);
-    free_device(em.em);
+    fg_status s;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+        s = expose_locked(E, em.em, tile_img, p->width_px, p->height_px, xc, p->seed + 1, &xs);
+        free_device(em.em);
+    }
     if (s != FG_OK) return s;
     const auto t2 = Clock::now();
     fg_development_config dc{p->threshold_photons, 0, p->filament_expansion, p->clumping_proximity_factor, 0.85};
@@ -806,7 +814,14 @@ extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img,
     develop_host(em.em, dc, rep);
     const auto t3 = Clock::now();
     fg_scan_config sc{p->width_px, p->height_px, p->samples_per_pixel, FG_SCAN_DNEG, p->silver_opacity, 1, 0};
-    s = scan_locked(E, em.em, sc, p->seed + 2, out, nullptr);
+    std::vector<Node> nodes;
+    std::vector<Crystal> recs;
+    if (!build_scan_bvh(em.em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: " + msg);
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+        s = scan_locked(E, em.em, nodes, recs, sc, p->seed + 2, out, nullptr);
+    }
     if (s != FG_OK) return s;
     if (stats) {
         stats->crystals = (uint64_t)em.em.n;

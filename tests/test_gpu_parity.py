@@ -350,3 +350,21 @@ def test_render_tile_matches_stage_pipeline(engine):
     F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
     D2, _ = F.scan_emulsion(em, F.ScanConfig(40, 32, 4, F.ScanMode.NegativeDensity, 0.25), 1236, engine)
     assert (D.reshape(32, 40).view(np.uint32) == D2.view(np.uint32)).all()
+
+
+def test_render_tile_all_gpu_from_threads_equals_sequential(engine):
+    """Callers on several threads (the reference's Rayon tile loop, image_pipeline.rs:
+    427-591) get exactly the results of sequential calls: the engine lock covers the GPU
+    phases while synthesis, development and the scan's BVH build run on each caller's
+    thread."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = []
+    for k in range(8):
+        p = F.GpuTileParams.for_preset("tri-x", 2.4, 48, 40, exposure=1.0, tile_seed=500 + 17 * k)
+        img = np.random.default_rng(k).uniform(0.0, 1.0, 48 * 40).astype(np.float32)
+        jobs.append((img, p))
+    seq = [engine.render_tile_all_gpu(img, p) for img, p in jobs]
+    with ThreadPoolExecutor(4) as pool:
+        par = list(pool.map(lambda j: engine.render_tile_all_gpu(*j), jobs))
+    for a, b in zip(seq, par):
+        assert (a.view(np.uint32) == b.view(np.uint32)).all()
