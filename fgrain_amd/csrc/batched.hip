@@ -63,6 +63,7 @@ struct Bufs {
     uint32_t *leafcnt;
     Node *dnodes;
     Crystal *drecs;
+    float4 *dconst;        // down_constants of the developed-crystal records
     Seg *seg[2][kSegClasses];   // [level parity][size class]
     uint32_t *seg_count;        // [3 rotating levels][size class]
     uint32_t *node_count;  // per tile
@@ -868,6 +869,9 @@ __global__ void __launch_bounds__(256) k_dev_records(Bufs B, const Crystal *recs
     const double fe = B.fe[s];
     for (int a = 0; a < 3; a++) r.h[a] = (float)(B.dims64[3 * (size_t)s + a] * fe);
     B.drecs[e] = r;
+    float k[4];
+    down_constants(r.q, r.h, r.morph, k);
+    B.dconst[e] = make_float4(k[0], k[1], k[2], k[3]);
 }
 
 __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, uint32_t n_tiles)
@@ -881,7 +885,8 @@ __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, 
     const Node *nodes = B.dnodes + 2ull * T.crystal_base;
     const float4 *recs = reinterpret_cast<const float4 *>(B.drecs + T.crystal_base);
     const float Tm = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, S.g, S.ss, T.pw, T.ph, S.z0,
-                                S.sigma, T.skey0, T.skey1, nodes, B.node_count[t], recs);
+                                S.sigma, T.skey0, T.skey1, nodes, B.node_count[t], recs,
+                                B.dconst + T.crystal_base);
     if (valid && threadIdx.x % L == 0u)
         out[T.out_base + (size_t)py * T.out_stride + px] = scan_value(Tm, S.mode);
 }
@@ -953,6 +958,7 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.leafcnt, 2 * n));
     TRY(alloc(b, B.dnodes, 2 * n));
     TRY(alloc(b, B.drecs, n));
+    TRY(alloc(b, B.dconst, n));
     // a level's segments partition its developed crystals: at most n / 65 have more
     // than kWaveSegment crystals, n / 17 more than kLaneSegment
     for (int p = 0; p < 2; p++) {

@@ -50,6 +50,15 @@ __device__ __forceinline__ Rec load_rec(const float4 *recs, uint32_t k)
 // margin well above the float rounding of crystal_hit) the exact test would miss too,
 // so skipping it changes no result.  The rays of one pixel sit in adjacent lanes, so
 // the test mostly rejects for the whole group at once.
+// the same test with the bounding-sphere radius R = sqrtf(r2) precomputed (down_constants)
+__device__ __forceinline__ bool sphere_may_hit_z_r(const float4 &a, float R, float ox, float oy)
+{
+    const float vx = a.x - ox, vy = a.y - oy;
+    const float d2 = vx * vx + vy * vy;
+    const float m = R * 1.001f + 1e-4f * (fabsf(vx) + fabsf(vy)) + 1e-4f;
+    return d2 <= m * m;
+}
+
 __device__ __forceinline__ bool sphere_may_hit_z(const float4 &a, const float4 &c, float ox, float oy)
 {
     const float vx = a.x - ox, vy = a.y - oy;
@@ -731,14 +740,18 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
 // queues the leaves it hits (up to kScanQueue, in an LDS column of its own,
 // lq[q * kQueueLanes]) and keeps stepping; steps continue until 90 % of the walking lanes
 // have a full queue or have ended their walk, then every lane tests the crystals of its
-// queued leaves in the order found.  The chords are added in depth-first order either
+// queued leaves in the order found.  consts holds each record's down_constants (the
+// ray direction in the crystal frame is a constant of the crystal for rays along -z);
+// crystal_hit_down with them is crystal_hit bit for bit.  The chords are added in
+// depth-first order either
 // way, so queueing changes no bit (frame scans 261 -> 148 ms Tri-X, 417 -> 191 ms
 // T-Max against testing each leaf when it is hit).  Lanes with active = false take part
 // in no step.
 constexpr uint32_t kScanQueue = 8;
 
 __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t n_nodes, const float4 *recs,
-                                                    float ox, float oy, float oz, bool active, uint32_t *lq)
+                                                    const float4 *consts, float ox, float oy, float oz,
+                                                    bool active, uint32_t *lq)
 {
     float P = 0.0f;
     uint32_t i = active ? 0u : n_nodes, nq = 0;
@@ -763,10 +776,12 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
             const uint32_t lf = lq[kQueueLanes * q];
             const uint32_t first = lf >> 4, cnt = lf & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
-                if (!sphere_may_hit_z(recs[3 * k], recs[3 * k + 2], ox, oy)) continue;
+                const float4 kc = consts[k];
+                if (!sphere_may_hit_z_r(recs[3 * k], kc.w, ox, oy)) continue;
                 const Rec r = load_rec(recs, k);
+                const float kk[4] = {kc.x, kc.y, kc.z, kc.w};
                 float t0, t1;
-                if (crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, 0.0f, 0.0f, -1.0f, t0, t1))
+                if (crystal_hit_down(r.c, r.q, r.h, r.morph, kk, ox, oy, oz, t0, t1))
                     P += fmaxf(t1 - t0, 0.0f);
             }
         }
@@ -817,7 +832,7 @@ __device__ __forceinline__ float scan_pixel(bool valid, uint32_t pix, uint32_t p
                                             uint32_t sub, uint32_t L, uint32_t g, uint32_t ss,
                                             float pw, float ph, float z0, float sigma, uint32_t key0,
                                             uint32_t key1, const Node *nodes, uint32_t n_nodes,
-                                            const float4 *recs)
+                                            const float4 *recs, const float4 *consts)
 {
     __shared__ uint32_t lq_all[kScanQueue][kQueueLanes];
     uint32_t *lq = &lq_all[0][threadIdx.x];
@@ -828,7 +843,7 @@ __device__ __forceinline__ float scan_pixel(bool valid, uint32_t pix, uint32_t p
             const uint32_t sx = r % g, sy = (r / g) % g, ax = (r / (g * g)) % ss, ay = r / (g * g * ss);
             float ox, oy;
             scan_origin(pix, px, py, ax, ay, sx, sy, g, ss, pw, ph, key0, key1, ox, oy);
-            acc += expf(-sigma * silver_path_queued(nodes, n_nodes, recs, ox, oy, z0, true, lq));
+            acc += expf(-sigma * silver_path_queued(nodes, n_nodes, recs, consts, ox, oy, z0, true, lq));
         }
     }
     for (uint32_t off = L >> 1; off >= 1u; off >>= 1) acc += __shfl_xor(acc, (int)off, 64);

@@ -66,7 +66,7 @@ extern "C" fg_status fg_engine_create(fg_engine **out)
 extern "C" void fg_engine_destroy(fg_engine *E)
 {
     if (!E) return;
-    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->out, &E->out2,
+    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->scan_consts, &E->out, &E->out2,
                       &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
                       &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
                       &E->gmap, &E->partials, &E->pgrad, &E->gout, &E->boost, &E->dboost,
@@ -576,11 +576,15 @@ static fg_status scan_locked(fg_engine *E, const Emulsion &em, const std::vector
     if (!recs.empty())
         HIP_TRY(hipMemcpyAsync(E->scan_recs.p, recs.data(), recs.size() * sizeof(Crystal),
                                hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(E->scan_consts.ensure(recs.size() * sizeof(float4)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(launch_down_constants(E->scan_recs.as<float4>(), (uint32_t)recs.size(), E->scan_consts.as<float4>(),
+                                  E->stream), FG_ERR_PIPELINE);
     const size_t npx = (size_t)cfg.width_px * cfg.height_px;
     HIP_TRY(E->out.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
     ScanArgs a{};
     a.nodes = E->scan_nodes.as<Node>();
     a.recs = E->scan_recs.as<float4>();
+    a.consts = E->scan_consts.as<float4>();
     a.n_nodes = (uint32_t)nodes.size();
     a.w_px = cfg.width_px;
     a.h_px = cfg.height_px;
@@ -856,7 +860,7 @@ struct fg_frame {
     DevBuf d_layout;
     std::vector<Emulsion> ems;
     int64_t n_crystals = 0, n_nodes = 0;
-    DevBuf d_nodes, d_recs, d_counts, d_emap, d_dneg, d_dneg2, d_stats, d_scan_nodes, d_scan_recs;
+    DevBuf d_nodes, d_recs, d_counts, d_emap, d_dneg, d_dneg2, d_stats, d_scan_nodes, d_scan_recs, d_scan_consts;
     bool rendered = false;
     bool otf_applied = false;
     uint32_t last_mode = FG_FRAME_PLAIN;
@@ -1029,7 +1033,7 @@ extern "C" void fg_frame_destroy(fg_frame *F)
     if (!F) return;
     if (F->batched) batched_destroy(F->batched);
     for (DevBuf *b : {&F->d_nodes, &F->d_recs, &F->d_counts, &F->d_emap, &F->d_dneg, &F->d_dneg2,
-                      &F->d_stats, &F->d_scan_nodes, &F->d_scan_recs, &F->d_layout})
+                      &F->d_stats, &F->d_scan_nodes, &F->d_scan_recs, &F->d_scan_consts, &F->d_layout})
         b->release();
     delete F;
 }
@@ -1090,9 +1094,13 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
         if (!recs.empty())
             HIP_TRY(hipMemcpyAsync(F->d_scan_recs.p, recs.data(), recs.size() * sizeof(Crystal),
                                    hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+        HIP_TRY(F->d_scan_consts.ensure(recs.size() * sizeof(float4)), FG_ERR_BUFFER_RANGE);
+        HIP_TRY(launch_down_constants(F->d_scan_recs.as<float4>(), (uint32_t)recs.size(),
+                                      F->d_scan_consts.as<float4>(), E->stream), FG_ERR_PIPELINE);
         ScanArgs s{};
         s.nodes = F->d_scan_nodes.as<Node>();
         s.recs = F->d_scan_recs.as<float4>();
+        s.consts = F->d_scan_consts.as<float4>();
         s.n_nodes = (uint32_t)nodes.size();
         s.w_px = t.tw;
         s.h_px = t.th;

@@ -146,6 +146,67 @@ FG_HD bool crystal_hit(const float c[3], const float q[4], const float h[3], uin
     return false;
 }
 
+// Constants of a crystal for rays along -z (the scan): the direction (0, 0, -1) rotated
+// into the crystal frame depends on the crystal only, so its inverse (cube) or its
+// components over the semi-axes (other shapes) are computed once per crystal; k[3] is the
+// bounding-sphere radius of the scan's pre-test.  crystal_hit_down with these constants
+// performs exactly crystal_hit's operations for the direction (0, 0, -1).
+FG_HD void down_constants(const float q[4], const float h[3], uint32_t morph, float k[4])
+{
+    float ex, ey, ez;
+    qinv_rotate(q, 0.0f, 0.0f, -1.0f, ex, ey, ez);
+    if (morph == 0u) {
+        k[0] = 1.0f / ex;
+        k[1] = 1.0f / ey;
+        k[2] = 1.0f / ez;
+    } else {
+        k[0] = ex / h[0];
+        k[1] = ey / h[1];
+        k[2] = ez / h[2];
+    }
+    const float hm = fmaxf(fmaxf(h[0], h[1]), h[2]);
+    const float r2 = morph == 0u ? h[0] * h[0] + h[1] * h[1] + h[2] * h[2] : hm * hm;
+    k[3] = sqrtf(r2);
+}
+
+FG_HD bool crystal_hit_down(const float c[3], const float q[4], const float h[3], uint32_t morph,
+                            const float k[4], float ox, float oy, float oz, float &t0, float &t1)
+{
+    float lx, ly, lz;
+    qinv_rotate(q, ox - c[0], oy - c[1], oz - c[2], lx, ly, lz);
+    if (morph == 0u) {
+        const float ix = k[0], iy = k[1], iz = k[2];
+        const float a1 = (-h[0] - lx) * ix, a2 = (h[0] - lx) * ix;
+        const float a3 = (-h[1] - ly) * iy, a4 = (h[1] - ly) * iy;
+        const float a5 = (-h[2] - lz) * iz, a6 = (h[2] - lz) * iz;
+        const float tmin = fmaxf(fmaxf(fminf(a1, a2), fminf(a3, a4)), fminf(a5, a6));
+        const float tmax = fminf(fminf(fmaxf(a1, a2), fmaxf(a3, a4)), fmaxf(a5, a6));
+        if (tmax >= tmin && tmax > 0.0f) {
+            t0 = fmaxf(tmin, 0.0f);
+            t1 = tmax;
+            return true;
+        }
+        return false;
+    }
+    const float px = lx / h[0], py = ly / h[1], pz = lz / h[2];
+    const float qx = k[0], qy = k[1], qz = k[2];
+    const float a = qx * qx + qy * qy + qz * qz;
+    const float b = 2.0f * (px * qx + py * qy + pz * qz);
+    const float cc = (px * px + py * py + pz * pz) - 1.0f;
+    const float disc = b * b - 4.0f * a * cc;
+    if (disc < 0.0f) return false;
+    const float s = sqrtf(disc);
+    const float r0 = (-b - s) / (2.0f * a);
+    const float r1 = (-b + s) / (2.0f * a);
+    const float te = fminf(r0, r1), tx = fmaxf(r0, r1);
+    if (tx > 0.0f) {
+        t0 = fmaxf(te, 0.0f);
+        t1 = tx;
+        return true;
+    }
+    return false;
+}
+
 // Chord of a crystal whose semi-axes are scaled by s (about its centre) and the chord's
 // derivative with respect to s.  The chord is crystal_hit's on the scaled record; the
 // derivative is the closed form in the crystal frame: for the box, the exit and (when

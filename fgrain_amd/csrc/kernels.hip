@@ -49,8 +49,18 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
     const bool valid = pix < A.w_px * A.h_px;
     const uint32_t px = valid ? pix % A.w_px : 0u, py = valid ? pix / A.w_px : 0u;
     const float T = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, A.g, A.ss, A.pw, A.ph, A.z0,
-                               A.sigma, A.key0, A.key1, A.nodes, A.n_nodes, A.recs);
+                               A.sigma, A.key0, A.key1, A.nodes, A.n_nodes, A.recs, A.consts);
     if (valid && threadIdx.x % L == 0u) A.out[(size_t)py * A.ostride + px] = scan_value(T, A.mode);
+}
+
+__global__ void __launch_bounds__(256) down_constants_kernel(const float4 *recs, uint32_t n, float4 *consts)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Rec r = load_rec(recs, k);
+    float c[4];
+    down_constants(r.q, r.h, r.morph, c);
+    consts[k] = make_float4(c[0], c[1], c[2], c[3]);
 }
 
 // ------------------------------------------------------- traversal for tests ---
@@ -169,6 +179,13 @@ hipError_t launch_transport(const TransportArgs &a, hipStream_t s)
     if (a.n_photons == 0) return hipSuccess;
     const uint64_t blocks = (a.n_photons + 255) / 256;
     hipLaunchKernelGGL(transport_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_down_constants(const float4 *recs, uint32_t n, float4 *consts, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(down_constants_kernel, dim3((n + 255) / 256), dim3(256), 0, s, recs, n, consts);
     return hipGetLastError();
 }
 
