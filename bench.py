@@ -282,7 +282,8 @@ def main():
         args.steps = 3
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(tiles would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(a frame's tiles would shard across GPUs, a contiguous range of tiles per rank)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:
