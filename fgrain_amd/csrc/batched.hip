@@ -235,18 +235,22 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, Develo
     const uint32_t n_nodes = live ? T.n_nodes : 0u;
     uint32_t i = 0, nq = 0;
     for (;;) {
-        const uint32_t walking = (uint32_t)__popcll(__ballot(i < n_nodes));
+        const uint32_t walking = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(i < n_nodes));
         if (walking == 0u) break;
         const uint32_t stop = walking / 10u;
+        uint32_t lim = n_nodes;   // 0 once the lane's queue is full
         for (uint32_t it = 0;; it++) {
-            const bool stepping = nq < kEdgeQueue && i < n_nodes;
-            const uint32_t ns = (uint32_t)__popcll(__ballot(stepping));
+            const bool stepping = i < lim;
+            const uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(stepping));
             if (ns == 0u || (it > 0u && ns <= stop)) break;
             if (stepping) {
                 const NodeView v = load_node(nd0, i);
                 const bool ov = v.lo[0] <= qhi[0] && v.hi[0] >= qlo[0] && v.lo[1] <= qhi[1] &&
                                 v.hi[1] >= qlo[1] && v.lo[2] <= qhi[2] && v.hi[2] >= qlo[2];
-                if (ov && v.prim != kInterior) lq[256u * nq++] = v.prim;
+                if (ov && v.prim != kInterior) {
+                    lq[256u * nq++] = v.prim;
+                    if (nq == kEdgeQueue) lim = 0u;
+                }
                 i = (ov && v.prim == kInterior) ? i + 1 : v.skip;
             }
         }

@@ -571,11 +571,16 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
         cnt.bounces += (unsigned long long)__popcll(__ballot(started));
         // ---- node steps (reference DFS order via skip links)
         {
-            const uint32_t walking = (uint32_t)__popcll(__ballot(phase == kWalk));
-            const uint32_t stop = (walking * (100u - min(C.wait_pct, 100u))) / 100u;
+            // wave-uniform counts in scalar registers, and a lane's step condition as one
+            // compare: lim is the node count while the lane walks with room in its queue,
+            // else 0 (a compound condition costs the ballot two more VALU operations)
+            const uint32_t walking = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(phase == kWalk));
+            const uint32_t stop =
+                __builtin_amdgcn_readfirstlane((walking * (100u - min(C.wait_pct, 100u))) / 100u);
+            uint32_t lim = phase == kWalk ? T.n_nodes : 0u;
             for (uint32_t it = 0;; it++) {
-                const bool stepping = phase == kWalk && nq < (uint32_t)kLeafQueue && i < T.n_nodes;
-                const uint32_t ns = (uint32_t)__popcll(__ballot(stepping));
+                const bool stepping = i < lim;
+                const uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(stepping));
                 if (ns == 0u || (it > 0u && ns <= stop)) break;
                 cnt.nodes += ns;
                 if (stepping) {
@@ -583,7 +588,10 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     const bool near = !prune || (nd.lo[0] <= h0 && nd.hi[0] >= s0 && nd.lo[1] <= h1 &&
                                                  nd.hi[1] >= s1 && nd.lo[2] <= h2 && nd.hi[2] >= s2);
                     const bool hit = near && box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
-                    if (hit && nd.prim != kInterior) lq[kQueueLanes * nq++] = nd.prim;
+                    if (hit && nd.prim != kInterior) {
+                        lq[kQueueLanes * nq++] = nd.prim;
+                        if (nq == (uint32_t)kLeafQueue) lim = 0u;
+                    }
                     i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
                 }
             }
@@ -756,17 +764,21 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
     float P = 0.0f;
     uint32_t i = active ? 0u : n_nodes, nq = 0;
     for (;;) {
-        const uint32_t walking = (uint32_t)__popcll(__ballot(i < n_nodes));
+        const uint32_t walking = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(i < n_nodes));
         if (walking == 0u) break;
         const uint32_t stop = walking / 10u;
+        uint32_t lim = n_nodes;   // 0 once the lane's queue is full
         for (uint32_t it = 0;; it++) {
-            const bool stepping = nq < kScanQueue && i < n_nodes;
-            const uint32_t ns = (uint32_t)__popcll(__ballot(stepping));
+            const bool stepping = i < lim;
+            const uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(stepping));
             if (ns == 0u || (it > 0u && ns <= stop)) break;
             if (stepping) {
                 const NodeView nd = load_node(nodes, i);
                 const bool hit = nd.lo[0] < ox && ox < nd.hi[0] && nd.lo[1] < oy && oy < nd.hi[1];
-                if (hit && nd.prim != kInterior) lq[kQueueLanes * nq++] = nd.prim;
+                if (hit && nd.prim != kInterior) {
+                    lq[kQueueLanes * nq++] = nd.prim;
+                    if (nq == kScanQueue) lim = 0u;
+                }
                 i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
             }
         }
