@@ -384,6 +384,8 @@ extern "C" fg_status fg_emulsion_set_state(fg_emulsion *E, const uint32_t *count
 fg_status ensure_device(fg_engine *E, Emulsion &em)
 {
     if (em.dev) return FG_OK;
+    if (!offsets_fit_32(em.nodes.size(), em.recs.size()))
+        return fail(FG_ERR_INVALID_ARG, "emulsion: node or crystal arrays of 4 GiB or more");
     em.dev = new DeviceEmulsion();
     DeviceEmulsion &d = *em.dev;
     d.n = em.n;
@@ -1215,6 +1217,8 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
         in.scan.sigma = (float)D.scan.silver_opacity;
         if ((uint64_t)F->n_crystals * 2 >= 0xFFFFFFFFull)
             return fail(FG_ERR_INVALID_ARG, "batched frame: too many crystals for 32-bit node ids");
+        if (!offsets_fit_32((uint64_t)F->n_nodes, (uint64_t)F->n_crystals))
+            return fail(FG_ERR_INVALID_ARG, "batched frame: node or crystal arrays of 4 GiB or more");
         HIP_TRY(batched_create(in, E->stream, &F->batched), FG_ERR_BUFFER_RANGE);
     }
     HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);

@@ -48,6 +48,13 @@ struct DevBuf {
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
+// The transport addresses nodes (32 B) and crystal records (48 B) by 32-bit byte
+// offsets from the start of their arrays (raw buffer loads).
+inline bool offsets_fit_32(uint64_t n_nodes, uint64_t n_records)
+{
+    return n_nodes * 32ull < (1ull << 32) && n_records * 48ull < (1ull << 32);
+}
+
 namespace fg {
 struct DeviceEmulsion {
     DevBuf nodes, recs, counts;
