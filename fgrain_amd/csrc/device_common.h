@@ -803,6 +803,7 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
 {
     float P = 0.0f;
     uint32_t i = active ? 0u : n_nodes, nq = 0;
+    const __amdgpu_buffer_rsrc_t nodes_r = buffer_of(nodes), recs_r = buffer_of(recs), consts_r = buffer_of(consts);
     for (;;) {
         const uint32_t walking = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(i < n_nodes));
         if (walking == 0u) break;
@@ -813,7 +814,7 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
             const uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(stepping));
             if (ns == 0u || (it > 0u && ns <= stop)) break;
             if (stepping) {
-                const NodeView nd = load_node(nodes, i);
+                const NodeView nd = load_node_buf(nodes_r, i);
                 const bool hit = nd.lo[0] < ox && ox < nd.hi[0] && nd.lo[1] < oy && oy < nd.hi[1];
                 if (hit && nd.prim != kInterior) {
                     lq[kQueueLanes * nq++] = nd.prim;
@@ -828,9 +829,9 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
             const uint32_t lf = lq[kQueueLanes * q];
             const uint32_t first = lf >> 4, cnt = lf & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
-                const float4 kc = consts[k];
-                if (!sphere_may_hit_z_r(recs[3 * k], kc.w, ox, oy)) continue;
-                const Rec r = load_rec(recs, k);
+                const float4 kc = buf_load4(consts_r, k << 4);
+                if (!sphere_may_hit_z_r(buf_load4(recs_r, k * 48u), kc.w, ox, oy)) continue;
+                const Rec r = load_rec_buf(recs_r, k);
                 const float kk[4] = {kc.x, kc.y, kc.z, kc.w};
                 float t0, t1;
                 if (crystal_hit_down(r.c, r.q, r.h, r.morph, kk, ox, oy, oz, t0, t1))
