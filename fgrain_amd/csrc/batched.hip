@@ -231,7 +231,9 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, Develo
     const double R = (ri + T.maxb * P.fe) * P.pf;
     const float qlo[3] = {__double2float_rd(px - R), __double2float_rd(py - R), __double2float_rd(pz - R)};
     const float qhi[3] = {__double2float_ru(px + R), __double2float_ru(py + R), __double2float_ru(pz + R)};
-    const Node *nd0 = nodes + T.node_base;
+    // the resource is the frame's node array (wave-uniform); lanes of one wave may sit in
+    // different tiles, so a lane's tile offset goes into its byte offset
+    const __amdgpu_buffer_rsrc_t nodes_r = buffer_of(nodes);
     const uint32_t n_nodes = live ? T.n_nodes : 0u;
     uint32_t i = 0, nq = 0;
     for (;;) {
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, Develo
             const uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(stepping));
             if (ns == 0u || (it > 0u && ns <= stop)) break;
             if (stepping) {
-                const NodeView v = load_node(nd0, i);
+                const NodeView v = load_node_buf(nodes_r, T.node_base + i);
                 const bool ov = v.lo[0] <= qhi[0] && v.hi[0] >= qlo[0] && v.lo[1] <= qhi[1] &&
                                 v.hi[1] >= qlo[1] && v.lo[2] <= qhi[2] && v.hi[2] >= qlo[2];
                 if (ov && v.prim != kInterior) {
