@@ -530,8 +530,19 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     rng.init(p, S.key0, S.key1);
                     float x, y;
                     if (C.ppp > 0) {
-                        const uint64_t pix = p / C.ppp;
-                        const uint32_t px = (uint32_t)(pix % S.tw), py = (uint32_t)(pix / S.tw);
+                        // a tile's packet ids fit 32 bits (2^32 packets would be 3.4e7 pixels
+                        // at 128 per pixel), where the divisions are a few instructions
+                        // instead of a 64-bit software division each
+                        uint32_t px, py;
+                        if (p < (1ull << 32)) {
+                            const uint32_t pix = (uint32_t)p / C.ppp;
+                            px = pix % S.tw;
+                            py = pix / S.tw;
+                        } else {
+                            const uint64_t pix = p / C.ppp;
+                            px = (uint32_t)(pix % S.tw);
+                            py = (uint32_t)(pix / S.tw);
+                        }
                         const float ux = rng.u(), uy = rng.u();
                         x = ((float)px + ux) * S.pw;
                         y = ((float)py + uy) * S.ph;
