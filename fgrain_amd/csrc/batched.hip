@@ -218,7 +218,7 @@ constexpr uint32_t kEdgeQueue = 8;
 
 __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, DevelopParams P)
 {
-    __shared__ uint32_t lq_all[kEdgeQueue][256];
+    __shared__ uint32_t lq_all[kEdgeQueue][kQueueLanes];
     uint32_t *lq = &lq_all[0][threadIdx.x];
     const uint32_t nd = *B.dev_count;
     if (blockIdx.x * blockDim.x >= nd) return;   // whole block past the list
@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, Develo
                 const bool ov = v.lo[0] <= qhi[0] && v.hi[0] >= qlo[0] && v.lo[1] <= qhi[1] &&
                                 v.hi[1] >= qlo[1] && v.lo[2] <= qhi[2] && v.hi[2] >= qlo[2];
                 if (ov && v.prim != kInterior) {
-                    lq[256u * nq++] = v.prim;
+                    lq[kQueueLanes * nq++] = v.prim;
                     if (nq == kEdgeQueue) lim = 0u;
                 }
                 i = (ov && v.prim == kInterior) ? i + 1 : v.skip;
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) k_edges(Bufs B, const Node *nodes, Develo
 #pragma unroll 1
         for (uint32_t qd = 0; qd < kEdgeQueue; qd++) {
             if (qd >= nq) continue;
-            const uint32_t lf = lq[256u * qd];
+            const uint32_t lf = lq[kQueueLanes * qd];
             const uint32_t first = lf >> 4, cnt = lf & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
                 const uint64_t j = (uint64_t)T.crystal_base + k;
