@@ -212,13 +212,15 @@ def test_autotuner_on_a_small_crop(engine, monkeypatch):
     assert engine.tuning()["transport_chunk"] == 64   # restored
 
 
-def test_full_frame_properties(engine):
-    """The benchmark's full frame (6000 x 4000 px, 247 Tri-X tiles, 3.07e9 photon paths) on
-    the batched path, through properties that do not depend on its size: graph replays
-    are bit-identical, another launch tuning (packet chunk, leaf wait, transport
-    occupancy) deposits the same photons, every latent photon belongs to a counted
-    absorption event, and D_neg stays inside its mapped range."""
-    cfg = W.load_config("frame24mp_trix_cubic")
+@pytest.mark.parametrize("workload", ["frame24mp_trix_cubic", "frame24mp_tmax_tabular"])
+def test_full_frame_properties(engine, workload):
+    """The benchmarks' full frames (6000 x 4000 px, 247 tiles, 3.07e9 photon paths; Tri-X
+    cubic, and T-Max tabular with uneven tile densities and a film base) on the batched
+    path, through properties that do not depend on their size: graph replays are
+    bit-identical, another launch tuning (packet chunk, leaf wait, transport occupancy)
+    deposits the same photons, every latent photon belongs to a counted absorption
+    event, and D_neg stays inside its mapped range."""
+    cfg = W.load_config(workload)
     fr = W.Frame(engine, cfg, W.workload_exposure(cfg))
     old = engine.tuning()
     try:
