@@ -244,3 +244,22 @@ def test_full_frame_properties(engine, workload):
     assert np.isfinite(D1).all()
     # the scanner OTF is a convex 3-tap blend, so it keeps the range up to rounding
     assert D1.min() >= 0.05 - 1e-5 and D1.max() <= 4.05 + 1e-5
+
+
+@pytest.mark.parametrize("size", [(7, 5), (33, 2), (1, 1)])
+def test_tiny_batched_frame_equals_plain(engine, size):
+    """Frames of a few pixels (one partial tile, a handful of crystals, at most a few
+    developed): the batched device path's counters, lists and developed-crystal BVH at
+    their smallest sizes equal the per-tile path."""
+    w, h = size
+    cfg = _cfg(ppp=64, seed=29)
+    fr = W.Frame(engine, cfg, np.full((h, w), 3.0, np.float32), frame=(w, h))
+    sp = fr.render(W.Frame.PLAIN)
+    Dp, cp, (dp, fp, kp) = fr.read(), fr.counts(), fr.development()
+    sb = fr.render(W.Frame.BATCHED)
+    Db, cb, (db, fb, kb) = fr.read(), fr.counts(), fr.development()
+    assert sb["tiles"] == 1 and sb["photons"] == 64 * w * h
+    assert np.array_equal(cp, cb) and np.array_equal(dp, db) and np.array_equal(kp, kb)
+    assert np.array_equal(fp, fb)
+    assert sp["developed"] == sb["developed"] and sp["clumps"] == sb["clumps"]
+    assert np.abs(Dp - Db).max() <= 1e-5
