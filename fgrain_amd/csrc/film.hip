@@ -133,14 +133,86 @@ __global__ void __launch_bounds__(1024) k_bbox_reduce(const int32_t *part, uint3
     }
 }
 
+// The filter passes of one tile; kInterior: the tile and its 3-pixel halo lie inside the
+// gate, so every tap is inside and the gate tests drop out (same operations, same order).
+template <bool kInterior>
+__device__ __forceinline__ void optics_tile(float (*hl)[kOX + 6], float (*tn)[kOX], float (*tw)[kOX],
+                                            int x0, int y0, int top, int bottom, int left, int right,
+                                            uint32_t W, uint32_t H, float *h_eff)
+{
+    constexpr float KN[3] = {0.15f, 0.70f, 0.15f};
+    constexpr float KW[7] = {0.03f, 0.12f, 0.20f, 0.30f, 0.20f, 0.12f, 0.03f};
+    const int t = threadIdx.x;
+    for (int i = t; i < (kPY + 2) * kOX; i += 256) {
+        const int r = i / kOX, c = i % kOX;
+        const int y = y0 - 1 + r, x = x0 + c;
+        float v = 0.0f;
+        if (kInterior || (y >= top && y < bottom && x >= left && x < right)) {
+            float sum = 0.0f, ws = 0.0f;
+            for (int j = 0; j < 3; j++) {
+                const int tap = x + j - 1;
+                if (kInterior || (tap >= left && tap < right)) {
+                    sum += hl[r + 2][c + 2 + j] * KN[j];
+                    ws += KN[j];
+                }
+            }
+            v = sum / fmaxf(ws, 1e-6f);
+        }
+        tn[r][c] = v;
+    }
+    for (int i = t; i < (kPY + 6) * kOX; i += 256) {
+        const int r = i / kOX, c = i % kOX;
+        const int y = y0 - 3 + r, x = x0 + c;
+        float v = 0.0f;
+        if (kInterior || (y >= top && y < bottom && x >= left && x < right)) {
+            float sum = 0.0f, ws = 0.0f;
+            for (int j = 0; j < 7; j++) {
+                const int tap = x + j - 3;
+                if (kInterior || (tap >= left && tap < right)) {
+                    sum += hl[r][c + j] * KW[j];
+                    ws += KW[j];
+                }
+            }
+            v = sum / fmaxf(ws, 1e-6f);
+        }
+        tw[r][c] = v;
+    }
+    __syncthreads();
+    for (int i = t; i < kPY * kOX; i += 256) {
+        const int r = i / kOX, c = i % kOX;
+        const int y = y0 + r, x = x0 + c;
+        if (y >= (int)H || x >= (int)W) continue;
+        float h = 0.0f;
+        if (kInterior || (y >= top && y < bottom && x >= left && x < right)) {
+            float sn = 0.0f, wn = 0.0f;
+            for (int j = 0; j < 3; j++) {
+                const int tap = y + j - 1;
+                if (kInterior || (tap >= top && tap < bottom)) {
+                    sn += tn[r + j][c] * KN[j];
+                    wn += KN[j];
+                }
+            }
+            float sw = 0.0f, ww = 0.0f;
+            for (int j = 0; j < 7; j++) {
+                const int tap = y + j - 3;
+                if (kInterior || (tap >= top && tap < bottom)) {
+                    sw += tw[r + j][c] * KW[j];
+                    ww += KW[j];
+                }
+            }
+            const float on = sn / fmaxf(wn, 1e-6f), ow = sw / fmaxf(ww, 1e-6f);
+            h = 0.90f * on + 0.10f * ow;
+        }
+        h_eff[(size_t)y * W + x] = h;
+    }
+}
+
 // apply_camera_and_gelatin_optics (image_pipeline.rs:155-249): narrow [0.15, 0.70, 0.15]
 // and wide 7-tap separable passes with taps restricted to the gate and renormalised,
 // h = 0.90 narrow + 0.10 wide inside the gate, 0 outside.
 __global__ void __launch_bounds__(256) k_film_optics(const uint8_t *gray, uint32_t W, uint32_t H,
                                                      const int32_t *bbox, float *h_eff)
 {
-    constexpr float KN[3] = {0.15f, 0.70f, 0.15f};
-    constexpr float KW[7] = {0.03f, 0.12f, 0.20f, 0.30f, 0.20f, 0.12f, 0.03f};
     __shared__ float hl[kPY + 6][kOX + 6];
     __shared__ float tn[kPY + 2][kOX];
     __shared__ float tw[kPY + 6][kOX];
@@ -167,68 +239,10 @@ __global__ void __launch_bounds__(256) k_film_optics(const uint8_t *gray, uint32
         if (i < kTile) hl[i / (kOX + 6)][i % (kOX + 6)] = g8[j] == 0xFFFFFFFFu ? 0.0f : lut[g8[j]];
     }
     __syncthreads();
-    for (int i = t; i < (kPY + 2) * kOX; i += 256) {
-        const int r = i / kOX, c = i % kOX;
-        const int y = y0 - 1 + r, x = x0 + c;
-        float v = 0.0f;
-        if (y >= top && y < bottom && x >= left && x < right) {
-            float sum = 0.0f, ws = 0.0f;
-            for (int j = 0; j < 3; j++) {
-                const int tap = x + j - 1;
-                if (tap >= left && tap < right) {
-                    sum += hl[r + 2][c + 2 + j] * KN[j];
-                    ws += KN[j];
-                }
-            }
-            v = sum / fmaxf(ws, 1e-6f);
-        }
-        tn[r][c] = v;
-    }
-    for (int i = t; i < (kPY + 6) * kOX; i += 256) {
-        const int r = i / kOX, c = i % kOX;
-        const int y = y0 - 3 + r, x = x0 + c;
-        float v = 0.0f;
-        if (y >= top && y < bottom && x >= left && x < right) {
-            float sum = 0.0f, ws = 0.0f;
-            for (int j = 0; j < 7; j++) {
-                const int tap = x + j - 3;
-                if (tap >= left && tap < right) {
-                    sum += hl[r][c + j] * KW[j];
-                    ws += KW[j];
-                }
-            }
-            v = sum / fmaxf(ws, 1e-6f);
-        }
-        tw[r][c] = v;
-    }
-    __syncthreads();
-    for (int i = t; i < kPY * kOX; i += 256) {
-        const int r = i / kOX, c = i % kOX;
-        const int y = y0 + r, x = x0 + c;
-        if (y >= (int)H || x >= (int)W) continue;
-        float h = 0.0f;
-        if (y >= top && y < bottom && x >= left && x < right) {
-            float sn = 0.0f, wn = 0.0f;
-            for (int j = 0; j < 3; j++) {
-                const int tap = y + j - 1;
-                if (tap >= top && tap < bottom) {
-                    sn += tn[r + j][c] * KN[j];
-                    wn += KN[j];
-                }
-            }
-            float sw = 0.0f, ww = 0.0f;
-            for (int j = 0; j < 7; j++) {
-                const int tap = y + j - 3;
-                if (tap >= top && tap < bottom) {
-                    sw += tw[r + j][c] * KW[j];
-                    ww += KW[j];
-                }
-            }
-            const float on = sn / fmaxf(wn, 1e-6f), ow = sw / fmaxf(ww, 1e-6f);
-            h = 0.90f * on + 0.10f * ow;
-        }
-        h_eff[(size_t)y * W + x] = h;
-    }
+    if (x0 - 3 >= left && x0 + kOX + 3 <= right && y0 - 3 >= top && y0 + kPY + 3 <= bottom)
+        optics_tile<true>(hl, tn, tw, x0, y0, top, bottom, left, right, W, H, h_eff);
+    else
+        optics_tile<false>(hl, tn, tw, x0, y0, top, bottom, left, right, W, H, h_eff);
 }
 
 __device__ __forceinline__ uint32_t luma_of(uint32_t r8, uint32_t g8, uint32_t b8, float rw, float gw, float bw)
