@@ -825,7 +825,8 @@ __global__ void __launch_bounds__(256) k_leafcount(Bufs B, uint64_t cap)
 }
 
 // threaded position and skip of every node: a right child sits right after its
-// parent, a left child after its right sibling's subtree (2 * leaves - 1 nodes)
+// parent, a left child after its right sibling's subtree (2 * leaves - 1 nodes); a
+// leaf ends its right-child ancestor's subtree (or the tree), so its skip is slot + 1
 __global__ void __launch_bounds__(256) k_layout(Bufs B, uint64_t cap)
 {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

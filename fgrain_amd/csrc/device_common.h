@@ -643,7 +643,9 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                         lq[kQueueLanes * nq++] = nd.prim;
                         if (nq == (uint32_t)kLeafQueue) lim = 0u;
                     }
-                    i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
+                    // a leaf's skip link is its own index + 1 in the threaded layout, so a
+                    // hit node of either kind continues at i + 1
+                    i = hit ? i + 1 : nd.skip;
                 }
             }
         }
@@ -831,7 +833,7 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
                     lq[kQueueLanes * nq++] = nd.prim;
                     if (nq == kScanQueue) lim = 0u;
                 }
-                i = (hit && nd.prim == kInterior) ? i + 1 : nd.skip;
+                i = hit ? i + 1 : nd.skip;   // a leaf's skip link is its index + 1
             }
         }
 #pragma unroll 1

@@ -32,7 +32,9 @@ constexpr uint32_t kTagAttr = 0x41545452u;
 // 32-byte BVH node in threaded ("skip-link") pre-order: a node's right subtree follows
 // it directly and its left subtree follows the right one, so walking "hit -> i + 1,
 // miss or leaf done -> skip" visits nodes in exactly the order of a depth-first
-// traversal that pops the right child first, with no stack.
+// traversal that pops the right child first, with no stack.  A leaf's subtree is the
+// leaf alone, so its skip is its own index + 1 and the walk reduces to "hit -> i + 1,
+// miss -> skip" for both kinds of node (both builders, host and device, keep this).
 struct alignas(32) Node {
     float lo[3];
     float hi[3];

@@ -102,11 +102,8 @@ def _threaded_walk_hits(nodes, recs_idx, cs, o, d):
                 cid = int(recs_idx[k])
                 if O.crystal_intersect(cs, cid, o32, d32, fp32=True) is not None:
                     out.append(cid)
-            i = int(nodes[i, 6])
-        elif hit:
-            i += 1
-        else:
-            i = int(nodes[i, 6])
+        # a hit node of either kind continues at i + 1 (a leaf's skip link is i + 1)
+        i = i + 1 if hit else int(nodes[i, 6])
     return out
 
 
@@ -169,6 +166,7 @@ def test_device_records_and_outward_rounded_boxes():
         assert nodes[slot, 6] == skip
         if b["count"][r] > 0:
             assert nodes[slot, 7] == (b["offset"][r] << 4) | b["count"][r]
+            assert skip == slot + 1   # the kernels' walk relies on this
         else:
             assert nodes[slot, 7] == 0xFFFFFFFF
             rs = slot + 1
