@@ -806,9 +806,10 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
 // crystal_hit_down with them is crystal_hit bit for bit.  The chords are added in
 // depth-first order either
 // way, so queueing changes no bit (frame scans 261 -> 148 ms Tri-X, 417 -> 191 ms
-// T-Max against testing each leaf when it is hit).  Lanes with active = false take part
+// T-Max against testing each leaf when it is hit; queue depths 8 / 12 / 16: 91.8 / 91.3 /
+// 90.3 ms Tri-X, 113.8 / 112.2 / 111.8 ms T-Max).  Lanes with active = false take part
 // in no step.
-constexpr uint32_t kScanQueue = 8;
+constexpr uint32_t kScanQueue = 16;
 
 __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t n_nodes, const float4 *recs,
                                                     const float4 *consts, float ox, float oy, float oz,
