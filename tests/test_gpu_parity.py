@@ -288,6 +288,35 @@ def test_scan_matches_fp64_oracle(engine, ss):
     assert np.abs(D - O.dneg(To)).max() <= 1e-4
 
 
+def test_scan_refills_its_leaf_queue_and_matches_oracle(engine):
+    """A layer 200 deep: every vertical ray crosses more than 16 leaf boxes (the scan's
+    per-lane leaf queue depth), so each ray fills the queue and resumes its walk several
+    times; the result must still equal the oracle's depth-first sum."""
+    cfg = trix_config(24, 24, depth=200.0)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 5)
+    # leaf boxes of the full layer that contain each pixel centre: with every crystal
+    # developed the scan's tree splits the same centroids with boxes at least as large,
+    # so this counts a lower bound of the leaves each ray visits
+    nodes, _ = em.device_layout()
+    f = nodes.view(np.float32)
+    leaf = nodes[:, 7] != 0xFFFFFFFF
+    lo, hi = f[leaf, 0:2], f[leaf, 3:5]
+    xs = np.arange(24) + 0.5
+    per_ray = np.array([[np.sum((lo[:, 0] < x) & (x < hi[:, 0]) & (lo[:, 1] < y) & (y < hi[:, 1]))
+                         for x in xs] for y in xs])
+    assert per_ray.min() > 16 and per_ray.max() > 48
+    em.set_state(counts=np.full(len(em), 50, np.uint32))
+    F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
+    st = em.state()
+    assert st["developed"].all()
+    sc = F.ScanConfig(24, 24, 4, F.ScanMode.NegativeTransmittance, 0.001, 1)
+    T, _ = F.scan_emulsion(em, sc, 7, engine)
+    cs = oracle_crystals(em, st["developed"], st["fe"])
+    To = O.scan(cs, 24, 24, 200, 24, 24, spp=4, ss=1, sigma=0.001, seed=7)
+    assert 0.2 < To.min() and To.max() < 0.8
+    assert np.abs(T - To).max() <= 1e-5
+
+
 def test_scan_otf_matches_oracle(engine):
     cfg = trix_config(40, 40)
     em = F.VirtualEmulsionLayer.synthesize(cfg, 3)
