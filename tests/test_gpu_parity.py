@@ -118,6 +118,38 @@ def test_transport_kernels_and_tunings_give_identical_counts(engine, tuning):
     assert st0["prims_tested"] == st1["prims_tested"]
 
 
+def test_transport_steps_that_overflow_the_leaf_queue(engine):
+    """A 200-deep layer with little scattering (mean free path ~330 px, step cap 300):
+    one step crosses the whole layer and the 40-odd leaf boxes on its way, several times
+    the persistent kernel's per-lane leaf queue, so lanes stop with a full queue, test
+    it and resume their walk.  The one-thread-per-packet kernel tests each leaf when it
+    is hit (no queue): both must deposit exactly the same photons, and agree with the
+    oracle's replay of the same streams."""
+    cfg = trix_config(32, 32, depth=200.0)
+    cfg.optics.scattering_coeff = 0.001
+    emap = np.full((32, 32), 1.0, np.float32)
+    ref = F.VirtualEmulsionLayer.synthesize(cfg, 21)
+    engine.set_tuning()
+    g, st0 = _expose_gpu(engine, ref, cfg, emap, ppp=8, seed=4)
+    em = F.VirtualEmulsionLayer.synthesize(cfg, 21)
+    try:
+        engine.set_tuning(transport_kernel=1)
+        _, st1 = _expose_gpu(engine, em, cfg, emap, ppp=8, seed=4)
+    finally:
+        engine.set_tuning()
+    assert np.array_equal(ref.absorbed_photons(), em.absorbed_photons())
+    assert st0["prims_tested"] == st1["prims_tested"] and st0["abs_total"] == st1["abs_total"]
+    # crystals tested per bounce (about 100 in the oracle's full walk): a leaf holds at
+    # most 4 crystals here, so more than 48 means more leaves than one 12-deep queue
+    assert st0["prims_tested"] > 48 * st0["bounces"]
+    cs = oracle_crystals(ref)
+    oc, ost = O.expose(cs, emap, **expose_kwargs(cfg), seed=4, sampler=1, ppp=8)
+    oc = oc.astype(np.int64)
+    assert ost["abs_total"] > 0
+    assert abs(g.sum() - oc.sum()) <= 1e-3 * oc.sum()
+    assert np.mean(g == oc) >= 0.98
+
+
 def test_transport_replay_with_halation(engine):
     cfg = trix_config(48, 48, depth=16.0, r_ahu=0.9)
     cfg.optics.antihalation_density = 0.05
