@@ -469,7 +469,6 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
 {
     static_assert(kMode == kCounts || !kFrame, "replay runs on one tile");
     const uint32_t lane = threadIdx.x & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t chunk = max(C.chunk, 64u);
     unsigned long long pool = 0;
     uint32_t pool_left = 0;
@@ -507,12 +506,17 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
         // ---- hand out packets to lanes without one
         const unsigned long long nm = __ballot(phase == kNeed);
         if (nm) {
-            const uint32_t n = (uint32_t)__popcll(nm), r = (uint32_t)__popcll(nm & below);
+            // n lanes need a packet; r = how many of them sit below this lane (mbcnt)
+            const uint32_t n = (uint32_t)__popcll(nm);
+            const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(nm >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
             unsigned long long id = pool + r;
             if (n > pool_left) {
                 unsigned long long nb = 0;
                 if (lane == 0) nb = atomicAdd(work, (unsigned long long)chunk);
-                nb = __shfl(nb, 0, 64);
+                // the pool is wave-uniform: keep it in scalar registers
+                nb = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(nb >> 32)) << 32) |
+                     __builtin_amdgcn_readfirstlane((uint32_t)nb);
                 if (r >= pool_left) id = nb + (r - pool_left);
                 pool = nb + (n - pool_left);
                 pool_left = chunk - (n - pool_left);

@@ -73,6 +73,9 @@ FG_HD void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
 #pragma unroll
 #endif
     for (int r = 0; r < 10; r++) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" : "+v"(k0), "+v"(k1));
+#endif
         const uint32_t h0 = mulhi32(0xD2511F53u, c[0]), l0 = 0xD2511F53u * c[0];
         const uint32_t h1 = mulhi32(0xCD9E8D57u, c[2]), l1 = 0xCD9E8D57u * c[2];
         const uint32_t n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
