@@ -51,6 +51,12 @@ class ExposeStatsC(C.Structure):
         ("kernel_ms", C.c_float), ("pad_", C.c_float)]
 
 
+class DeviceTileC(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("records", C.c_void_p), ("counts", C.c_void_p),
+                ("exposure", C.c_void_p), ("n_nodes", C.c_uint32), ("n_crystals", C.c_uint32),
+                ("tw", C.c_uint32), ("th", C.c_uint32)]
+
+
 class DevelopmentConfigC(C.Structure):
     _fields_ = [("threshold_photons", C.c_uint32), ("pad_", C.c_uint32),
                 ("filament_expansion", C.c_double), ("clumping_proximity_factor", C.c_double),
@@ -185,6 +191,8 @@ SIGNATURES = {
     "fg_emulsion_set_state": (_st, [_vp, _u32p, _u8p, _f64p]),
     "fg_expose": (_st, [_vp, _vp, _f32p, C.c_uint32, C.c_uint32, C.POINTER(ExposureConfigC),
                         C.c_uint64, C.POINTER(ExposeStatsC)]),
+    "fg_expose_noalloc": (_st, [_vp, _vp, C.POINTER(DeviceTileC), C.POINTER(ExposureConfigC), C.c_uint64,
+                                _vp, C.POINTER(ExposeStatsC)]),
     "fg_develop": (_st, [_vp, _vp, C.POINTER(DevelopmentConfigC), C.POINTER(DevelopmentReportC)]),
     "fg_scan": (_st, [_vp, _vp, C.POINTER(ScanConfigC), C.c_uint64, _f32p]),
     "fg_traverse_all_hits": (_st, [_vp, _vp, C.c_int64, _f32p, _f32p, C.c_int32, _i32p, _i32p,

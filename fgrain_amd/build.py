@@ -24,7 +24,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-munsafe-fp-atomics"]
 HOST_SOURCES = ["host.cpp", "engine.cpp", "adjoint_host.cpp", "film_host.cpp"]
-DEVICE_SOURCES = ["kernels.hip", "batched.hip", "adjoint.hip", "synth.hip", "film.hip"]
+DEVICE_SOURCES = ["kernels.hip", "batched.hip", "adjoint.hip", "synth.hip", "film.hip", "noalloc.hip"]
 
 
 def _run(cmd):

@@ -59,6 +59,11 @@ extern "C" fg_status fg_engine_create(fg_engine **out)
               std::to_string(prop.multiProcessorCount) + " CUs)";
     E->cus = prop.multiProcessorCount;
     E->persistent_grid = E->cus * transport_persistent_blocks_per_cu();
+    if (noalloc_init(E) != FG_OK) {
+        const std::string m = g_last_error;
+        fg_engine_destroy(E);
+        return fail(FG_ERR_DEVICE_REQUEST, m);
+    }
     *out = E;
     return FG_OK;
 }

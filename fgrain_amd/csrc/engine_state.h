@@ -86,6 +86,8 @@ struct fg_engine {
     DevBuf dose64, dose32, prob, dprob, gprob, gdose, gmap, partials, pgrad, gout, boost, dboost,
         gboost;
     DevBuf synth_cells;        // device synthesis grid
+    // device address of this GPU's scratch block of fg_expose_noalloc (noalloc.hip)
+    void *noalloc_scratch = nullptr;
     // full-frame film compositing; film_mu (taken before mu) serialises the film calls,
     // which hold these buffers across several engine calls
     std::mutex film_mu;
@@ -98,6 +100,9 @@ struct fg_emulsion {
 };
 
 fg_status ensure_device(fg_engine *E, fg::Emulsion &em);
+// Looks up the current device's scratch block of fg_expose_noalloc (noalloc.hip); called
+// once per engine by fg_engine_create with the engine's device current.
+fg_status noalloc_init(fg_engine *E);
 namespace fg {
 // Tables of an overlapped layout (film_host.cpp) uploaded into buf on stream (complete
 // on return); L points into buf.

@@ -244,6 +244,36 @@ fg_status fg_traverse_first_hit(fg_engine *engine, const fg_emulsion *em, int64_
                                 const float *origins, const float *dirs, int32_t *idx,
                                 float *t0, float *t1);
 
+/* Allocation-free transport of one tile.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc).  Same packets, draws and deposits as fg_expose on the
+ * same emulsion, seed and configuration (the latent counts are identical), but every
+ * array is the caller's device memory:
+ *   nodes, records: the emulsion's threaded layout as fg_emulsion_get_device_layout
+ *                   writes it (n_nodes * 32 and n_crystals * 48 bytes);
+ *   counts:         n_crystals u32 latent counts in leaf order (record order), ADDED to;
+ *   exposure:       tw * th f32 row-major map (NULL = uniform 1; required when
+ *                   photons_per_pixel > 0).
+ * em supplies the patch extents, optics and sizes (host data only; its device copy is
+ * not used).  The only other device memory is a fixed FG_NOALLOC_SCRATCH_BYTES block per
+ * GPU (work and statistics counters) that the library holds until the process ends; its
+ * need does not grow with the tile.  Calls that overlap on one GPU share that block and
+ * race.  stream: a hipStream_t, or NULL for the engine's stream.  stats NULL: the call
+ * only enqueues (no host synchronisation, so it can be captured); stats given: it waits
+ * for the kernel and reports its counters and time.  The call allocates nothing. */
+#define FG_NOALLOC_SCRATCH_BYTES 256u
+typedef struct fg_device_tile {
+    const void *nodes;
+    const void *records;
+    uint32_t *counts;
+    const float *exposure;
+    uint32_t n_nodes, n_crystals;
+    uint32_t tw, th;
+} fg_device_tile;
+fg_status fg_expose_noalloc(fg_engine *engine, const fg_emulsion *em, const fg_device_tile *tile,
+                            const fg_exposure_config *cfg, uint64_t seed, void *stream,
+                            fg_expose_stats *stats);
+
 /* ------------------------------------------------------ drop-in tile render ----- */
 /* Fill params the way image_pipeline.rs:455-515 and 924-976 do for a preset
  * ("tri-x", "tri-x-1600", "t-max", "hp5"), a tile of tw x th pixels at grain size

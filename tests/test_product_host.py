@@ -43,6 +43,7 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         "fg_param_grads": _lib.ParamGradsC, "fg_adjoint_stats": _lib.AdjointStatsC,
         "fg_film_options": _lib.FilmOptionsC, "fg_film_plan": _lib.FilmPlanC,
         "fg_film_report": _lib.FilmReportC, "fg_composite_stats": _lib.CompositeStatsC,
+        "fg_device_tile": _lib.DeviceTileC,
     }
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "fgrain_hip.h"\nint main(void){\n' +
