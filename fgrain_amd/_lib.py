@@ -116,7 +116,8 @@ class FrameStatsC(C.Structure):
         "tiles", "crystals", "developed", "clumps", "photons", "rays", "pixels", "bounces",
         "nodes_visited", "prims_tested", "abs_events")] + [
         (n, C.c_float) for n in ("ms_total", "ms_transport", "ms_develop", "ms_scan",
-                                 "ms_kernel_transport", "ms_kernel_scan")]
+                                 "ms_kernel_transport", "ms_kernel_scan")] + [
+        ("graph_used", C.c_uint32), ("pad_", C.c_uint32)]
 
 
 class ExpectedModelC(C.Structure):

@@ -1076,16 +1076,25 @@ hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph)
 {
     if (!use_graph) return launch_sequence(b, s);
     if (!b->exec) {
+        if (b->graph) {
+            (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+        }
         TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         b->capturing = true;
         hipError_t e = launch_sequence(b, s);
         b->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(s, &g);
-        if (e != hipSuccess) return e;
-        if (e2 != hipSuccess) return e2;
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            // nothing of a failed capture is kept: the caller falls back to direct launches
+            if (g) (void)hipGraphDestroy(g);
+            b->exec = nullptr;
+            return e;
+        }
         b->graph = g;
-        TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     }
     return hipGraphLaunch(b->exec, s);
 }

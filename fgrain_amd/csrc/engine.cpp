@@ -1235,6 +1235,7 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
         e = batched_render(F->batched, E->stream, false);
     }
     HIP_TRY(e, FG_ERR_PIPELINE);
+    st->graph_used = F->graph_failed ? 0u : 1u;
     HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
     std::vector<TileDev> tiles;
     uint32_t err[4];

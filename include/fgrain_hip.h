@@ -322,6 +322,9 @@ typedef struct fg_frame_stats {
     float ms_total, ms_transport, ms_develop, ms_scan;
     float ms_kernel_transport;    /* summed transport kernel time (HIP events) */
     float ms_kernel_scan;
+    uint32_t graph_used;          /* 1: the batched render replayed its hipGraph; 0: plain
+                                     mode, or direct launches after a failed capture */
+    uint32_t pad_;
 } fg_frame_stats;
 
 fg_status fg_frame_create(fg_engine *engine, const fg_frame_desc *desc, fg_frame **out);

@@ -78,8 +78,10 @@ def test_batched_graph_replay_is_deterministic(engine, frame):
     cfg, fr = frame
     outs = []
     for _ in range(3):
-        fr.render(W.Frame.BATCHED)
+        st = fr.render(W.Frame.BATCHED)
+        assert st["graph_used"] == 1      # the replayed hipGraph, not direct launches
         outs.append((fr.read(), fr.counts()))
+    assert fr.render(W.Frame.PLAIN)["graph_used"] == 0
     for D, c in outs[1:]:
         assert (D.view(np.uint32) == outs[0][0].view(np.uint32)).all()
         assert np.array_equal(c, outs[0][1])
