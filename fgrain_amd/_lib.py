@@ -48,7 +48,7 @@ class ExposureConfigC(C.Structure):
 class ExposeStatsC(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "launched", "bounces", "nodes_visited", "prims_tested", "abs_events", "abs_total")] + [
-        ("kernel_ms", C.c_float), ("pad_", C.c_float)]
+        ("kernel_ms", C.c_float), ("pad_", C.c_float), ("atomics", C.c_uint64)]
 
 
 class DeviceTileC(C.Structure):
@@ -78,7 +78,7 @@ class ScanConfigC(C.Structure):
 class TuningC(C.Structure):
     _fields_ = [("transport_kernel", C.c_uint32), ("transport_chunk", C.c_uint32),
                 ("transport_wait_pct", C.c_uint32), ("scan_rays_per_thread", C.c_uint32),
-                ("transport_waves", C.c_uint32)]
+                ("transport_waves", C.c_uint32), ("transport_prereduce", C.c_uint32)]
 
 
 class TileParamsC(C.Structure):
@@ -117,7 +117,7 @@ class FrameStatsC(C.Structure):
         "nodes_visited", "prims_tested", "abs_events")] + [
         (n, C.c_float) for n in ("ms_total", "ms_transport", "ms_develop", "ms_scan",
                                  "ms_kernel_transport", "ms_kernel_scan")] + [
-        ("graph_used", C.c_uint32), ("pad_", C.c_uint32)]
+        ("graph_used", C.c_uint32), ("pad_", C.c_uint32), ("atomics", C.c_uint64)]
 
 
 class ExpectedModelC(C.Structure):
@@ -134,7 +134,7 @@ class ParamGradsC(C.Structure):
 
 class AdjointStatsC(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("packets", "bounces", "events")] + [
-        (n, C.c_float) for n in ("ms_replay", "ms_scan", "ms_total", "pad_")]
+        (n, C.c_float) for n in ("ms_replay", "ms_scan", "ms_total", "pad_")] + [("atomics", C.c_uint64)]
 
 
 class FilmOptionsC(C.Structure):

@@ -109,7 +109,7 @@ class VirtualEmulsionLayer:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h and L._lib is not None:
+        if h and L is not None and L._lib is not None:
             L._lib.fg_emulsion_destroy(h)
 
     @classmethod
@@ -286,17 +286,17 @@ class GpuEngine:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h and getattr(self, "_owned", False) and L._lib is not None:
+        if h and getattr(self, "_owned", False) and L is not None and L._lib is not None:
             L._lib.fg_engine_destroy(h)
 
     def adapter_name(self) -> str:
         return L.lib().fg_engine_adapter_name(self._h).decode()
 
     def set_tuning(self, transport_kernel=0, transport_chunk=64, transport_wait_pct=80,
-                   scan_rays_per_thread=0, transport_waves=6):
+                   scan_rays_per_thread=0, transport_waves=6, transport_prereduce=0):
         """Launch parameters (speed only; results do not depend on them)."""
         t = L.TuningC(transport_kernel, transport_chunk, transport_wait_pct, scan_rays_per_thread,
-                      transport_waves)
+                      transport_waves, transport_prereduce)
         check(L.lib().fg_engine_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

@@ -209,6 +209,7 @@ void fill_stats(fg_engine *E, const Setup &S, fg_adjoint_stats *st, float ms_rep
     st->packets = S.r.t.n_photons;
     st->bounces = c[1];
     st->events = c[2];
+    st->atomics = c[3];
     st->ms_replay = ms_replay;
     st->ms_scan = ms_scan;
     st->ms_total = std::chrono::duration<float, std::milli>(Clock::now() - t0).count();

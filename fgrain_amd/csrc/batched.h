@@ -43,14 +43,17 @@ struct BatchedInput {
 };
 
 hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **out);
-// Render the frame; use_graph replays a captured hipGraph (captured on first use).
+// Render the frame; use_graph replays the captured hipGraphs of its three stages
+// (transport; development + developed-crystal BVH; scan), captured on first use.
 hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph);
+// The graph call that failed when batched_render with use_graph returned an error.
+const char *batched_graph_failure(const BatchedFrame *b);
 hipError_t batched_read_tiles(BatchedFrame *b, hipStream_t s, std::vector<TileDev> &tiles,
                               uint32_t *err_flags);
 hipError_t batched_read_develop(BatchedFrame *b, hipStream_t s, std::vector<uint8_t> &dev,
                                 std::vector<double> &fe, std::vector<int32_t> &clump);
 // Kernel times of the last render (HIP events around transport, development + BVH,
-// and scan, recorded on the render stream; inside the graph as event-record nodes).
+// and scan, recorded on the render stream between the stages).
 hipError_t batched_kernel_ms(BatchedFrame *b, float *transport, float *develop, float *scan);
 void batched_destroy(BatchedFrame *b);
 

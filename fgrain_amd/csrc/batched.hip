@@ -155,29 +155,43 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
 // persistent transport over every tile's packets, built for 6 waves per SIMD (80 VGPRs)
 // and for 7 (72 VGPRs, more spills): which is faster depends on the frame (Tri-X frame
 // 1363 / 1285 ms, T-Max frame 3806 / 3877 ms), so the tuning chooses
+template <bool kPre>
 __device__ __forceinline__ void transport_frame(const TransportArgs &C, const Bufs &B, const Node *nodes,
                                                 const float4 *recs, uint32_t *counts, const float *emap,
                                                 uint32_t n_tiles, uint64_t total_photons)
 {
     LaneCounters cnt;
-    transport_persistent<true>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work, total_photons, cnt);
+    transport_persistent<true, kCounts, kPre>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work,
+                                              total_photons, cnt);
     flush_lane_counters(cnt, C.stats);
 }
 
+// kPre: deposits pre-reduced across the wave before the global atomics (the tuning's
+// transport_prereduce)
+template <bool kPre>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) k_transport(TransportArgs C, Bufs B, const Node *nodes,
                                                    const float4 *recs, uint32_t *counts,
                                                    const float *emap, uint32_t n_tiles,
                                                    uint64_t total_photons)
 {
-    transport_frame(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
+    transport_frame<kPre>(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
 }
 
+template <bool kPre>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))) k_transport_w7(TransportArgs C, Bufs B, const Node *nodes,
                                                    const float4 *recs, uint32_t *counts,
                                                    const float *emap, uint32_t n_tiles,
                                                    uint64_t total_photons)
 {
-    transport_frame(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
+    transport_frame<kPre>(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
+}
+
+using TransportKernel = void (*)(TransportArgs, Bufs, const Node *, const float4 *, uint32_t *, const float *,
+                                 uint32_t, uint64_t);
+static TransportKernel transport_kernel_for(uint32_t waves, bool pre)
+{
+    if (waves == 7u) return pre ? k_transport_w7<true> : k_transport_w7<false>;
+    return pre ? k_transport<true> : k_transport<false>;
 }
 
 __global__ void __launch_bounds__(256) k_threshold(Bufs B, const uint32_t *counts, uint64_t n,
@@ -910,10 +924,11 @@ struct BatchedFrame {
     uint64_t total_photons = 0;
     uint32_t total_blocks = 0;
     int transport_grid = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // transport and scan brackets
-    bool capturing = false;
+    // one captured graph per stage (transport; development + developed-crystal BVH; scan)
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // stage brackets
+    const char *graph_step = "";   // the graph call that failed last (batched_graph_failure)
 };
 
 template <class T>
@@ -1022,82 +1037,132 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(hipGetDevice(&dev));
     TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     if (in.transport_waves != 6u && in.transport_waves != 7u) return hipErrorInvalidValue;
-    const void *tk = in.transport_waves == 7u ? reinterpret_cast<const void *>(k_transport_w7)
-                                              : reinterpret_cast<const void *>(k_transport);
+    const void *tk = reinterpret_cast<const void *>(transport_kernel_for(in.transport_waves, in.transport.prereduce != 0u));
     TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tk, 256, 0));
     b->transport_grid = std::max(1, cus * std::max(per_cu, 1));
     for (auto &e : b->ev) TRY(hipEventCreate(&e));
     return hipSuccess;
 }
 
-static hipError_t mark(BatchedFrame *b, int k, hipStream_t s)
-{
-    return hipEventRecordWithFlags(b->ev[k], s, b->capturing ? hipEventRecordExternal : 0u);
-}
+// A failed launch or event record of the sequence names itself in b->graph_step.
+#define TRY_STEP(b, what, x)           \
+    do {                               \
+        hipError_t e_ = (x);           \
+        if (e_ != hipSuccess) {        \
+            (b)->graph_step = (what);  \
+            return e_;                 \
+        }                              \
+    } while (0)
+#define TRY_LAUNCH(b, name) TRY_STEP(b, name, hipGetLastError())
 
-static hipError_t launch_sequence(BatchedFrame *b, hipStream_t s)
+// Stage 0: reset and transport; 1: development and the developed-crystal BVH; 2: scan.
+static hipError_t launch_stage(BatchedFrame *b, int stage, hipStream_t s)
 {
+    // every launch is checked through hipGetLastError: clear what an earlier, unrelated
+    // call of this thread may have left there, or a stale error would read as a failed
+    // launch (and a failed capture)
+    (void)hipGetLastError();
     const BatchedInput &in = b->in;
     Bufs &B = b->B;
     const uint64_t n = (uint64_t)in.n_crystals;
     const uint32_t nb = (uint32_t)((n + 255) / 256), nb2 = (uint32_t)((2 * n + 255) / 256);
-    hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>(nb2, 8192)), dim3(256), 0, s, B, in.counts,
-                       in.stats, n, b->n_tiles);
-    TransportArgs C = in.transport;
-    C.stats = in.stats;
-    C.estride = in.frame_w;
-    TRY(mark(b, 0, s));
-    hipLaunchKernelGGL(in.transport_waves == 7u ? k_transport_w7 : k_transport, dim3(b->transport_grid),
-                       dim3(256), 0, s, C, B, in.nodes, reinterpret_cast<const float4 *>(in.recs),
-                       in.counts, in.emap, b->n_tiles, b->total_photons);
-    TRY(mark(b, 1, s));
-    hipLaunchKernelGGL(k_threshold, dim3(nb), dim3(256), 0, s, B, in.counts, n, in.develop);
-    hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, in.develop);
-    hipLaunchKernelGGL(k_roots, dim3(nb), dim3(256), 0, s, B, n);
-    hipLaunchKernelGGL(k_tile_scans, dim3(b->n_tiles), dim3(256), 0, s, B);
-    hipLaunchKernelGGL(k_boost, dim3(nb), dim3(256), 0, s, B, n, in.develop);
-    hipLaunchKernelGGL(k_root_segments, dim3((b->n_tiles + 255) / 256), dim3(256), 0, s, B, b->n_tiles);
-    for (int level = 0; level < kMaxLevels; level++) {
-        hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(kLevelThreads), 0, s, B, level);
-        hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
-        hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
+    if (stage == 0) {
+        hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>(nb2, 8192)), dim3(256), 0, s, B, in.counts,
+                           in.stats, n, b->n_tiles);
+        TRY_LAUNCH(b, "k_reset");
+        TransportArgs C = in.transport;
+        C.stats = in.stats;
+        C.estride = in.frame_w;
+        hipLaunchKernelGGL(transport_kernel_for(in.transport_waves, C.prereduce != 0u), dim3(b->transport_grid),
+                           dim3(256), 0, s, C, B, in.nodes, reinterpret_cast<const float4 *>(in.recs),
+                           in.counts, in.emap, b->n_tiles, b->total_photons);
+        TRY_LAUNCH(b, "k_transport");
+        return hipSuccess;
+    }
+    if (stage == 1) {
+        hipLaunchKernelGGL(k_threshold, dim3(nb), dim3(256), 0, s, B, in.counts, n, in.develop);
+        TRY_LAUNCH(b, "k_threshold");
+        hipLaunchKernelGGL(k_edges, dim3(nb), dim3(256), 0, s, B, in.nodes, in.develop);
+        TRY_LAUNCH(b, "k_edges");
+        hipLaunchKernelGGL(k_roots, dim3(nb), dim3(256), 0, s, B, n);
+        TRY_LAUNCH(b, "k_roots");
+        hipLaunchKernelGGL(k_tile_scans, dim3(b->n_tiles), dim3(256), 0, s, B);
+        TRY_LAUNCH(b, "k_tile_scans");
+        hipLaunchKernelGGL(k_boost, dim3(nb), dim3(256), 0, s, B, n, in.develop);
+        TRY_LAUNCH(b, "k_boost");
+        hipLaunchKernelGGL(k_root_segments, dim3((b->n_tiles + 255) / 256), dim3(256), 0, s, B, b->n_tiles);
+        TRY_LAUNCH(b, "k_root_segments");
+        for (int level = 0; level < kMaxLevels; level++) {
+            hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(kLevelThreads), 0, s, B, level);
+            TRY_LAUNCH(b, "k_bvh_level");
+            hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
+            TRY_LAUNCH(b, "k_bvh_level_small");
+            hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
+            TRY_LAUNCH(b, "k_bvh_level_tiny");
     }
     hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B);
+    TRY_LAUNCH(b, "k_build_check");
     hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, 2 * n);
+    TRY_LAUNCH(b, "k_leafcount");
     hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, 2 * n);
+    TRY_LAUNCH(b, "k_layout");
     hipLaunchKernelGGL(k_dev_records, dim3(nb), dim3(256), 0, s, B, in.recs, n);
-    TRY(mark(b, 2, s));
+    TRY_LAUNCH(b, "k_dev_records");
+    return hipSuccess;
+    }
     hipLaunchKernelGGL(k_scan, dim3(b->total_blocks), dim3(256), 0, s, B, in.scan, in.dneg, b->n_tiles);
-    TRY(mark(b, 3, s));
-    return hipGetLastError();
+    TRY_LAUNCH(b, "k_scan");
+    return hipSuccess;
 }
 
-hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph)
+// Stage k of the render as a graph, captured on first use; a failed capture keeps
+// nothing (the caller falls back to direct launches).
+static hipError_t stage_graph(BatchedFrame *b, int k, hipStream_t s)
 {
-    if (!use_graph) return launch_sequence(b, s);
-    if (!b->exec) {
-        if (b->graph) {
-            (void)hipGraphDestroy(b->graph);
-            b->graph = nullptr;
+    if (!b->exec[k]) {
+        if (b->graph[k]) {
+            (void)hipGraphDestroy(b->graph[k]);
+            b->graph[k] = nullptr;
         }
+        b->graph_step = "hipStreamBeginCapture";
         TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        b->capturing = true;
-        hipError_t e = launch_sequence(b, s);
-        b->capturing = false;
+        hipError_t e = launch_stage(b, k, s);   // names its failing launch in graph_step
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(s, &g);
-        if (e == hipSuccess) e = e2;
-        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (e == hipSuccess && e2 != hipSuccess) {
+            e = e2;
+            b->graph_step = "hipStreamEndCapture";
+        }
+        if (e == hipSuccess) {
+            e = hipGraphInstantiate(&b->exec[k], g, nullptr, nullptr, 0);
+            if (e != hipSuccess) b->graph_step = "hipGraphInstantiate";
+        }
         if (e != hipSuccess) {
-            // nothing of a failed capture is kept: the caller falls back to direct launches
             if (g) (void)hipGraphDestroy(g);
-            b->exec = nullptr;
+            b->exec[k] = nullptr;
             return e;
         }
-        b->graph = g;
+        b->graph[k] = g;
     }
-    return hipGraphLaunch(b->exec, s);
+    b->graph_step = "hipGraphLaunch";
+    return hipGraphLaunch(b->exec[k], s);
 }
+
+// The three stages bracketed by the timing events, which are recorded on the stream
+// between the stages rather than inside a graph: event-record nodes of captured graphs
+// are not available with every HIP runtime a process may load (the one bundled with
+// PyTorch rejects them).
+hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph)
+{
+    TRY(hipEventRecord(b->ev[0], s));
+    for (int k = 0; k < 3; k++) {
+        TRY(use_graph ? stage_graph(b, k, s) : launch_stage(b, k, s));
+        TRY(hipEventRecord(b->ev[k + 1], s));
+    }
+    return hipSuccess;
+}
+
+const char *batched_graph_failure(const BatchedFrame *b) { return b ? b->graph_step : ""; }
 
 hipError_t batched_read_tiles(BatchedFrame *b, hipStream_t s, std::vector<TileDev> &tiles, uint32_t *err)
 {
@@ -1136,8 +1201,10 @@ void batched_destroy(BatchedFrame *b)
     if (!b) return;
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int k = 0; k < 3; k++) {
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    }
     for (void *p : b->allocs) (void)hipFree(p);
     delete b;
 }

@@ -172,23 +172,69 @@ struct PhotonCounters {
 // (popcounts of ballots, wave-uniform, so they live in scalar registers), crystal tests,
 // absorbing hits and latent photons per lane in 32 bits (a lane handles far fewer than
 // 2^32 of each).
+// atomics: global atomic additions of deposits the lane issued (after the wave
+// pre-reduction).
 struct LaneCounters {
     unsigned long long bounces = 0, nodes = 0;
-    uint32_t prims = 0, abs = 0, tot = 0;
+    uint32_t prims = 0, abs = 0, tot = 0, atomics = 0;
 };
 
 __device__ __forceinline__ void flush_lane_counters(const LaneCounters &c, unsigned long long *stats)
 {
     const unsigned long long pr = wave_sum((unsigned long long)c.prims),
                              ab = wave_sum((unsigned long long)c.abs),
-                             t = wave_sum((unsigned long long)c.tot);
+                             t = wave_sum((unsigned long long)c.tot),
+                             at = wave_sum((unsigned long long)c.atomics);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&stats[0], c.bounces);
         atomicAdd(&stats[1], c.nodes);
         atomicAdd(&stats[2], pr);
         atomicAdd(&stats[3], ab);
         atomicAdd(&stats[4], t);
+        atomicAdd(&stats[5], at);
     }
+}
+
+// Wave pre-reduction of deposits before the global atomics (record_absorption,
+// crystal.rs:221-223, is a relaxed fetch_add per absorbing hit).  The lanes of a wave
+// that deposit in the same step mostly follow neighbouring packets of one pixel, so they
+// often deposit into the same crystal.  Groups are formed one crystal at a time: the
+// lowest pending lane names a slot, a ballot finds every pending lane with that slot;
+// a group of one lane issues its own atomic, a larger group adds its values into the
+// wave's LDS word and the naming lane moves the sum to HBM with one atomic.  Integer
+// sums do not depend on their order, so the counts are bit-identical to one atomic per
+// lane.  Called by the active lanes of a wave (any subset); acc is the wave's LDS word
+// (zero on entry and on return); returns the global atomics this lane issued.
+template <class T>
+__device__ __forceinline__ uint32_t wave_aggregated_add(T *dst, uint32_t slot, T val, bool want, T *acc)
+{
+    const uint64_t pend = __builtin_amdgcn_ballot_w64(want);
+    if (pend == 0ull) return 0u;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pend);
+    const uint32_t key = __builtin_amdgcn_readlane(slot, leader);
+    const bool mine = want && slot == key;
+    const uint64_t grp = __builtin_amdgcn_ballot_w64(mine);
+    uint32_t issued = 0;
+    if (want && !mine) {
+        atomicAdd(&dst[slot], val);
+        issued = 1u;
+    }
+    if ((grp & (grp - 1)) == 0) {
+        if (mine) {
+            atomicAdd(&dst[key], val);
+            issued = 1u;
+        }
+    } else {
+        if (mine) atomicAdd(acc, val);
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+        if (lane == leader) {
+            atomicAdd(&dst[key], atomicExch(acc, (T)0));
+            issued = 1u;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    return issued;
 }
 
 __device__ __forceinline__ void flush_counters(PhotonCounters &c, unsigned long long *stats)
@@ -458,7 +504,7 @@ constexpr uint32_t kQueueLanes = 256;
 // (single tile only), with R the replay outputs and sums the lane's parameter sums.
 enum : int { kCounts = 0, kReplayDose = 1, kReplayGrad = 2 };
 
-template <bool kFrame, int kMode = kCounts>
+template <bool kFrame, int kMode = kCounts, bool kPre = false>
 __device__ __forceinline__ void transport_persistent(const TransportArgs &C, const TileDev *tiles,
                                                      uint32_t n_tiles, const Node *nodes,
                                                      const float4 *recs, uint32_t *counts,
@@ -487,6 +533,16 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     uint32_t b = 0, i = 0, nq = 0;   // nq: leaves queued in this lane's column of lq
     __shared__ uint32_t lq_all[kLeafQueue][kQueueLanes];
     uint32_t *lq = &lq_all[0][threadIdx.x];
+    // per-wave words of the deposit pre-reduction (wave_aggregated_add)
+    __shared__ uint32_t agg_u32[kQueueLanes / 64];
+    __shared__ double agg_f64[kQueueLanes / 64];
+    if constexpr (kPre) {
+        if ((threadIdx.x & 63) == 0) {
+            agg_u32[threadIdx.x >> 6] = 0u;
+            agg_f64[threadIdx.x >> 6] = 0.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
     // replay: exposure of the packet's pixel, relative weight tangents, HG score and the
     // packet's gradient sums (flushed when the lane takes its next packet)
     float E = 0.f, w_mu = 0.f, w_r = 0.f, score = 0.f, a_e = 0.f, a_mu = 0.f, a_g = 0.f, a_r = 0.f;
@@ -675,8 +731,13 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     const uint32_t whole = fl > 0.0f ? (uint32_t)fl : 0u;
                     const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
                     const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
-                    if (k_lat > 0u) {
+                    if constexpr (kPre) {
+                        cnt.atomics += wave_aggregated_add(counts_b, T.crystal_base + k, k_lat, k_lat > 0u,
+                                                           &agg_u32[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+                    } else if (k_lat > 0u) {
                         atomicAdd(&counts_b[T.crystal_base + k], k_lat);
+                    }
+                    if (k_lat > 0u) {
                         cnt.abs++;
                         cnt.tot += k_lat;
                     }
@@ -691,7 +752,13 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                         a_r += base * E * (w_r * pa);
                         a_g += base * E * dep * score;
                     } else {
-                        atomicAdd(&R->dose[k], (double)(((E * w) * pa) * r.sens * 5.0f));
+                        const double dep = (double)(((E * w) * pa) * r.sens * 5.0f);
+                        if constexpr (kPre) {
+                            cnt.atomics += wave_aggregated_add(R->dose, k, dep, true,
+                                                               &agg_f64[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+                        } else {
+                            atomicAdd(&R->dose[k], dep);
+                        }
                     }
                     const float m = 1.0f - pa * 0.9f;
                     const float dm = -0.9f * L * (1.0f - pa);
@@ -794,6 +861,8 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             }
         }
     }
+    // without the pre-reduction every absorbing hit (every event of a dose replay) is one atomic
+    if constexpr (!kPre) cnt.atomics = cnt.abs;
 }
 
 // Path length through developed silver of the ray at (ox, oy) along -z: for a ray

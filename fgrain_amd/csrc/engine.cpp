@@ -2,6 +2,7 @@
 // stages, the drop-in tile render and device-resident frames.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
@@ -93,16 +94,18 @@ extern "C" fg_status fg_engine_set_tuning(fg_engine *E, const fg_tuning *t)
 {
     if (!E || !t) return fail(FG_ERR_INVALID_ARG, "fg_engine_set_tuning: null argument");
     if (t->transport_kernel > 1 || t->transport_wait_pct > 100 || t->scan_rays_per_thread > 64 ||
-        (t->transport_waves != 0 && t->transport_waves != 6 && t->transport_waves != 7))
+        (t->transport_waves != 0 && t->transport_waves != 6 && t->transport_waves != 7) ||
+        t->transport_prereduce > 1)
         return fail(FG_ERR_INVALID_ARG,
                     "fg_engine_set_tuning: transport_kernel must be 0 or 1, wait_pct <= 100, "
-                    "scan_rays_per_thread <= 64, transport_waves 0, 6 or 7");
+                    "scan_rays_per_thread <= 64, transport_waves 0, 6 or 7, transport_prereduce 0 or 1");
     std::lock_guard<std::mutex> lk(E->mu);
     E->tuning.transport_kernel = t->transport_kernel;
     E->tuning.transport_chunk = t->transport_chunk ? std::max(t->transport_chunk, 64u) : 64u;
     E->tuning.transport_wait_pct = t->transport_wait_pct ? t->transport_wait_pct : 80u;
     E->tuning.scan_rays_per_thread = t->scan_rays_per_thread;
     E->tuning.transport_waves = t->transport_waves ? t->transport_waves : 6u;
+    E->tuning.transport_prereduce = t->transport_prereduce;
     return FG_OK;
 }
 
@@ -414,6 +417,7 @@ void fill_transport(TransportArgs &a, const fg_emulsion_config &ec, const fg_exp
 {
     a.chunk = tn.transport_chunk;
     a.wait_pct = tn.transport_wait_pct;
+    a.prereduce = tn.transport_prereduce;
     a.base = (float)x.base_thickness;
     a.reach = g_full_traversal ? -1.0f : (float)reach * 1.0001f;
     const fg_optics &o = ec.optics;
@@ -495,6 +499,7 @@ static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, ui
         stats->prims_tested = st[2];
         stats->abs_events = st[3];
         stats->abs_total = st[4];
+        stats->atomics = st[5];
         float ms = 0;
         (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
         stats->kernel_ms = ms;
@@ -1229,8 +1234,12 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
     HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
     hipError_t e = batched_render(F->batched, E->stream, !F->graph_failed);
     if (e != hipSuccess && !F->graph_failed) {
-        // graph capture unavailable: launch the same sequence directly from now on
+        // graph capture unavailable: launch the same sequence directly from now on, and say
+        // so once (fg_frame_stats.graph_used reports it per render)
         F->graph_failed = true;
+        fprintf(stderr, "fgrain_hip: hipGraph of the batched frame unavailable (%s: %s); "
+                        "launching its kernels directly\n",
+                batched_graph_failure(F->batched), hipGetErrorString(e));
         (void)hipGetLastError();
         e = batched_render(F->batched, E->stream, false);
     }
@@ -1250,6 +1259,7 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
     st->nodes_visited = cnt[1];
     st->prims_tested = cnt[2];
     st->abs_events = cnt[3];
+    st->atomics = cnt[5];
     const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
     const uint64_t g = std::max((uint32_t)llround(sqrt((double)spp)), 1u), ss = std::max(D.scan.supersample, 1u);
     for (const TileDev &t : tiles) {

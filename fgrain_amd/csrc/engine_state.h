@@ -75,7 +75,7 @@ struct fg_engine {
     int cus = 0;
     int persistent_grid = 0;
     int replay_grid = 0;       // resident blocks of the persistent replay kernels
-    fg_tuning tuning{0u, 64u, 80u, 0u, 6u};
+    fg_tuning tuning{0u, 64u, 80u, 0u, 6u, 0u};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string name;

@@ -561,6 +561,7 @@ size_t film_bbox_size(uint32_t W, uint32_t H) { return 4u + 4u * (size_t)(((uint
 hipError_t launch_film_optics(const uint8_t *gray, uint32_t W, uint32_t H, int32_t *bbox, float *h_eff,
                               hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     const uint64_t n = (uint64_t)W * H;
     if (n == 0) return hipSuccess;
     if (n > 0x80000000ull) return hipErrorInvalidValue;
@@ -574,6 +575,7 @@ hipError_t launch_film_optics(const uint8_t *gray, uint32_t W, uint32_t H, int32
 hipError_t launch_film_luma(const uint8_t *rgb, uint64_t n, float rw, float gw, float bw, uint8_t *gray,
                             hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (n == 0) return hipSuccess;
     const uint64_t quads = (n + 3) / 4;
     hipLaunchKernelGGL(k_film_luma, dim3((uint32_t)((quads + 255) / 256)), dim3(256), 0, s, rgb, n, rw, gw, bw, gray);
@@ -582,6 +584,7 @@ hipError_t launch_film_luma(const uint8_t *rgb, uint64_t n, float rw, float gw, 
 
 hipError_t launch_film_blend(const FilmLayout &L, const float *tiles, float *d_norm, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (!L.W || !L.H) return hipSuccess;
     hipLaunchKernelGGL(k_film_blend, dim3(cdiv(L.W, 256), L.H), dim3(256), 0, s, L, tiles, d_norm);
     return hipGetLastError();
@@ -594,6 +597,7 @@ hipError_t launch_film_composite(const FilmLayout &L, const FilmParams &P, const
                                  float *d_high, double *partials, float *fstats, float *grain_out,
                                  uint8_t *out, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (!L.W || !L.H) return hipSuccess;
     if (P.rad > (uint32_t)kMaxRad) return hipErrorInvalidValue;
     const int rad = (int)P.rad;

@@ -37,7 +37,9 @@ struct TransportArgs {
     // reflected packet re-enters displaced by base * (tan of the way down + tan of the
     // way up), and is lost when that leaves the patch
     float base;
-    float pad2;
+    // 1: deposits are pre-reduced across the wave before the global atomics
+    // (wave_aggregated_add); 0: one atomic per absorbing hit.  Same counts either way.
+    uint32_t prereduce;
 };
 
 // Path replay of the expected model (adjoint.hip): the transport's packets without the

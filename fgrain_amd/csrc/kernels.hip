@@ -26,11 +26,12 @@ __global__ void __launch_bounds__(256) transport_kernel(TransportArgs A)
     if (A.stats) flush_counters(cnt, A.stats);
 }
 
-// Persistent transport of one tile.
+// Persistent transport of one tile (kPre: deposits pre-reduced across the wave).
+template <bool kPre>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) transport_persistent_kernel(TransportArgs A, unsigned long long *work)
 {
     LaneCounters cnt;
-    transport_persistent<false>(A, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work, A.n_photons, cnt);
+    transport_persistent<false, kCounts, kPre>(A, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work, A.n_photons, cnt);
     if (A.stats) flush_lane_counters(cnt, A.stats);
 }
 
@@ -176,6 +177,7 @@ __global__ void __launch_bounds__(256) otf3_kernel(const float *in, float *out, 
 
 hipError_t launch_transport(const TransportArgs &a, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (a.n_photons == 0) return hipSuccess;
     const uint64_t blocks = (a.n_photons + 255) / 256;
     hipLaunchKernelGGL(transport_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, a);
@@ -184,6 +186,7 @@ hipError_t launch_transport(const TransportArgs &a, hipStream_t s)
 
 hipError_t launch_down_constants(const float4 *recs, uint32_t n, float4 *consts, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(down_constants_kernel, dim3((n + 255) / 256), dim3(256), 0, s, recs, n, consts);
     return hipGetLastError();
@@ -192,15 +195,19 @@ hipError_t launch_down_constants(const float4 *recs, uint32_t n, float4 *consts,
 hipError_t launch_transport_persistent(const TransportArgs &a, unsigned long long *work, int grid,
                                        hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (a.n_photons == 0) return hipSuccess;
-    hipLaunchKernelGGL(transport_persistent_kernel, dim3(grid), dim3(256), 0, s, a, work);
+    if (a.prereduce)
+        hipLaunchKernelGGL(transport_persistent_kernel<true>, dim3(grid), dim3(256), 0, s, a, work);
+    else
+        hipLaunchKernelGGL(transport_persistent_kernel<false>, dim3(grid), dim3(256), 0, s, a, work);
     return hipGetLastError();
 }
 
 int transport_persistent_blocks_per_cu()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(transport_persistent_kernel),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(transport_persistent_kernel<false>),
                                                      256, 0) != hipSuccess)
         return 1;
     return n > 0 ? n : 1;
@@ -208,6 +215,7 @@ int transport_persistent_blocks_per_cu()
 
 hipError_t launch_scan(const ScanArgs &a, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (a.w_px == 0 || a.h_px == 0) return hipSuccess;
     const uint32_t ppb = 256u / scan_lanes(a.g * a.g * a.ss * a.ss, a.rays_per_lane);
     const uint32_t blocks = (uint32_t)(((uint64_t)a.w_px * a.h_px + ppb - 1) / ppb);
@@ -219,6 +227,7 @@ hipError_t launch_all_hits(const Node *nodes, uint32_t n_nodes, const float4 *re
                            const float *o, const float *d, int64_t n_rays, int32_t max_hits,
                            int32_t *n_hits, int32_t *slot, float *t0, float *t1, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (n_rays == 0) return hipSuccess;
     hipLaunchKernelGGL(all_hits_kernel, dim3((uint32_t)((n_rays + 255) / 256)), dim3(256), 0, s,
                        nodes, n_nodes, recs, o, d, n_rays, max_hits, n_hits, slot, t0, t1);
@@ -229,6 +238,7 @@ hipError_t launch_first_hit(const Node *nodes, uint32_t n_nodes, const float4 *r
                             const float *o, const float *d, int64_t n_rays, int32_t *slot,
                             float *t0, float *t1, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (n_rays == 0) return hipSuccess;
     hipLaunchKernelGGL(first_hit_kernel, dim3((uint32_t)((n_rays + 255) / 256)), dim3(256), 0, s,
                        nodes, n_nodes, recs, o, d, n_rays, slot, t0, t1);
@@ -237,6 +247,7 @@ hipError_t launch_first_hit(const Node *nodes, uint32_t n_nodes, const float4 *r
 
 hipError_t launch_otf3(const float *in, float *out, uint32_t w, uint32_t h, hipStream_t s)
 {
+    (void)hipGetLastError();  // report only this launcher's launches
     if (w == 0 || h == 0) return hipSuccess;
     hipLaunchKernelGGL(otf3_kernel, dim3((w + 15) / 16, (h + 15) / 16), dim3(256), 0, s, in, out, w, h);
     return hipGetLastError();

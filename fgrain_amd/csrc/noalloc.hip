@@ -87,6 +87,7 @@ extern "C" fg_status fg_expose_noalloc(fg_engine *E, const fg_emulsion *em_h, co
     a.counts = t->counts;
     a.stats = sc;
 
+    (void)hipGetLastError();  // the check below reports this launch only
     hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, s, sc, kScratchWords);
     HIP_TRY(hipGetLastError(), FG_ERR_PIPELINE);
     if (stats) HIP_TRY(hipEventRecord(E->ev0, s), FG_ERR_CHANNEL);
@@ -106,6 +107,7 @@ extern "C" fg_status fg_expose_noalloc(fg_engine *E, const fg_emulsion *em_h, co
     stats->prims_tested = st[2];
     stats->abs_events = st[3];
     stats->abs_total = st[4];
+    stats->atomics = st[5];
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
     stats->kernel_ms = ms;

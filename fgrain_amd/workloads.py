@@ -185,7 +185,7 @@ class Frame:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h and L._lib is not None:
+        if h and L is not None and L._lib is not None:
             L._lib.fg_frame_destroy(h)
 
     @property

@@ -88,6 +88,8 @@ typedef struct fg_expose_stats {
     uint64_t abs_total;     /* sum of deposited latent photons */
     float kernel_ms;        /* transport kernel time (HIP events) */
     float pad_;
+    uint64_t atomics;       /* global atomic additions issued for the deposits after the
+                               wave pre-reduction (abs_events without it) */
 } fg_expose_stats;
 
 /* DevelopmentConfig (src/development.rs:12-38) */
@@ -179,6 +181,12 @@ typedef struct fg_tuning {
                                       r > 1 gives a pixel rays / r adjacent lanes (<= 64) */
     uint32_t transport_waves;    /* waves per SIMD of the batched frame transport: 6 (80
                                     VGPRs) or 7 (72 VGPRs, more spills); 0 means 6 */
+    uint32_t transport_prereduce;/* 1: latent-photon deposits of the persistent transport
+                                    are pre-reduced across each wave before the global
+                                    atomics (39 % fewer atomics on the Tri-X frame, 4-6 %
+                                    more transport time); 0 (default): one atomic per
+                                    absorbing hit.  Same counts.  The dose replay of the
+                                    adjoint always pre-reduces its f64 deposits. */
 } fg_tuning;
 fg_status fg_engine_set_tuning(fg_engine *engine, const fg_tuning *tuning);
 fg_status fg_engine_get_tuning(const fg_engine *engine, fg_tuning *tuning);
@@ -290,7 +298,8 @@ fg_status fg_render_tile_all_gpu(fg_engine *engine, const float *tile_img,
 
 /* ------------------------------------------------------------- frames ---------- */
 #define FG_FRAME_PLAIN 0u   /* one launch set + synchronisation per tile */
-#define FG_FRAME_BATCHED 1u /* persistent work-queue kernels over all tiles, one graph */
+#define FG_FRAME_BATCHED 1u /* persistent work-queue kernels over all tiles, replayed as
+                               captured hipGraphs (transport; development; scan) */
 
 /* A frame of width x height pixels cut into tile x tile tiles (edge tiles partial),
  * one emulsion per tile (seed per image_pipeline.rs:430-434), pixel units scaled by
@@ -325,6 +334,7 @@ typedef struct fg_frame_stats {
     uint32_t graph_used;          /* 1: the batched render replayed its hipGraph; 0: plain
                                      mode, or direct launches after a failed capture */
     uint32_t pad_;
+    uint64_t atomics;             /* global atomic additions of the deposits (batched) */
 } fg_frame_stats;
 
 fg_status fg_frame_create(fg_engine *engine, const fg_frame_desc *desc, fg_frame **out);
@@ -389,6 +399,8 @@ typedef struct fg_adjoint_stats {
     float ms_scan;      /* expected scan or its adjoint */
     float ms_total;     /* the whole call, host included */
     float pad_;
+    uint64_t atomics;   /* global f64 atomic additions of the dose pass after the wave
+                           pre-reduction (events without it); 0 for the gradient pass */
 } fg_adjoint_stats;
 
 /* Expected-model forward on one tile: dose and prob per crystal (crystal order, each

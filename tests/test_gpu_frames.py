@@ -108,6 +108,44 @@ def test_batched_transport_occupancy_gives_identical_frames(engine, frame):
     assert (D6.view(np.uint32) == D7.view(np.uint32)).all()
 
 
+def test_deposit_prereduction_gives_identical_counts_with_fewer_atomics(engine, frame):
+    """The wave pre-reduction of deposits (record_absorption, crystal.rs:221-223, as one
+    atomic per group of lanes depositing into the same crystal) gives the same latent
+    counts and frame bit for bit, in both occupancy builds of the frame transport and in
+    the per-tile transport, with fewer global atomics than absorbing hits."""
+    cfg, fr = frame
+    old = engine.tuning()
+    outs = {}
+    try:
+        for waves in (6, 7):
+            for pre in (0, 1):
+                engine.set_tuning(**dict(old, transport_waves=waves, transport_prereduce=pre))
+                st = fr.render(W.Frame.BATCHED)
+                outs[waves, pre] = (fr.read(), fr.counts(), st["abs_events"], st["atomics"])
+        with pytest.raises(GpuError, match="transport_prereduce"):
+            engine.set_tuning(**dict(old, transport_prereduce=2))
+        x0, y0, tw, th, seed = fr.tile_ranges()[0]
+        tile_counts = []
+        for pre in (0, 1):
+            engine.set_tuning(**dict(old, transport_prereduce=pre))
+            em = F.VirtualEmulsionLayer.synthesize(W.emulsion_config(cfg, float(tw), float(th)), seed)
+            st = F.expose_emulsion(em, np.ascontiguousarray(fr.exposure[y0:y0 + th, x0:x0 + tw]),
+                                   W.exposure_config(cfg), seed + 1, engine)
+            tile_counts.append((em.absorbed_photons(), st["abs_events"], st["atomics"]))
+    finally:
+        engine.set_tuning(**old)
+    D0, c0, ev0, at0 = outs[6, 0]
+    assert at0 == ev0 > 0
+    for key, (D, c, ev, at) in outs.items():
+        assert np.array_equal(c, c0), key
+        assert (D.view(np.uint32) == D0.view(np.uint32)).all(), key
+        assert ev == ev0
+        if key[1] == 1:
+            assert 0 < at < ev, key
+    (a0, e0, t0), (a1, e1, t1) = tile_counts
+    assert np.array_equal(a0, a1) and e0 == e1 and t0 == e0 and 0 < t1 < e1
+
+
 def test_batched_development_matches_oracle(engine):
     cfg = _cfg(ppp=2, seed=11)
     w, h = 400, 200
