@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -33,51 +35,91 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, exposure, tile_px=320):
-    """The oracle (C restatement of the reference's Rayon path, chunks = 8 * threads)
-    timed on this host's cores over tile (0, 0) of the frame: synthesis excluded,
-    transport + development + scan included."""
-    import numpy as np
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def sample_tiles(ranges, nx, ny, n=24):
+    """A stratified sample of n tiles: four edge tiles (partial in this frame: top of the
+    right column, middle of the right column, middle of the bottom row, the corner), the
+    rest evenly spaced over the other tiles in raster order."""
+    edge = sorted({nx - 1, (ny // 2) * nx + nx - 1, (ny - 1) * nx + nx // 2, ny * nx - 1})
+    rest = [i for i in range(len(ranges)) if i not in edge]
+    picks = [rest[int(round(k))] for k in np.linspace(0, len(rest) - 1, n - len(edge))]
+    return sorted(set(edge + picks))
+
+
+def cpu_baseline(cfg, exposure, frame, n_tiles=24, ppp_sample=32):
+    """The oracle (C restatement of the reference's Rayon path: chunks = 8 * threads,
+    row-parallel scan) timed on this host's cores over a stratified sample of the frame's
+    tiles (edge tiles included), each at ppp_sample of the workload's photons per pixel:
+    transport, development and scan timed per tile, synthesis excluded.  Projected to the
+    frame by packets: transport time x (ppp / ppp_sample) + development + scan, summed over
+    the sample and scaled by tiles / sampled tiles."""
     from oracle import oracle as O
 
     e, o, x = cfg["emulsion"], cfg["emulsion"]["optics"], cfg["exposure"]
+    d, sc = cfg["development"], cfg["scan"]
     threads = os.cpu_count() or 1
-    cs = O.synthesize(W=tile_px, H=tile_px, D=e["depth"], r_min=e["r_min"],
-                      mean_radius=e["mean_crystal_radius"], size_sigma=e["size_variance"],
-                      morph=e["morphology"], aspect=e["tabular_aspect_ratio"],
-                      base_sens=e["base_sensitivity"], sens_sigma=e["sensitivity_sigma"],
-                      min_radius=e["min_radius"], min_half_thickness=e["min_half_thickness"],
-                      seed=cfg["seed"])
-    emap = np.ascontiguousarray(exposure[:tile_px, :tile_px])
-    t0 = time.perf_counter()
-    counts, st = O.expose(cs, emap, W=tile_px, H=tile_px, D=e["depth"], mu_a=o["absorption_coeff"],
-                          mu_s=o["scattering_coeff"], g=o["anisotropy_g"], n_refr=o["refractive_index"],
-                          r_ahu=o["antihalation_reflectance"], od_ahu=o["antihalation_density"],
-                          mu_abs=x["crystal_absorption_coeff"], theta_max=x["illumination_angle_rad"],
-                          max_bounces=x["max_bounces"], seed=cfg["seed"] + 1, sampler=0,
-                          photons_per_area=float(x["photons_per_pixel"]), threads=threads)
-    t1 = time.perf_counter()
-    d = cfg["development"]
-    dev, fe, _, _ = O.develop(cs, counts, d["threshold_photons"], d["filament_expansion"],
-                              d["clumping_proximity_factor"])
-    cs.developed, cs.fe = dev, fe
-    t2 = time.perf_counter()
-    s = cfg["scan"]
-    O.scan(cs, tile_px, tile_px, e["depth"], tile_px, tile_px, spp=s["samples_per_pixel"],
-           ss=s["supersample"], sigma=s["silver_opacity"], seed=cfg["seed"] + 2, threads=threads)
-    t3 = time.perf_counter()
-    photons = st["launched"]
+    ranges = frame.tile_ranges()
+    nx, ny = -(-frame.width // frame.tile), -(-frame.height // frame.tile)
+    picks = sample_tiles(ranges, nx, ny, n_tiles)
+    ppp = x["photons_per_pixel"]
+    t_tr = t_dev = t_scan = 0.0
+    paths = 0
+    n_edge = 0
+    for k in picks:
+        x0, y0, tw, th, seed = ranges[k]
+        n_edge += int(tw < frame.tile or th < frame.tile)
+        cs = O.synthesize(W=tw, H=th, D=e["depth"], r_min=e["r_min"], mean_radius=e["mean_crystal_radius"],
+                          size_sigma=e["size_variance"], morph=e["morphology"], aspect=e["tabular_aspect_ratio"],
+                          base_sens=e["base_sensitivity"], sens_sigma=e["sensitivity_sigma"],
+                          min_radius=e["min_radius"], min_half_thickness=e["min_half_thickness"], seed=seed)
+        emap = np.ascontiguousarray(exposure[y0:y0 + th, x0:x0 + tw])
+        t0 = time.perf_counter()
+        counts, st = O.expose(cs, emap, W=tw, H=th, D=e["depth"], mu_a=o["absorption_coeff"],
+                              mu_s=o["scattering_coeff"], g=o["anisotropy_g"], n_refr=o["refractive_index"],
+                              r_ahu=o["antihalation_reflectance"], od_ahu=o["antihalation_density"],
+                              mu_abs=x["crystal_absorption_coeff"], theta_max=x["illumination_angle_rad"],
+                              max_bounces=x["max_bounces"], seed=seed + 1, sampler=0,
+                              photons_per_area=float(ppp_sample), threads=threads,
+                              base_thickness=x.get("base_thickness", 0.0))
+        t1 = time.perf_counter()
+        dev, fe, _, _ = O.develop(cs, counts, d["threshold_photons"], d["filament_expansion"],
+                                  d["clumping_proximity_factor"])
+        cs.developed, cs.fe = dev, fe
+        t2 = time.perf_counter()
+        O.scan(cs, tw, th, e["depth"], tw, th, spp=sc["samples_per_pixel"], ss=sc["supersample"],
+               sigma=sc["silver_opacity"], seed=seed + 2, threads=threads)
+        t3 = time.perf_counter()
+        t_tr += t1 - t0
+        t_dev += t2 - t1
+        t_scan += t3 - t2
+        paths += st["launched"]
+    scale_p = ppp / ppp_sample
+    frame_s = (t_tr * scale_p + t_dev + t_scan) * len(ranges) / len(picks)
+    frame_paths = sum(tw * th for (_, _, tw, th, _) in ranges) * ppp
     return {
-        "value": photons / (t3 - t0) / 1e6,
+        "value": frame_paths / frame_s / 1e6,
         "unit": "Mphoton-paths/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"tile (0,0) {tile_px}x{tile_px} px of the frame, {photons} photon paths "
-                   f"(reference chunking, {st['chunks']} chunks), develop, scan ss="
-                   f"{s['supersample']}; synthesis excluded"),
-        "transport_s": t1 - t0, "develop_s": t2 - t1, "scan_s": t3 - t2,
-        "scan_mpix_per_s": tile_px * tile_px / (t3 - t2) / 1e6,
-        "transport_mpaths_per_s": photons / (t1 - t0) / 1e6,
+        "cpu_model": _cpu_model(),
+        "sample": (f"{len(picks)} of {len(ranges)} tiles ({n_edge} edge tiles), stratified over the frame, "
+                   f"at {ppp_sample} of {ppp} photons per pixel ({paths} paths, reference chunking); "
+                   f"transport x {scale_p:g} + development + scan per tile, x {len(ranges) / len(picks):.2f} "
+                   f"to the frame; synthesis excluded; development and scan on the sample's smaller "
+                   f"developed set"),
+        "tiles": picks,
+        "sample_s": {"transport": t_tr, "develop": t_dev, "scan": t_scan},
+        "frame_s_projected": frame_s,
+        "transport_mpaths_per_s": paths / t_tr / 1e6,
     }
 
 
@@ -260,6 +302,31 @@ def film_main(args, engine, cfg):
     print(json.dumps(out), flush=True)
 
 
+def valu_probe(iters, blocks=4096):
+    """The VALU calibration kernel alone (blocks x 256 threads, 64 * iters + 4 VALU
+    instructions per wave): one JSON line with its known instruction count and time, for
+    scaling rocprofv3's SQ_INSTS_VALU and reading the device's VALU issue rate."""
+    import ctypes as C
+    import re
+
+    import fgrain_amd as F
+    from fgrain_amd import _lib as L
+
+    engine = F.GpuEngine.new()
+    m = re.search(r"(\d+) CUs", engine.adapter_name())
+    cus = int(m.group(1)) if m else 256
+    ms, per_wave = C.c_float(), C.c_uint64()
+    for _ in range(3):  # the last of three runs
+        L.check(L.lib().fg_calibrate_valu(engine._h, blocks, iters, C.byref(ms), C.byref(per_wave)))
+    waves = blocks * 4
+    insts = waves * per_wave.value
+    rate = insts / (ms.value * 1e-3)                 # wave instructions per second
+    print(json.dumps({"valu_probe_blocks": blocks, "waves": waves, "valu_per_wave": per_wave.value,
+                      "valu_insts": insts, "ms_kernel": ms.value, "cus": cus,
+                      "insts_per_s": rate,
+                      "issue_clock_ghz": rate * 4 / (cus * 4) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +343,8 @@ def main():
                     help="ignore fgrain_amd/configs/tuning/<workload>.json (autotuner output)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (counter calibration)")
+    ap.add_argument("--valu-probe", type=int, default=0, metavar="ITERS",
+                    help="only run the VALU calibration kernel (64 * ITERS + 4 VALU per wave)")
     args = ap.parse_args()
     args.steps_given = args.steps is not None
     if args.steps is None:
@@ -300,6 +369,9 @@ def main():
     from fgrain_amd import workloads as W
 
     from fgrain_amd.autotune import load_tuning
+
+    if args.valu_probe:
+        return valu_probe(args.valu_probe)
 
     cfg = W.load_config(args.workload)
     engine = F.GpuEngine.new()
@@ -334,43 +406,51 @@ def main():
         else:
             for k in ("ms_kernel_transport", "ms_kernel_scan", "ms_transport", "ms_develop", "ms_scan"):
                 agg[k] += st[k]
+            agg["graph_used"] = min(agg["graph_used"], st["graph_used"])
     elapsed = time.perf_counter() - t0
     ms_step = elapsed / args.steps * 1e3
     photons = agg["photons"]
     pixels = agg["pixels"]
-    # dominant kernel: transport.  Algorithmic bytes per launch = bytes per photon path of
-    # the reference algorithm (oracle count over sampled tiles of this workload: 4 B
-    # exposure + 32 B per BVH node + 48 B per crystal record + 8 B per absorbing hit on
-    # the full infinite-ray traversal) x the paths one launch processes.
+    # Dominant kernel: the frame transport, one launch per step.  Its HBM roofline is taken
+    # on the bytes it actually moves to and from HBM: rocprofv3 FETCH_SIZE + WRITE_SIZE of
+    # that kernel in one frame, each scaled by what the counter reads for a copy of known
+    # size (profiles/traffic_<workload>.json), over the kernel's live HIP-event time, so
+    # the fraction cannot exceed 1.  The reference algorithm's bytes per path (oracle count
+    # over sampled tiles: 4 B exposure + 32 B per BVH node + 48 B per crystal record + 8 B
+    # per absorbing hit on the full infinite-ray traversal) are reported beside it: they
+    # are served from L2 and the MALL, not HBM.  What bounds the kernel is instruction
+    # issue: the VALU issue fraction comes from the same committed profile (SQ_INSTS_VALU
+    # scaled by the VALU calibration kernel, 4 cycles per wave64 instruction).
     per_path = json.load(open(os.path.join(ROOT, "tests", "golden", "roofline_bytes.json"))).get(
         args.workload, {}).get("transport_per_path")
     if per_path is None:
         raise SystemExit(f"no algorithmic byte count for {args.workload}: "
-                         "run tests/golden/make_roofline_bytes.py {args.workload}")
+                         f"run tests/golden/make_roofline_bytes.py {args.workload}")
     alg_bytes = per_path["bytes"] * photons
     dev_bytes = 4 * photons + 32 * agg["nodes_visited"] + 48 * agg["prims_tested"] + 8 * agg["abs_events"]
-    traffic = None
     prof = {}
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath) and args.mode == "batched":
-        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs,
-        # copy-calibrated (profiles/transport_scan_frame24mp_batched.md)
         prof = json.load(open(tpath))
-        traffic = prof["transport_traffic_bytes"]
+    ms_kernel = agg["ms_kernel_transport"] / args.steps
     roof = None
-    if agg["ms_kernel_transport"] > 0:
-        ach = alg_bytes * args.steps / (agg["ms_kernel_transport"] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": MI355X_HBM_GBPS, "unit": "GB/s",
-                "frac": ach / MI355X_HBM_GBPS, "traffic": traffic,
-                "bytes_per_path": per_path["bytes"],
-                "launches_per_step": 1 if args.mode == "batched" else agg["tiles"]}
+    if ms_kernel > 0:
+        alg_gbps = alg_bytes / (ms_kernel * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": None, "peak": MI355X_HBM_GBPS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": prof.get("kernel", "transport"),
+                "launches_per_step": 1 if args.mode == "batched" else agg["tiles"],
+                "algorithmic": {"bytes_per_path": per_path["bytes"], "bytes": alg_bytes,
+                                "gbps": alg_gbps, "frac_of_hbm": alg_gbps / MI355X_HBM_GBPS,
+                                "note": "reference traversal bytes, served from L2/MALL"}}
         if prof:
-            # what bounds the kernel instead: the algorithmic bytes are served from L2 and
-            # the MALL (frac above 1); lane utilisation and L2 hit rate from the same
-            # committed rocprofv3 runs
-            roof["profiled"] = {"valu_lane_util": prof.get("transport_valu_lane_util"),
-                                "l2_hit_rate": prof.get("transport_l2_hit_rate"),
-                                "kernel": prof.get("kernel")}
+            traffic = prof["transport_traffic_bytes"]
+            ach = traffic / (ms_kernel * 1e-3) / 1e9
+            roof.update({"achieved": ach, "frac": ach / MI355X_HBM_GBPS, "traffic": traffic,
+                         "traffic_profile_ms": prof.get("transport_kernel_ms")})
+            roof["issue"] = {k: prof.get(k) for k in (
+                "transport_valu_insts_per_path", "transport_valu_issue_frac",
+                "transport_valu_issue_frac_of_probe", "transport_valu_lane_util", "transport_l2_hit_rate",
+                "valu_counter_scale")}
     out = {
         "metric": "Mphoton-paths/s + scanned Mpix/s, 24MP Tri-X, 1 GPU; adjoint/forward time ratio",
         "value": photons / (ms_step * 1e-3) / 1e6,
@@ -395,6 +475,8 @@ def main():
         "ms_breakdown": {k: agg[k] / args.steps for k in ("ms_transport", "ms_develop", "ms_scan",
                                                           "ms_kernel_transport", "ms_kernel_scan")},
         "roofline": roof,
+        "graph_used": agg.get("graph_used"),
+        "transport_atomics": agg.get("atomics"),
         "transport_per_path_executed": {
             "bytes": dev_bytes / photons, "nodes": agg["nodes_visited"] / photons,
             "prims": agg["prims_tested"] / photons, "bounces": agg["bounces"] / photons},
@@ -404,16 +486,21 @@ def main():
     if args.mode == "batched" and not args.no_plain:
         log("plain per-tile comparison step ...")
         t1 = time.perf_counter()
-        frame.render(W.Frame.PLAIN)
+        sp = frame.render(W.Frame.PLAIN)
         plain_ms = (time.perf_counter() - t1) * 1e3
         out["plain_ms_per_step"] = plain_ms
         out["batched_speedup_vs_plain"] = plain_ms / ms_step
+        out["plain_kernels_ms"] = {"transport": sp["ms_kernel_transport"], "scan": sp["ms_kernel_scan"],
+                                   "develop_host": sp["ms_develop"]}
+        out["batched_speedup_vs_plain_kernels"] = (
+            (sp["ms_kernel_transport"] + sp["ms_kernel_scan"]) /
+            ((agg["ms_kernel_transport"] + agg["ms_kernel_scan"]) / args.steps))
     if not args.no_adjoint:
         log("adjoint leg (tile512_trix_adjoint) ...")
         out["adjoint"] = adjoint_leg(engine)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, host cores) ...")
-        out["cpu_baseline"] = cpu_baseline(cfg, exposure)
+        out["cpu_baseline"] = cpu_baseline(cfg, exposure, frame)
     print(json.dumps(out), flush=True)
 
 

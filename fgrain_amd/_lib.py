@@ -48,7 +48,7 @@ class ExposureConfigC(C.Structure):
 class ExposeStatsC(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "launched", "bounces", "nodes_visited", "prims_tested", "abs_events", "abs_total")] + [
-        ("kernel_ms", C.c_float), ("pad_", C.c_float), ("atomics", C.c_uint64)]
+        ("kernel_ms", C.c_float), ("pad_", C.c_float), ("atomics", C.c_uint64), ("packets_run", C.c_uint64)]
 
 
 class DeviceTileC(C.Structure):
@@ -117,7 +117,7 @@ class FrameStatsC(C.Structure):
         "nodes_visited", "prims_tested", "abs_events")] + [
         (n, C.c_float) for n in ("ms_total", "ms_transport", "ms_develop", "ms_scan",
                                  "ms_kernel_transport", "ms_kernel_scan")] + [
-        ("graph_used", C.c_uint32), ("pad_", C.c_uint32), ("atomics", C.c_uint64)]
+        ("graph_used", C.c_uint32), ("pad_", C.c_uint32), ("atomics", C.c_uint64), ("packets_run", C.c_uint64)]
 
 
 class ExpectedModelC(C.Structure):
@@ -179,6 +179,7 @@ SIGNATURES = {
     "fg_get_engine": (_st, [C.POINTER(_vp)]),
     "fg_engine_set_tuning": (_st, [_vp, C.POINTER(TuningC)]),
     "fg_engine_get_tuning": (_st, [_vp, C.POINTER(TuningC)]),
+    "fg_calibrate_valu": (_st, [_vp, C.c_uint32, C.c_uint32, _f32p, C.POINTER(C.c_uint64)]),
     "fg_emulsion_synthesize": (_st, [C.POINTER(EmulsionConfigC), C.c_uint64, C.POINTER(_vp)]),
     "fg_emulsion_synthesize_device": (_st, [_vp, C.POINTER(EmulsionConfigC), C.c_uint64, C.POINTER(_vp)]),
     "fg_emulsion_from_crystals": (_st, [C.POINTER(EmulsionConfigC), C.c_int64, _f64p, _f64p,

@@ -6,6 +6,7 @@ the persistent lane-regenerating transport; the kernel choice applies to per-til
 * leaf-batch threshold: BVH node steps continue until this percentage of the walking
   lanes hold a full leaf queue (wait_pct);
 * transport occupancy: 6 or 7 waves per SIMD (register budget against latency hiding);
+* deposit pre-reduction across the wave before the global atomics (on or off);
 * scan tile shape: rays of a pixel per lane (the pixel's rays / that many adjacent lanes).
 The BVH walk is stackless (threaded skip links), so there is no traversal-stack depth to
 tune.  Transport and scan are separate kernels, so their knobs are searched separately:
@@ -34,6 +35,7 @@ TRANSPORT_GRID = {
     "transport_chunk": [64, 128, 256, 512],
     "transport_wait_pct": [65, 80, 90, 95, 100],
     "transport_waves": [6, 7],
+    "transport_prereduce": [0, 1],
 }
 SCAN_RAYS_PER_LANE = [1, 2, 4, 8, 16]
 
@@ -67,7 +69,7 @@ def tune(engine, workload: str, tiles=(4, 3), reps=3, log=print) -> dict:
     frame = W.Frame(engine, cfg, exposure, frame=(w, h))
     log(f"autotune {workload}: crop {w}x{h} px, {len(frame.tile_ranges())} tiles, {frame.crystals} crystals")
     default = {"transport_kernel": 0, "transport_chunk": 64, "transport_wait_pct": 80, "scan_rays_per_thread": 0,
-               "transport_waves": 6}
+               "transport_waves": 6, "transport_prereduce": 0}
     engine.set_tuning(**default)
     frame.render(W.Frame.BATCHED)
     ref_counts = frame.counts().copy()

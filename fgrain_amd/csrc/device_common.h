@@ -173,9 +173,10 @@ struct PhotonCounters {
 // absorbing hits and latent photons per lane in 32 bits (a lane handles far fewer than
 // 2^32 of each).
 // atomics: global atomic additions of deposits the lane issued (after the wave
-// pre-reduction).
+// pre-reduction); packets: packets the wave started (wave-uniform), which must come to
+// the launch's packet count.
 struct LaneCounters {
-    unsigned long long bounces = 0, nodes = 0;
+    unsigned long long bounces = 0, nodes = 0, packets = 0;
     uint32_t prims = 0, abs = 0, tot = 0, atomics = 0;
 };
 
@@ -192,6 +193,7 @@ __device__ __forceinline__ void flush_lane_counters(const LaneCounters &c, unsig
         atomicAdd(&stats[3], ab);
         atomicAdd(&stats[4], t);
         atomicAdd(&stats[5], at);
+        atomicAdd(&stats[7], c.packets);
     }
 }
 
@@ -570,9 +572,12 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             if (n > pool_left) {
                 unsigned long long nb = 0;
                 if (lane == 0) nb = atomicAdd(work, (unsigned long long)chunk);
-                // the pool is wave-uniform: keep it in scalar registers
-                nb = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(nb >> 32)) << 32) |
-                     __builtin_amdgcn_readfirstlane((uint32_t)nb);
+                // the pool is wave-uniform: keep it in scalar registers (readfirstlane
+                // returns int: each half goes through uint32_t, or a low half of 2^31 or
+                // more would be sign-extended into the high half)
+                const uint32_t nb_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(nb >> 32));
+                const uint32_t nb_lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)nb);
+                nb = ((unsigned long long)nb_hi << 32) | (unsigned long long)nb_lo;
                 if (r >= pool_left) id = nb + (r - pool_left);
                 pool = nb + (n - pool_left);
                 pool_left = chunk - (n - pool_left);
@@ -580,6 +585,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 pool += n;
                 pool_left -= n;
             }
+            cnt.packets += (unsigned long long)__popcll(__builtin_amdgcn_ballot_w64(phase == kNeed && id < total));
             if (phase == kNeed) {
                 if (id >= total) {
                     phase = kDone;

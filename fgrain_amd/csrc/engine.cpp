@@ -133,6 +133,25 @@ extern "C" fg_status fg_get_engine(fg_engine **out)
     return FG_OK;
 }
 
+extern "C" fg_status fg_calibrate_valu(fg_engine *E, uint32_t blocks, uint32_t iters, float *ms_kernel,
+                                      uint64_t *valu_per_wave)
+{
+    if (!E || blocks == 0 || iters == 0 || blocks > (1u << 20))
+        return fail(FG_ERR_INVALID_ARG, "fg_calibrate_valu: engine, 1..2^20 blocks and iters > 0 required");
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    HIP_TRY(E->out.ensure((size_t)blocks * sizeof(float)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(launch_valu_probe(E->out.as<float>(), blocks, iters, E->stream), FG_ERR_PIPELINE);
+    HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, E->ev0, E->ev1), FG_ERR_CHANNEL);
+    if (ms_kernel) *ms_kernel = ms;
+    if (valu_per_wave) *valu_per_wave = 64ull * iters + 4ull;
+    return FG_OK;
+}
+
 // ---------------------------------------------------------------- emulsion -------
 static bool valid_cfg(const fg_emulsion_config *c, std::string &msg)
 {
@@ -500,6 +519,7 @@ static fg_status expose_locked(fg_engine *E, Emulsion &em, const float *emap, ui
         stats->abs_events = st[3];
         stats->abs_total = st[4];
         stats->atomics = st[5];
+        stats->packets_run = st[7];
         float ms = 0;
         (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
         stats->kernel_ms = ms;
@@ -1147,6 +1167,8 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
     st->nodes_visited = cnt[1];
     st->prims_tested = cnt[2];
     st->abs_events = cnt[3];
+    st->atomics = cnt[5];
+    st->packets_run = cnt[7];
     st->ms_kernel_transport = ms_tk;
     st->ms_kernel_scan = ms_sk;
     st->ms_transport = (float)t_tr;
@@ -1260,6 +1282,7 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
     st->prims_tested = cnt[2];
     st->abs_events = cnt[3];
     st->atomics = cnt[5];
+    st->packets_run = cnt[7];
     const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
     const uint64_t g = std::max((uint32_t)llround(sqrt((double)spp)), 1u), ss = std::max(D.scan.supersample, 1u);
     for (const TileDev &t : tiles) {

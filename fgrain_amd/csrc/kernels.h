@@ -108,6 +108,9 @@ hipError_t launch_all_hits(const Node *nodes, uint32_t n_nodes, const float4 *re
 hipError_t launch_first_hit(const Node *nodes, uint32_t n_nodes, const float4 *recs,
                             const float *o, const float *d, int64_t n_rays, int32_t *slot,
                             float *t0, float *t1, hipStream_t s);
+// blocks x 256 threads of the VALU calibration kernel (64 * iters + 4 VALU per wave);
+// out: blocks floats (never written in practice)
+hipError_t launch_valu_probe(float *out, uint32_t blocks, uint32_t iters, hipStream_t s);
 hipError_t launch_otf3(const float *in, float *out, uint32_t w, uint32_t h, hipStream_t s);
 
 }  // namespace fg

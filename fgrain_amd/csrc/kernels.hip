@@ -173,7 +173,35 @@ __global__ void __launch_bounds__(256) otf3_kernel(const float *in, float *out, 
     out[(size_t)gy * w + gx] = u * k0 + c * k1 + dn * k2;
 }
 
+// ------------------------------------------------- VALU counter calibration -----
+// A known instruction count for the profiler's SQ_INSTS_VALU: per wave, 64 v_fmac_f32
+// per iteration plus 4 outside the loop (v_cvt, two v_mov, v_cmp; the store is never
+// taken), no other VALU instruction (checked in the gfx950 assembly of this kernel).
+#define FG_FMAC8(x)                                                                              \
+    asm volatile("v_fmac_f32 %0, %1, %2\n v_fmac_f32 %0, %1, %2\n v_fmac_f32 %0, %1, %2\n"        \
+                 " v_fmac_f32 %0, %1, %2\n v_fmac_f32 %0, %1, %2\n v_fmac_f32 %0, %1, %2\n"       \
+                 " v_fmac_f32 %0, %1, %2\n v_fmac_f32 %0, %1, %2"                                  \
+                 : "+v"(x)                                                                       \
+                 : "v"(b), "v"(c));
+__global__ void __launch_bounds__(256) valu_probe_kernel(float *out, uint32_t iters, float b, float c)
+{
+    float a = (float)threadIdx.x;
+    for (uint32_t i = 0; i < iters; i++) {
+        FG_FMAC8(a) FG_FMAC8(a) FG_FMAC8(a) FG_FMAC8(a) FG_FMAC8(a) FG_FMAC8(a) FG_FMAC8(a) FG_FMAC8(a)
+    }
+    if (a == 1.2345f) out[blockIdx.x] = a;
+}
+#undef FG_FMAC8
+
 }  // namespace
+
+hipError_t launch_valu_probe(float *out, uint32_t blocks, uint32_t iters, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launcher's launches
+    if (blocks == 0 || iters == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, s, out, iters, 1.0f, 0.0f);
+    return hipGetLastError();
+}
 
 hipError_t launch_transport(const TransportArgs &a, hipStream_t s)
 {

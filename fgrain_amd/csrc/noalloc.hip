@@ -2,7 +2,7 @@
 // callers that cannot allocate (code under HIP graph capture, serving loops that
 // forbid hipMalloc).  Inputs and outputs are the caller's device memory; the only
 // memory beyond them is a fixed 64-byte block per GPU (the persistent kernel's work
-// counter and the five statistics counters), a static device array whose address is
+// counter and its statistics counters), a static device array whose address is
 // looked up once per engine at creation.  Every call on a GPU shares that block, so
 // calls that overlap on one GPU race; each call clears it with a fill kernel first
 // (no memset node, so the sequence can be captured and replayed).
@@ -18,8 +18,9 @@ using namespace fg;
 
 namespace {
 
-// statistics counters 0..4 (bounces, nodes, crystal tests, absorbing hits, latent
-// photons), the persistent kernel's packet counter at kWorkWord, one spare word
+// statistics counters 0..5 (bounces, nodes, crystal tests, absorbing hits, latent
+// photons, global atomics), the persistent kernel's work counter at kWorkWord, packets
+// started at 7
 constexpr uint32_t kScratchWords = 8;
 constexpr uint32_t kWorkWord = 6;
 static_assert(kScratchWords * sizeof(unsigned long long) <= FG_NOALLOC_SCRATCH_BYTES,
@@ -108,6 +109,7 @@ extern "C" fg_status fg_expose_noalloc(fg_engine *E, const fg_emulsion *em_h, co
     stats->abs_events = st[3];
     stats->abs_total = st[4];
     stats->atomics = st[5];
+    stats->packets_run = st[7];
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
     stats->kernel_ms = ms;

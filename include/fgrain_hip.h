@@ -90,6 +90,8 @@ typedef struct fg_expose_stats {
     float pad_;
     uint64_t atomics;       /* global atomic additions issued for the deposits after the
                                wave pre-reduction (abs_events without it) */
+    uint64_t packets_run;   /* packets the persistent kernel started (= launched; 0 for
+                               the one-thread-per-packet kernel) */
 } fg_expose_stats;
 
 /* DevelopmentConfig (src/development.rs:12-38) */
@@ -190,6 +192,13 @@ typedef struct fg_tuning {
 } fg_tuning;
 fg_status fg_engine_set_tuning(fg_engine *engine, const fg_tuning *tuning);
 fg_status fg_engine_get_tuning(const fg_engine *engine, fg_tuning *tuning);
+
+/* Measurement support (no reference counterpart): runs blocks x 256 threads of a kernel
+ * that issues exactly 64 * iters + 4 VALU instructions per wave (*valu_per_wave), so a
+ * profiler's VALU instruction counter can be scaled and the device's issue rate read
+ * from *ms_kernel (HIP events). */
+fg_status fg_calibrate_valu(fg_engine *engine, uint32_t blocks, uint32_t iters, float *ms_kernel,
+                            uint64_t *valu_per_wave);
 
 /* ---------------------------------------------------------------- emulsion ----- */
 /* VirtualEmulsionLayer::synthesize (emulsion.rs:155-220): Bridson 3D + per-crystal
@@ -335,6 +344,7 @@ typedef struct fg_frame_stats {
                                      mode, or direct launches after a failed capture */
     uint32_t pad_;
     uint64_t atomics;             /* global atomic additions of the deposits (batched) */
+    uint64_t packets_run;         /* packets the batched transport started (= photons) */
 } fg_frame_stats;
 
 fg_status fg_frame_create(fg_engine *engine, const fg_frame_desc *desc, fg_frame **out);

@@ -275,6 +275,8 @@ def test_full_frame_properties(engine, workload):
     finally:
         engine.set_tuning(**old)
     assert s1["tiles"] == 247 and s1["photons"] == 128 * 6000 * 4000 and s1["rays"] == 16 * 6000 * 4000
+    # every packet of the frame was started (packet ids run past 2^31 here)
+    assert s1["packets_run"] == s1["photons"] and s3["packets_run"] == s3["photons"]
     assert np.array_equal(c1, c2) and (D1.view(np.uint32) == D2.view(np.uint32)).all()
     assert np.array_equal(c1, c3)
     assert (s3["bounces"], s3["nodes_visited"], s3["prims_tested"], s3["abs_events"]) == \
@@ -284,6 +286,27 @@ def test_full_frame_properties(engine, workload):
     assert np.isfinite(D1).all()
     # the scanner OTF is a convex 3-tap blend, so it keeps the range up to rounding
     assert D1.min() >= 0.05 - 1e-5 and D1.max() <= 4.05 + 1e-5
+
+
+def test_packet_ids_past_2_pow_31_batched_equals_plain(engine):
+    """A frame of 2.2e9 packets (two tiles at 10600 photons per pixel): the batched
+    transport's frame-wide packet ids pass 2^31 (a sign-extended 32-bit id would drop
+    every packet beyond it), and a one-tile frame of 2.2e9 packets takes the per-tile
+    kernel's ids past 2^31 too.  Every packet is started, and the counts and images of
+    the batched and the per-tile path are identical."""
+    cfg = _cfg(ppp=10600, seed=41)
+    for (w, h), ppp in (((640, 320), 10600), ((320, 320), 21000)):
+        cfg["exposure"]["photons_per_pixel"] = ppp
+        print(f"frame {w}x{h} ppp {ppp}: {w * h * ppp} packets", flush=True)
+        fr = W.Frame(engine, cfg, W.ramp_exposure(w, h, 0.01, 0.05), frame=(w, h))
+        sb = fr.render(W.Frame.BATCHED)
+        cb, Db = fr.counts(), fr.read()
+        sp = fr.render(W.Frame.PLAIN)
+        cp, Dp = fr.counts(), fr.read()
+        assert sb["photons"] > 2 ** 31
+        assert sb["packets_run"] == sb["photons"] == sp["packets_run"]
+        assert np.array_equal(cb, cp)
+        assert np.abs(Db - Dp).max() <= 1e-5
 
 
 @pytest.mark.parametrize("size", [(7, 5), (33, 2), (1, 1)])

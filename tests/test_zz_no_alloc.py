@@ -49,7 +49,8 @@ def _noalloc(engine, em, emap, x, seed, stream=None, with_stats=True):
     return counts, {k: getattr(st, k) for k, _ in L.ExposeStatsC._fields_ if k != "pad_"}
 
 
-STAT_KEYS = ("launched", "bounces", "nodes_visited", "prims_tested", "abs_events", "abs_total")
+STAT_KEYS = ("launched", "bounces", "nodes_visited", "prims_tested", "abs_events", "abs_total", "atomics",
+             "packets_run")
 
 
 def test_noalloc_transport_equals_fg_expose_on_two_calls(engine):
