@@ -324,7 +324,7 @@ def valu_probe(iters, blocks=4096):
     print(json.dumps({"valu_probe_blocks": blocks, "waves": waves, "valu_per_wave": per_wave.value,
                       "valu_insts": insts, "ms_kernel": ms.value, "cus": cus,
                       "insts_per_s": rate,
-                      "issue_clock_ghz": rate * 4 / (cus * 4) / 1e9}), flush=True)
+                      "insts_per_simd_per_ns": rate / (cus * 4) / 1e9}), flush=True)
 
 
 def main():
