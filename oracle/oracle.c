@@ -542,6 +542,91 @@ void orc_all_hits_f32(const orc_crystals *cs, int64_t n_rays, const float *o,
     bvh_free(&b);
 }
 
+/* First hit of each ray (the GPU's fg_traverse_first_hit): the smallest t_enter over
+ * intersect_ray_all's hits (bvh.rs:300-347, crystal.rs:140-217), ties to the earlier hit
+ * in depth-first callback order; f32 twin of the device arithmetic.  Threads split the
+ * rays (threads <= 0: one). */
+typedef struct {
+    allhits32_ctx base;
+    int32_t best;
+    float b0, b1;
+} first32_ctx;
+
+static inline int fh32_box(const onode *nd, first32_ctx *c) { return ah32_box(nd, &c->base); }
+static inline void fh32_prim(uint32_t id, first32_ctx *c)
+{
+    const ocrys32 *r = &c->base.cr[id];
+    float a, b;
+    if (crystal_intersect_s(r->c, r->q, r->dims, r->morph, c->base.o, c->base.d, &a, &b) &&
+        (c->best < 0 || a < c->b0)) {
+        c->best = (int32_t)id;
+        c->b0 = a;
+        c->b1 = b;
+    }
+}
+
+typedef struct {
+    const obvh *b;
+    const ocrys32 *cr;
+    const fbox *fb;
+    const float *o, *d;
+    int64_t r0, r1;
+    int32_t *idx;
+    float *t0, *t1;
+} first32_job;
+
+static void *first32_worker(void *arg)
+{
+    first32_job *j = (first32_job *)arg;
+    for (int64_t r = j->r0; r < j->r1; r++) {
+        first32_ctx c;
+        c.base.cr = j->cr;
+        c.base.fb = j->fb;
+        c.base.base = j->b->nodes;
+        for (int a = 0; a < 3; a++) {
+            c.base.o[a] = j->o[3 * r + a];
+            c.base.d[a] = j->d[3 * r + a];
+            c.base.inv[a] = 1.0f / c.base.d[a];
+        }
+        c.best = -1;
+        c.b0 = c.b1 = 0.0f;
+        tcount cnt = {0, 0};
+        if (j->b->nn > 0) DFS_TRAVERSE(j->b, fh32_box, fh32_prim, &c, &cnt);
+        j->idx[r] = c.best;
+        j->t0[r] = c.b0;
+        j->t1[r] = c.b1;
+    }
+    return NULL;
+}
+
+void orc_first_hit_f32(const orc_crystals *cs, int64_t n_rays, const float *o, const float *d,
+                       int32_t threads, int32_t *idx, float *t0, float *t1)
+{
+    obvh b;
+    bvh_build(cs, 1, &b);
+    ocrys32 *cr = (ocrys32 *)malloc(sizeof(ocrys32) * (size_t)(cs->n ? cs->n : 1));
+    for (int64_t i = 0; i < cs->n; i++) make_rec32(cs, i, &cr[i]);
+    fbox *fb = (fbox *)malloc(sizeof(fbox) * (size_t)(b.nn ? b.nn : 1));
+    for (int64_t k = 0; k < b.nn; k++)
+        for (int a = 0; a < 3; a++) {
+            fb[k].mn[a] = round_down(b.nodes[k].mn[a]);
+            fb[k].mx[a] = round_up(b.nodes[k].mx[a]);
+        }
+    const int nt = threads > 1 ? threads : 1;
+    first32_job *jobs = (first32_job *)malloc(sizeof(first32_job) * (size_t)nt);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nt);
+    for (int t = 0; t < nt; t++) {
+        jobs[t] = (first32_job){&b, cr, fb, o, d, n_rays * t / nt, n_rays * (t + 1) / nt, idx, t0, t1};
+        pthread_create(&th[t], NULL, first32_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+    free(fb);
+    free(cr);
+    bvh_free(&b);
+}
+
 /* BVH::query_radius (bvh.rs:350-402): inclusive box overlap, centre distance <= r. */
 static int64_t query_radius(const obvh *b, const orc_crystals *cs, const double c[3],
                             double r, int32_t *out, int64_t cap)

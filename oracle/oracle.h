@@ -98,6 +98,9 @@ void orc_all_hits_f64(const orc_crystals *cs, int64_t n_rays, const double *o,
 void orc_all_hits_f32(const orc_crystals *cs, int64_t n_rays, const float *o,
                       const float *d, int32_t max_hits, int32_t *nhits, int32_t *idx,
                       float *t0, float *t1);
+/* first hit per ray: smallest t_enter, ties to the earlier DFS hit; idx -1 = none */
+void orc_first_hit_f32(const orc_crystals *cs, int64_t n_rays, const float *o, const float *d,
+                       int32_t threads, int32_t *idx, float *t0, float *t1);
 int64_t orc_query_radius(const orc_crystals *cs, const double c[3], double r,
                          int32_t *out, int64_t cap);
 

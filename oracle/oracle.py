@@ -108,6 +108,8 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.orc_crystal_intersect_f64.argtypes = [C.POINTER(OrcCrystals), C.c_int64, _f64p, _f64p, _f64p]
+        L.orc_first_hit_f32.argtypes = [C.POINTER(OrcCrystals), C.c_int64, _f32p, _f32p, C.c_int32, _i32p,
+                                        _f32p, _f32p]
         L.orc_crystal_intersect_f32.argtypes = [C.POINTER(OrcCrystals), C.c_int64, _f32p, _f32p, _f32p]
         L.orc_bvh_build.argtypes = [C.POINTER(OrcCrystals), C.POINTER(OrcBvhOut)]
         L.orc_bvh_build.restype = C.c_int64
@@ -255,6 +257,21 @@ def all_hits(cs: Crystals, o, d, max_hits=256, fp32=False):
     if (nh > max_hits).any():
         raise ValueError("max_hits too small")
     return nh, idx, t0, t1
+
+
+def first_hit(cs: Crystals, o, d, threads=0):
+    """First hit per ray (f32 twin of the device traversal): (idx, t0, t1), idx -1 = none;
+    the smallest t_enter of intersect_ray_all's hits, ties to the earlier DFS hit."""
+    o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+    nr = o.shape[0]
+    idx = np.zeros(nr, np.int32)
+    t0 = np.zeros(nr, np.float32)
+    t1 = np.zeros(nr, np.float32)
+    cc = cs.c()
+    lib().orc_first_hit_f32(C.byref(cc), nr, _p(o, _f32p), _p(d, _f32p), threads or (os.cpu_count() or 1),
+                            _p(idx, _i32p), _p(t0, _f32p), _p(t1, _f32p))
+    return idx, t0, t1
 
 
 def query_radius(cs: Crystals, center, r, cap=4096):

@@ -48,3 +48,36 @@ def random_rays(n, W_, H_, D_, seed, scattered_frac=0.5):
     v /= np.linalg.norm(v, axis=1, keepdims=True)
     d = np.where((rng.random(n) < scattered_frac)[:, None], v, cone)
     return o.astype(np.float32), d.astype(np.float32)
+
+
+def halation_scene(W=64.0, D=24.0, zc=2.0, t=1.0):
+    """One cubic crystal wider than the patch (semi-axes 1000, 1000, t/2) centred at depth
+    zc: every packet of a collimated normal beam crosses it on the way down and again on a
+    specular reflection; a diffuse reflection crosses it when its step reaches it."""
+    return dict(pos=[[W / 2, W / 2, zc]], dims=[[1000.0, 1000.0, t / 2]], quat=[[1.0, 0, 0, 0]], W=W, D=D,
+                zc=zc, t=t)
+
+
+def halation_expected(mu, t, zc, D, n_refr, R, od=0.0, sens=1.0, p_diffuse=0.4):
+    """Expected latent photons per launched packet in halation_scene under a collimated
+    normal beam with no gelatin scattering or absorption (transport.rs:89-184,
+    optics.rs:147-196): Fresnel rejection r0 at normal incidence; the way down deposits
+    5 s pa (pa = 1 - exp(-mu t)) and leaves weight max(1 - 0.9 pa, 0.05); the backing
+    scales it by R 10^(-2 od); with probability 1 - p_diffuse the packet returns along the
+    same line (chord t again), otherwise in a cosine-weighted direction c = cos(theta),
+    whose chord through the crystal is cut by the step cap 1.5 D (the packet restarts at
+    D - 0.001); a scattering event ends its deposits (albedo 0)."""
+    from scipy import integrate
+    r0 = ((1 - n_refr) / (1 + n_refr)) ** 2
+    pa = 1 - np.exp(-mu * t)
+    rho = max(1 - 0.9 * pa, 0.05) * R * 10 ** (-2 * od)
+    z0 = D - 0.001
+    a, b, cap = z0 - (zc + t / 2), z0 - (zc - t / 2), 1.5 * D
+
+    def f(c):
+        L = max(min(b / c, cap) - a / c, 0.0)
+        return (1 - np.exp(-mu * L)) * 2 * c
+
+    brk = sorted({a / cap, b / cap})
+    diff, _ = integrate.quad(f, 0.0, 1.0, points=brk, epsabs=1e-13, epsrel=1e-12, limit=200)
+    return (1 - r0) * 5 * sens * (pa + rho * ((1 - p_diffuse) * pa + p_diffuse * diff))

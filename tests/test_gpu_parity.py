@@ -14,6 +14,7 @@ import pytest
 from scipy import stats
 
 import fgrain_amd as F
+from fgrain_amd import workloads as W
 from helpers import expose_kwargs, oracle_crystals, random_rays, trix_config
 from oracle import oracle as O
 
@@ -279,6 +280,85 @@ def test_transport_means_at_1e8_photons_vs_reference_sampler(engine):
     assert abs(z.mean()) < 5 / np.sqrt(ok.sum()) + 0.02
     assert 0.85 < z.std() < 1.15
     assert stats.ks_2samp(g * n_gpu / 16, c * n_gpu / 16).pvalue > 0.01
+
+
+def test_halation_branch_against_its_closed_form(engine):
+    """The device transport's anti-halation step (transport.rs:156-163, optics.rs:160-196)
+    against the closed form of helpers.halation_expected (collimated normal beam, no
+    gelatin scattering, one wide thin crystal, reflectance R with OD 0): 16 seeds of 2^22
+    packets each; the standard error comes from the spread of the seeds.  Swapping the
+    diffuse and specular probabilities, or dropping the backing, misses by far more."""
+    from helpers import halation_expected, halation_scene
+    sc = halation_scene()
+    for R in (0.0, 1.0):
+        cfg = trix_config(sc["W"], sc["W"], depth=sc["D"], r_ahu=R)
+        cfg.optics.absorption_coeff = 0.0
+        cfg.optics.scattering_coeff = 0.0
+        cfg.optics.refractive_index = 1.5
+        cfg.optics.antihalation_density = 0.0
+        means = []
+        for seed in range(16):
+            em = F.VirtualEmulsionLayer.from_crystals(cfg, sc["pos"], sc["dims"], sc["quat"], 0, 1.0)
+            st = F.expose_emulsion(em, None, F.ExposureConfig(1024.0, 0.0, 2.0, 48, 0), 300 + seed, engine)
+            assert st["launched"] == 1 << 22
+            means.append(int(em.absorbed_photons().sum()) / st["launched"])
+        m, se = np.mean(means), np.std(means, ddof=1) / 4.0
+        exp = halation_expected(2.0, sc["t"], sc["zc"], sc["D"], 1.5, R)
+        print(f"R {R}: mean {m:.6f} +- {se:.2e}, closed form {exp:.6f}", flush=True)
+        assert abs(m - exp) < 4.5 * se + 1e-4 * exp
+        if R > 0:
+            assert abs(halation_expected(2.0, sc["t"], sc["zc"], sc["D"], 1.5, R, p_diffuse=0.6) - exp) > 20 * se
+
+
+def test_c1_tile_first_hits_bit_exact(engine):
+    """The C1 tile (tile320_trix_cubic: 320 x 320 x 24 px, the reference seed's Tri-X
+    emulsion): first hits of 2^20 seeded rays (half from the top face, half inside the
+    slab; half in the 0.12 rad illumination cone, half in all directions) equal the FP32
+    oracle's bit for bit (index, t_enter, t_exit)."""
+    cfg = W.load_config("tile320_trix_cubic")
+    em = F.VirtualEmulsionLayer.synthesize(W.emulsion_config(cfg, 320.0, 320.0), cfg["seed"])
+    assert len(em) > 55000
+    o, d = random_rays(1 << 20, 320, 320, 24, seed=20)
+    idx, t0, t1 = engine.first_hit(em, o, d)
+    cs = oracle_crystals(em)
+    oi, ot0, ot1 = O.first_hit(cs, o, d)
+    assert (oi >= 0).sum() > (1 << 19)
+    assert (idx == oi).all()
+    hit = oi >= 0
+    assert (t0[hit].view(np.uint32) == ot0[hit].view(np.uint32)).all()
+    assert (t1[hit].view(np.uint32) == ot1[hit].view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("ss,scale", [(1, 1.0), (2, 1.0)])
+def test_c1_tile_scan_matches_fp64_oracle(engine, ss, scale):
+    """The C1 tile exposed, developed and scanned as the workload defines it (ramp 0.01-4.0,
+    64 photons per pixel, threshold 4, fe 1.2, pf 1.25, 4 rays per aperture; at that
+    exposure nearly every crystal develops and T stays below 0.04): the scan of the
+    developed set (transmittance.rs:65-171) at supersample 1 and 2 is within 1e-5 of the
+    FP64 oracle on the same rays, and D_neg within the same bound carried through the log."""
+    cfg = W.load_config("tile320_trix_cubic")
+    em = F.VirtualEmulsionLayer.synthesize(W.emulsion_config(cfg, 320.0, 320.0), cfg["seed"])
+    emap = W.workload_exposure(cfg) * np.float32(scale)
+    F.expose_emulsion(em, emap, W.exposure_config(cfg), cfg["seed"] + 1, engine)
+    rep = F.develop_emulsion(em, W.development_config(cfg), engine)
+    assert 0 < rep.developed_crystals < len(em)
+    st = em.state()
+    s = cfg["scan"]
+    sc = F.ScanConfig(320, 320, s["samples_per_pixel"], F.ScanMode.NegativeTransmittance, s["silver_opacity"], ss)
+    T, _ = F.scan_emulsion(em, sc, cfg["seed"] + 2, engine)
+    cs = oracle_crystals(em, st["developed"], st["fe"])
+    To = O.scan(cs, 320, 320, 24, 320, 320, spp=s["samples_per_pixel"], ss=ss, sigma=s["silver_opacity"],
+                seed=cfg["seed"] + 2)
+    print(f"C1 scan ss {ss} x{scale}: {rep.developed_crystals} of {len(em)} developed, "
+          f"T {To.min():.4f}..{To.max():.4f}, max |dT| {np.abs(T - To).max():.2e}", flush=True)
+    if scale < 1:
+        assert To.min() < 0.1 and To.max() > 0.9
+    assert np.abs(T - To).max() <= 1e-5
+    sc.mode = F.ScanMode.NegativeDensity
+    D, _ = F.scan_emulsion(em, sc, cfg["seed"] + 2, engine)
+    # D_neg = -log10(clamp(T, 1e-4, 1)) + 0.05: the T tolerance carried through the log
+    bound = 1e-5 / (np.log(10.0) * np.maximum(To, 1e-4)) + 1e-5
+    assert (np.abs(D - O.dneg(To)) <= bound).all()
 
 
 def test_development_exact(engine):

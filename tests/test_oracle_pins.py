@@ -208,6 +208,67 @@ def test_all_hits_order_is_right_first_dfs_and_complete_for_axis_aligned_cubes()
         assert sorted(order) == brute
 
 
+def test_first_hit_is_the_smallest_entry_over_all_crystals():
+    """The oracle's first hit (the twin of fg_traverse_first_hit): for axis-aligned cubes,
+    where the BVH finds every crystal a ray crosses, it is the brute-force smallest
+    t_enter over all crystals (f32 twin), ties to the earlier hit in the DFS order; for
+    rotated cubes it is the first smallest entry of intersect_ray_all's list."""
+    rng = np.random.default_rng(11)
+    n = 500
+    pos = rng.uniform(0, 1, (n, 3)) * np.array([30, 30, 10])
+    aligned = O.Crystals(pos, np.full((n, 3), 0.8), np.tile([1.0, 0, 0, 0], (n, 1)), np.zeros(n), np.ones(n))
+    q = Rotation.random(n, random_state=4).as_quat()[:, [3, 0, 1, 2]]
+    rotated = O.Crystals(pos, np.full((n, 3), 0.8), q, np.zeros(n), np.ones(n))
+    o = np.stack([rng.uniform(0, 30, 300), rng.uniform(0, 30, 300), rng.uniform(-1, 10, 300)], 1)
+    d = rng.normal(size=(300, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o, d = o.astype(np.float32), d.astype(np.float32)
+    for cs, brute in ((aligned, True), (rotated, False)):
+        idx, t0, _ = O.first_hit(cs, o, d, threads=3)
+        nh, oi, ot0, _ = O.all_hits(cs, o, d, max_hits=256, fp32=True)
+        assert (idx >= 0).sum() > 100
+        for r in range(len(o)):
+            if nh[r] == 0:
+                assert idx[r] == -1
+                continue
+            j = int(np.argmin(ot0[r, :nh[r]]))
+            assert idx[r] == oi[r, j] and t0[r] == ot0[r, j]
+            if brute:
+                hits = [(O.crystal_intersect(cs, i, o[r], d[r], fp32=True), i) for i in range(n)]
+                best = min(h[0][0] for h in hits if h[0] is not None)
+                tied = [i for h, i in hits if h is not None and h[0] == best]
+                dfs = [int(i) for i in oi[r, :nh[r]]]
+                assert idx[r] == min(tied, key=dfs.index)
+
+
+def test_halation_branch_against_its_closed_form():
+    """The anti-halation step of the transport (transport.rs:156-163, optics.rs:160-196)
+    against a closed form: a collimated normal beam, no gelatin scattering or absorption,
+    one wide thin crystal near the top, reflectance R with OD 0 (helpers.halation_expected:
+    the downward chord, then the reflected weight back along the same line with
+    probability 0.6 or cosine-diffusely with the chord cut by the step cap).  A branch
+    that skips the backing factor, reflects downward, scatters after the reflection or
+    swaps the diffuse and specular probabilities misses it by tens of standard errors."""
+    from helpers import halation_expected, halation_scene
+    sc = halation_scene()
+    cs = O.Crystals(np.array(sc["pos"]), np.array(sc["dims"]), np.array(sc["quat"]), np.zeros(1), np.ones(1))
+    for R in (0.0, 1.0):
+        means = []
+        for seed in range(4):
+            c, st = O.expose(cs, None, W=sc["W"], H=sc["W"], D=sc["D"], mu_a=0.0, mu_s=0.0, g=0.85, n_refr=1.5,
+                             r_ahu=R, od_ahu=0.0, mu_abs=2.0, theta_max=0.0, max_bounces=48, seed=100 + seed,
+                             sampler=0, photons_per_area=256.0, threads=4)
+            means.append(int(c.sum()) / st["launched"])
+        m, se = np.mean(means), np.std(means, ddof=1) / 2.0
+        exp = halation_expected(2.0, sc["t"], sc["zc"], sc["D"], 1.5, R)
+        assert abs(m - exp) < 4.5 * se + 1e-4 * exp, (R, m, exp, se)
+        if R > 0:
+            wrong = halation_expected(2.0, sc["t"], sc["zc"], sc["D"], 1.5, R, p_diffuse=0.6)
+            assert abs(wrong - exp) > 20 * se
+            no_backing = halation_expected(2.0, sc["t"], sc["zc"], sc["D"], 1.5, 0.0)
+            assert abs(no_backing - exp) > 100 * se
+
+
 # ---------------------------------------------------------------------- development
 def test_development_known_answer():
     """development.rs:197-265."""
