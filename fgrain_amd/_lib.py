@@ -204,6 +204,8 @@ SIGNATURES = {
                                   C.c_uint32, C.c_double, C.c_double, C.c_uint64]),
     "fg_render_tile_all_gpu": (_st, [_vp, _f32p, C.POINTER(TileParamsC), _f32p,
                                      C.POINTER(TileStatsC)]),
+    "fg_render_tile_delta_d": (_st, [_vp, C.c_int64, _f64p, _f64p, _f64p, _i32p, _f64p, C.POINTER(TileParamsC),
+                                     _f32p, _f32p]),
     "fg_frame_create": (_st, [_vp, C.POINTER(FrameDescC), C.POINTER(_vp)]),
     "fg_frame_destroy": (None, [_vp]),
     "fg_frame_render": (_st, [_vp, _vp, C.c_uint32, C.POINTER(FrameStatsC)]),

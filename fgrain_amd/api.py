@@ -320,6 +320,31 @@ class GpuEngine:
         self.last_tile_stats = {k: getattr(st, k) for k, _ in L.TileStatsC._fields_}
         return out
 
+    def render_tile_delta_d(self, developed_crystals: dict, params: "GpuTileParams",
+                            with_dneg: bool = False):
+        """GpuEngine::render_tile_delta_d (gpu.rs:941-1228): the developed crystals
+        (dict of arrays "pos" (n,3), "dims" (n,3) semi-axes, "quat" (n,4) (w,i,j,k),
+        "morph" (n,), "fe" (n,) filament expansion) scanned as scan_emulsion does, then
+        D_neg minus its 17-tap blur (gaussian.wgsl).  Returns width_px * height_px values (and D_neg
+        itself with with_dneg)."""
+        p = params.c
+        n_px = p.width_px * p.height_px
+        if n_px == 0:
+            z = np.zeros(0, np.float32)
+            return (z, z.copy()) if with_dneg else z
+        pos = np.ascontiguousarray(developed_crystals["pos"], np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        dims = np.ascontiguousarray(developed_crystals["dims"], np.float64).reshape(n, 3)
+        quat = np.ascontiguousarray(developed_crystals["quat"], np.float64).reshape(n, 4)
+        morph = np.ascontiguousarray(developed_crystals["morph"], np.int32).reshape(n)
+        fe = np.ascontiguousarray(developed_crystals["fe"], np.float64).reshape(n)
+        out = np.zeros(n_px, np.float32)
+        dneg = np.zeros(n_px, np.float32) if with_dneg else None
+        check(L.lib().fg_render_tile_delta_d(self._h, n, ptr(pos, L._f64p), ptr(dims, L._f64p),
+                                             ptr(quat, L._f64p), ptr(morph, L._i32p), ptr(fe, L._f64p),
+                                             C.byref(p), ptr(out, L._f32p), ptr(dneg, L._f32p)))
+        return (out, dneg) if with_dneg else out
+
     def traverse_all_hits(self, emulsion: VirtualEmulsionLayer, origins, dirs, max_hits=256):
         """BVH::intersect_ray_all on the GPU: (n_hits, idx, t0, t1) in DFS callback order."""
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)

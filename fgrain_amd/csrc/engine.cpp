@@ -72,7 +72,7 @@ extern "C" fg_status fg_engine_create(fg_engine **out)
 extern "C" void fg_engine_destroy(fg_engine *E)
 {
     if (!E) return;
-    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->scan_consts, &E->out, &E->out2,
+    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->scan_consts, &E->out, &E->out2, &E->hp,
                       &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
                       &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
                       &E->gmap, &E->partials, &E->pgrad, &E->gout, &E->boost, &E->dboost,
@@ -594,11 +594,12 @@ bool valid_scan(const fg_scan_config *c, std::string &msg)
     return true;
 }
 
-// The scan of an emulsion whose developed-crystal BVH (nodes, recs) was built on the
-// host by build_scan_bvh; the caller holds E->mu (the build needs no lock).
-static fg_status scan_locked(fg_engine *E, const Emulsion &em, const std::vector<Node> &nodes,
-                             const std::vector<Crystal> &recs, const fg_scan_config &cfg, uint64_t seed,
-                             float *out, float *kernel_ms)
+// Enqueues the scan of an emulsion whose developed-crystal BVH (nodes, recs) was built
+// on the host by build_scan_bvh, timed by E->ev0 / E->ev1; *res receives the device
+// buffer of the result (E->out, or E->out2 after the OTF).  The caller holds E->mu.
+static fg_status scan_enqueue(fg_engine *E, const Emulsion &em, const std::vector<Node> &nodes,
+                              const std::vector<Crystal> &recs, const fg_scan_config &cfg, uint64_t seed,
+                              const float **res_out)
 {
     HIP_TRY(E->scan_nodes.ensure(nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(E->scan_recs.ensure(recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
@@ -643,6 +644,19 @@ static fg_status scan_locked(fg_engine *E, const Emulsion &em, const std::vector
                 FG_ERR_PIPELINE);
         res = E->out2.as<float>();
     }
+    *res_out = res;
+    return FG_OK;
+}
+
+// scan_enqueue, then the result copied to the host array out (width_px * height_px).
+static fg_status scan_locked(fg_engine *E, const Emulsion &em, const std::vector<Node> &nodes,
+                             const std::vector<Crystal> &recs, const fg_scan_config &cfg, uint64_t seed,
+                             float *out, float *kernel_ms)
+{
+    const float *res = nullptr;
+    fg_status s = scan_enqueue(E, em, nodes, recs, cfg, seed, &res);
+    if (s != FG_OK) return s;
+    const size_t npx = (size_t)cfg.width_px * cfg.height_px;
     HIP_TRY(hipMemcpyAsync(out, res, npx * sizeof(float), hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
     HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
     if (kernel_ms) (void)hipEventElapsedTime(kernel_ms, E->ev0, E->ev1);
@@ -869,6 +883,60 @@ extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img,
         stats->ms_scan = ms_since(t3);
         stats->ms_total = ms_since(t0);
     }
+    return FG_OK;
+}
+
+extern "C" fg_status fg_render_tile_delta_d(fg_engine *E, int64_t n, const double *pos, const double *dims,
+                                            const double *quat, const int32_t *morph, const double *fe,
+                                            const fg_tile_params *p, float *out_delta_d, float *out_dneg)
+{
+    if (!E || !p || n < 0 || (n > 0 && (!pos || !dims || !quat || !morph || !fe)))
+        return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: null or negative argument");
+    if ((uint64_t)p->width_px * p->height_px == 0) return FG_OK;
+    if (!out_delta_d) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: null output");
+    if (n > (int64_t)(1u << 27)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: too many crystals");
+    if (!(p->film_w > 0) || !(p->film_h > 0) || !(p->film_d > 0) || !std::isfinite(p->film_w) ||
+        !std::isfinite(p->film_h) || !std::isfinite(p->film_d) || !std::isfinite(p->silver_opacity))
+        return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: film extents must be positive and finite");
+    const fg_scan_config sc{p->width_px, p->height_px, p->samples_per_pixel, FG_SCAN_DNEG, p->silver_opacity, 1, 0};
+    std::string msg;
+    if (!valid_scan(&sc, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: " + msg);
+    // every given crystal is developed with its own filament expansion (the caller passes
+    // the developed ones, gpu.rs:941-944); the scan builds its BVH over them
+    Emulsion em;
+    em.cfg = tile_emulsion_config(*p);
+    em.cfg.depth = p->film_d;
+    em.n = n;
+    em.pos.assign(pos, pos + 3 * n);
+    em.dims.assign(dims, dims + 3 * n);
+    em.quat.assign(quat, quat + 4 * n);
+    em.morph.assign(morph, morph + n);
+    em.sens.assign((size_t)n, 1.0);
+    em.developed.assign((size_t)n, 1u);
+    em.fe.assign(fe, fe + n);
+    for (int64_t i = 0; i < n; i++) {
+        if (morph[i] < 0 || morph[i] > 2)
+            return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: bad morphology");
+        if (!(fe[i] > 0) || !std::isfinite(fe[i]))
+            return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: filament expansion must be positive");
+    }
+    std::vector<Node> nodes;
+    std::vector<Crystal> recs;
+    if (!build_scan_bvh(em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: " + msg);
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    const float *dneg = nullptr;
+    fg_status s = scan_enqueue(E, em, nodes, recs, sc, p->seed, &dneg);
+    if (s != FG_OK) return s;
+    const size_t npx = (size_t)p->width_px * p->height_px;
+    HIP_TRY(E->hp.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(launch_highpass17(dneg, E->hp.as<float>(), p->width_px, p->height_px, E->stream), FG_ERR_PIPELINE);
+    HIP_TRY(hipMemcpyAsync(out_delta_d, E->hp.p, npx * sizeof(float), hipMemcpyDeviceToHost, E->stream),
+            FG_ERR_BUFFER_MAP);
+    if (out_dneg)
+        HIP_TRY(hipMemcpyAsync(out_dneg, dneg, npx * sizeof(float), hipMemcpyDeviceToHost, E->stream),
+                FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
     return FG_OK;
 }
 

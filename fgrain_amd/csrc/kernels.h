@@ -112,5 +112,7 @@ hipError_t launch_first_hit(const Node *nodes, uint32_t n_nodes, const float4 *r
 // out: blocks floats (never written in practice)
 hipError_t launch_valu_probe(float *out, uint32_t blocks, uint32_t iters, hipStream_t s);
 hipError_t launch_otf3(const float *in, float *out, uint32_t w, uint32_t h, hipStream_t s);
+// delta_d = D - the 17-tap blur of D, clamp-to-edge (gaussian.wgsl:14-59); in != out
+hipError_t launch_highpass17(const float *in, float *out, uint32_t w, uint32_t h, hipStream_t s);
 
 }  // namespace fg

@@ -273,6 +273,61 @@ hipError_t launch_first_hit(const Node *nodes, uint32_t n_nodes, const float4 *r
     return hipGetLastError();
 }
 
+// ------------------------------------------------------- scan high-pass ---------
+// delta_d = D - G(D) of the scan-only compatibility entry (gaussian.wgsl:21-59): the
+// shader's 17-tap blur with its literal weights (labelled sigma 2.5; they sum to 1.146,
+// so a flat field maps to -0.313 D) and clamp-to-edge taps, horizontal pass then
+// vertical pass, each summed in tap order as one f32 product and one f32 addition per
+// tap.  One 64 x 16 output tile per block: the clamped source tile
+// with an 8-pixel halo (32 x 80) and the horizontal pass of its 32 rows stay in LDS, so
+// D is read once and delta_d written once.
+namespace {
+constexpr int kHpR = 8, kHpX = 64, kHpY = 16;
+__constant__ float kHpW[2 * kHpR + 1] = {0.002621f, 0.007802f, 0.019445f, 0.040523f, 0.070624f, 0.103073f,
+                                         0.125925f, 0.134267f, 0.137440f, 0.134267f, 0.125925f, 0.103073f,
+                                         0.070624f, 0.040523f, 0.019445f, 0.007802f, 0.002621f};
+}  // namespace
+
+__global__ void __launch_bounds__(256) highpass17_kernel(const float *in, float *out, uint32_t w, uint32_t h)
+{
+    __shared__ float src[kHpY + 2 * kHpR][kHpX + 2 * kHpR + 1];
+    __shared__ float hor[kHpY + 2 * kHpR][kHpX + 1];
+    const int x0 = (int)blockIdx.x * kHpX, y0 = (int)blockIdx.y * kHpY;
+    // LDS cell (r, c) holds D at the clamped pixel (y0 - 8 + r, x0 - 8 + c), so the tap
+    // k of output column i is cell i + k + 8 in both directions
+    for (int k = threadIdx.x; k < (kHpY + 2 * kHpR) * (kHpX + 2 * kHpR); k += 256) {
+        const int r = k / (kHpX + 2 * kHpR), c = k % (kHpX + 2 * kHpR);
+        const int gx = min(max(x0 - kHpR + c, 0), (int)w - 1);
+        const int gy = min(max(y0 - kHpR + r, 0), (int)h - 1);
+        src[r][c] = in[(size_t)gy * w + gx];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (kHpY + 2 * kHpR) * kHpX; k += 256) {
+        const int r = k / kHpX, c = k % kHpX;
+        float sum = 0.0f;
+        for (int j = 0; j <= 2 * kHpR; j++) sum += src[r][c + j] * kHpW[j];
+        hor[r][c] = sum;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHpY * kHpX; k += 256) {
+        const int r = k / kHpX, c = k % kHpX;
+        const int gx = x0 + c, gy = y0 + r;
+        if (gx >= (int)w || gy >= (int)h) continue;
+        float sum = 0.0f;
+        for (int j = 0; j <= 2 * kHpR; j++) sum += hor[r + j][c] * kHpW[j];
+        out[(size_t)gy * w + gx] = src[r + kHpR][c + kHpR] - sum;
+    }
+}
+
+hipError_t launch_highpass17(const float *in, float *out, uint32_t w, uint32_t h, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launcher's launches
+    if (w == 0 || h == 0) return hipSuccess;
+    hipLaunchKernelGGL(highpass17_kernel, dim3((w + kHpX - 1) / kHpX, (h + kHpY - 1) / kHpY), dim3(256), 0, s,
+                       in, out, w, h);
+    return hipGetLastError();
+}
+
 hipError_t launch_otf3(const float *in, float *out, uint32_t w, uint32_t h, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launcher's launches

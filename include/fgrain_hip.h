@@ -305,6 +305,20 @@ fg_status fg_render_tile_all_gpu(fg_engine *engine, const float *tile_img,
                                  const fg_tile_params *params, float *out_dneg,
                                  fg_tile_stats *stats);
 
+/* GpuEngine::render_tile_delta_d (gpu.rs:941-1228), the scan-only compatibility entry:
+ * the n given crystals (pos, dims = semi-axes, quat = (w,i,j,k), morph 0 cubic / 1
+ * tabular / 2 octahedral, fe = each crystal's filament expansion) are all developed; they
+ * are scanned over the film_w x film_h x film_d patch at width_px x height_px with
+ * samples_per_pixel rays per pixel, silver_opacity and seed params->seed, as
+ * scan_emulsion does (transmittance.rs:65-171; D_neg = -log10 T + 0.05, no supersampling,
+ * no OTF), and out_delta_d receives D_neg minus its 17-tap blur (the shader's literal
+ * weights, which sum to 1.146; clamp-to-edge: gaussian.wgsl:14-59).  out_dneg (optional)
+ * receives D_neg itself.
+ * Zero pixels -> FG_OK with nothing written. */
+fg_status fg_render_tile_delta_d(fg_engine *engine, int64_t n, const double *pos, const double *dims,
+                                 const double *quat, const int32_t *morph, const double *fe,
+                                 const fg_tile_params *params, float *out_delta_d, float *out_dneg);
+
 /* ------------------------------------------------------------- frames ---------- */
 #define FG_FRAME_PLAIN 0u   /* one launch set + synchronisation per tile */
 #define FG_FRAME_BATCHED 1u /* persistent work-queue kernels over all tiles, replayed as

@@ -343,6 +343,29 @@ def otf3(img):
     return out
 
 
+# the 17 literal weights of gaussian.wgsl:14-19 (labelled sigma 2.5, radius 8; they sum
+# to 1.146, not to 1 as the shader's comment says)
+GAUSS17 = np.array([0.002621, 0.007802, 0.019445, 0.040523, 0.070624, 0.103073, 0.125925, 0.134267,
+                    0.137440, 0.134267, 0.125925, 0.103073, 0.070624, 0.040523, 0.019445, 0.007802,
+                    0.002621], np.float32)
+
+
+def highpass17(d):
+    """delta_d of render_tile_delta_d (gpu.rs:1180-1206): gaussian.wgsl:21-59 as a
+    horizontal pass into a temporary and a vertical pass writing orig - blur; taps
+    clamped to the edge, each pass an f32 sum in tap order (product, then addition)."""
+    d = np.ascontiguousarray(d, np.float32)
+    h, w = d.shape
+    xs, ys = np.arange(w), np.arange(h)
+    tmp = np.zeros_like(d)
+    for k in range(-8, 9):
+        tmp = tmp + d[:, np.clip(xs + k, 0, w - 1)] * GAUSS17[k + 8]
+    acc = np.zeros_like(d)
+    for k in range(-8, 9):
+        acc = acc + tmp[np.clip(ys + k, 0, h - 1), :] * GAUSS17[k + 8]
+    return d - acc
+
+
 def _synth(fn, *, W, H, D, r_min, mean_radius, size_sigma, morph, aspect=1.0, base_sens=1.2,
                sens_sigma=0.15, min_radius=0.15, min_half_thickness=0.05, k_samples=30, seed=0):
     cfg = OrcSynthCfg(W, H, D, r_min, mean_radius, size_sigma, aspect, base_sens, sens_sigma,

@@ -459,3 +459,41 @@ def test_tabular_synthesis_tilt_and_thickness():
     angle = 2 * np.arccos(np.clip(cs.quat[:, 0], -1, 1))
     assert (angle <= 0.17 + 1e-12).all()
     assert np.allclose(cs.quat[:, 3], 0.0)  # tilt axis lies in the film plane
+
+
+# ------------------------------------------------------------ scan high-pass (delta_d)
+def test_highpass17_against_scipy_correlation():
+    """gaussian.wgsl:21-59 twice (gpu.rs:1180-1206): the oracle's f32 passes against
+    scipy's separable correlation with edge replication in f64; a tap shifted by one
+    column or a pass on the wrong axis fails by orders of magnitude more."""
+    from scipy.ndimage import correlate1d
+    rng = np.random.default_rng(11)
+    d = rng.uniform(0.05, 3.0, (37, 53)).astype(np.float32)
+    w = O.GAUSS17.astype(np.float64)
+    want = d - correlate1d(correlate1d(d.astype(np.float64), w, axis=1, mode="nearest"), w, axis=0,
+                           mode="nearest")
+    got = O.highpass17(d)
+    assert np.abs(got - want).max() < 2e-6
+    shifted = d - correlate1d(correlate1d(d.astype(np.float64), w, axis=1, mode="nearest", origin=1), w,
+                              axis=0, mode="nearest")
+    assert np.abs(shifted - want).max() > 1e-2
+
+
+def test_highpass17_closed_forms():
+    """Flat field c: c (1 - S^2) with S = 1.146 the sum of the shader's weights; an
+    impulse far from the edges: delta minus the outer product of the weights; a field of
+    one row or one column is blurred along one axis only, the other axis repeating it."""
+    S = float(O.GAUSS17.astype(np.float64).sum())
+    assert S == pytest.approx(1.146, abs=1e-7)   # f32 weights
+    flat = O.highpass17(np.full((20, 30), 0.7, np.float32))
+    assert np.allclose(flat, 0.7 * (1.0 - S * S), atol=1e-6)
+    imp = np.zeros((41, 41), np.float32)
+    imp[20, 20] = 1.0
+    w = O.GAUSS17.astype(np.float64)
+    want = -np.outer(w, w)
+    want[8, 8] += 1.0
+    assert np.abs(O.highpass17(imp)[12:29, 12:29] - want).max() < 1e-7
+    row = np.random.default_rng(3).uniform(0, 1, (1, 25)).astype(np.float32)
+    from scipy.ndimage import correlate1d
+    assert np.abs(O.highpass17(row) - (row - S * correlate1d(row.astype(np.float64), w, axis=1,
+                                                              mode="nearest"))).max() < 1e-6
