@@ -46,6 +46,9 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
 // Render the frame; use_graph replays the captured hipGraphs of its three stages
 // (transport; development + developed-crystal BVH; scan), captured on first use.
 hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph);
+// Development and the developed-crystal BVH of the latent counts already in the input's
+// counts (direct launches; batched_kernel_ms then reports this stage alone).
+hipError_t batched_develop(BatchedFrame *b, hipStream_t s);
 // The graph call that failed when batched_render with use_graph returned an error.
 const char *batched_graph_failure(const BatchedFrame *b);
 hipError_t batched_read_tiles(BatchedFrame *b, hipStream_t s, std::vector<TileDev> &tiles,

@@ -63,7 +63,7 @@ struct Bufs {
     uint32_t *leafcnt;
     Node *dnodes;
     Crystal *drecs;
-    float4 *dconst;        // down_constants of the developed-crystal records
+    float4 *dconst;        // 2 per developed-crystal record: down_constants, centre residual
     Seg *seg[2][kSegClasses];   // [level parity][size class]
     uint32_t *seg_count;        // [3 rotating levels][size class]
     uint32_t *node_count;  // per tile
@@ -136,11 +136,13 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
         if (threadIdx.x == 0) *B.dev_count = 0u;
         if (threadIdx.x == 0) *B.work = 0ull;
     }
+    // counts and stats NULL: kept (development of given latent counts)
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += stride) {
-        B.leafcnt[i] = 0u;
+    const uint64_t lim = max(2 * n, max((uint64_t)n_tiles, 8ull));
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += stride) {
+        if (i < 2 * n) B.leafcnt[i] = 0u;
         if (i < n) {
-            counts[i] = 0u;
+            if (counts) counts[i] = 0u;
             B.csize[i] = 0u;
             B.root_flag[i] = 0u;
         }
@@ -148,7 +150,7 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
             B.node_count[i] = 0u;
             B.tiles[i].n_dev = B.tiles[i].clumps = B.tiles[i].max_clump = B.tiles[i].dev_nodes = 0u;
         }
-        if (i < 8) stats[i] = 0ull;
+        if (i < 8 && stats) stats[i] = 0ull;
     }
 }
 
@@ -892,7 +894,10 @@ __global__ void __launch_bounds__(256) k_dev_records(Bufs B, const Crystal *recs
     B.drecs[e] = r;
     float k[4];
     down_constants(r.q, r.h, r.morph, k);
-    B.dconst[e] = make_float4(k[0], k[1], k[2], k[3]);
+    B.dconst[2 * e] = make_float4(k[0], k[1], k[2], k[3]);
+    const double *p = &B.pos64[3 * (size_t)s];
+    B.dconst[2 * e + 1] = make_float4((float)(p[0] - (double)r.c[0]), (float)(p[1] - (double)r.c[1]),
+                                      (float)(p[2] - (double)r.c[2]), 0.0f);
 }
 
 __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, uint32_t n_tiles)
@@ -907,7 +912,7 @@ __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, 
     const float4 *recs = reinterpret_cast<const float4 *>(B.drecs + T.crystal_base);
     const float Tm = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, S.g, S.ss, T.pw, T.ph, S.z0,
                                 S.sigma, T.skey0, T.skey1, nodes, B.node_count[t], recs,
-                                B.dconst + T.crystal_base);
+                                B.dconst + 2ull * T.crystal_base);
     if (valid && threadIdx.x % L == 0u)
         out[T.out_base + (size_t)py * T.out_stride + px] = scan_value(Tm, S.mode);
 }
@@ -980,7 +985,7 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.leafcnt, 2 * n));
     TRY(alloc(b, B.dnodes, 2 * n));
     TRY(alloc(b, B.drecs, n));
-    TRY(alloc(b, B.dconst, n));
+    TRY(alloc(b, B.dconst, 2 * n));
     // a level's segments partition its developed crystals: at most n / 65 have more
     // than kWaveSegment crystals, n / 17 more than kLaneSegment
     for (int p = 0; p < 2; p++) {
@@ -1055,7 +1060,9 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     } while (0)
 #define TRY_LAUNCH(b, name) TRY_STEP(b, name, hipGetLastError())
 
-// Stage 0: reset and transport; 1: development and the developed-crystal BVH; 2: scan.
+// Stage 0: reset and transport; 1: development and the developed-crystal BVH; 2: scan;
+// kStageResetKeepCounts: the reset of stage 0 without the latent counts and counters.
+constexpr int kStageResetKeepCounts = 3;
 static hipError_t launch_stage(BatchedFrame *b, int stage, hipStream_t s)
 {
     // every launch is checked through hipGetLastError: clear what an earlier, unrelated
@@ -1065,11 +1072,14 @@ static hipError_t launch_stage(BatchedFrame *b, int stage, hipStream_t s)
     const BatchedInput &in = b->in;
     Bufs &B = b->B;
     const uint64_t n = (uint64_t)in.n_crystals;
-    const uint32_t nb = (uint32_t)((n + 255) / 256), nb2 = (uint32_t)((2 * n + 255) / 256);
-    if (stage == 0) {
-        hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>(nb2, 8192)), dim3(256), 0, s, B, in.counts,
-                           in.stats, n, b->n_tiles);
+    // at least one block per launch: a tile may hold no crystal
+    const uint32_t nb = std::max<uint32_t>(1u, (uint32_t)((n + 255) / 256));
+    const uint32_t nb2 = std::max<uint32_t>(1u, (uint32_t)((2 * n + 255) / 256));
+    if (stage == 0 || stage == kStageResetKeepCounts) {
+        hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>(nb2, 8192)), dim3(256), 0, s, B,
+                           stage == 0 ? in.counts : nullptr, stage == 0 ? in.stats : nullptr, n, b->n_tiles);
         TRY_LAUNCH(b, "k_reset");
+        if (stage == kStageResetKeepCounts) return hipSuccess;
         TransportArgs C = in.transport;
         C.stats = in.stats;
         C.estride = in.frame_w;
@@ -1160,6 +1170,16 @@ hipError_t batched_render(BatchedFrame *b, hipStream_t s, bool use_graph)
         TRY(hipEventRecord(b->ev[k + 1], s));
     }
     return hipSuccess;
+}
+
+hipError_t batched_develop(BatchedFrame *b, hipStream_t s)
+{
+    TRY(launch_stage(b, kStageResetKeepCounts, s));
+    TRY(hipEventRecord(b->ev[0], s));
+    TRY(hipEventRecord(b->ev[1], s));
+    TRY(launch_stage(b, 1, s));
+    TRY(hipEventRecord(b->ev[2], s));
+    return hipEventRecord(b->ev[3], s);
 }
 
 const char *batched_graph_failure(const BatchedFrame *b) { return b ? b->graph_step : ""; }

@@ -880,11 +880,11 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
 // queues the leaves it hits (up to kScanQueue, in an LDS column of its own,
 // lq[q * kQueueLanes]) and keeps stepping; steps continue until 90 % of the walking lanes
 // have a full queue or have ended their walk, then every lane tests the crystals of its
-// queued leaves in the order found.  consts holds each record's down_constants (the
-// ray direction in the crystal frame is a constant of the crystal for rays along -z);
-// crystal_hit_down with them is crystal_hit bit for bit.  The chords are added in
-// depth-first order either
-// way, so queueing changes no bit (frame scans 261 -> 148 ms Tri-X, 417 -> 191 ms
+// queued leaves in the order found.  consts holds two float4 per record: its
+// down_constants (the ray direction in the crystal frame is a constant of the crystal
+// for rays along -z) and its centre's rounding residual, with which crystal_chord_down
+// solves each crystal from the level of its centre.  The
+// chords are added in depth-first order either way, so queueing changes no bit (frame scans 261 -> 148 ms Tri-X, 417 -> 191 ms
 // T-Max against testing each leaf when it is hit; queue depths 8 / 12 / 16: 91.8 / 91.3 /
 // 90.3 ms Tri-X, 113.8 / 112.2 / 111.8 ms T-Max).  Lanes with active = false take part
 // in no step.
@@ -922,13 +922,12 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
             const uint32_t lf = lq[kQueueLanes * q];
             const uint32_t first = lf >> 4, cnt = lf & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
-                const float4 kc = buf_load4(consts_r, k << 4);
+                const float4 kc = buf_load4(consts_r, k << 5);
                 if (!sphere_may_hit_z_r(buf_load4(recs_r, k * 48u), kc.w, ox, oy)) continue;
                 const Rec r = load_rec_buf(recs_r, k);
-                const float kk[4] = {kc.x, kc.y, kc.z, kc.w};
-                float t0, t1;
-                if (crystal_hit_down(r.c, r.q, r.h, r.morph, kk, ox, oy, oz, t0, t1))
-                    P += fmaxf(t1 - t0, 0.0f);
+                const float4 cl = buf_load4(consts_r, (k << 5) + 16u);
+                const float kk[4] = {kc.x, kc.y, kc.z, kc.w}, lo[3] = {cl.x, cl.y, cl.z};
+                P += crystal_chord_down(r.c, r.q, r.h, r.morph, kk, lo, ox, oy, oz);
             }
         }
         nq = 0u;

@@ -72,7 +72,7 @@ extern "C" fg_status fg_engine_create(fg_engine **out)
 extern "C" void fg_engine_destroy(fg_engine *E)
 {
     if (!E) return;
-    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->scan_consts, &E->out, &E->out2, &E->hp,
+    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->scan_consts, &E->scan_lo, &E->out, &E->out2, &E->hp,
                       &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
                       &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
                       &E->gmap, &E->partials, &E->pgrad, &E->gout, &E->boost, &E->dboost,
@@ -551,25 +551,36 @@ extern "C" fg_status fg_expose(fg_engine *E, fg_emulsion *em, const float *emap,
     return expose_locked(E, em->em, emap, tw, th, *cfg, seed, stats);
 }
 
+static fg_status develop_device(fg_engine *E, Emulsion &em, const fg_development_config &cfg,
+                                fg_development_report &rep);
+
 extern "C" fg_status fg_develop(fg_engine *E, fg_emulsion *em, const fg_development_config *cfg,
                                 fg_development_report *report)
 {
     if (!em || !cfg) return fail(FG_ERR_INVALID_ARG, "fg_develop: null argument");
     fg_development_report rep;
-    if (E) {
-        std::lock_guard<std::mutex> lk(E->mu);
-        develop_host(em->em, *cfg, rep);
+    if (E && em->em.n > 0) {
+        const fg_status s = develop_device(E, em->em, *cfg, rep);
+        if (s != FG_OK) return s;
     } else {
-        develop_host(em->em, *cfg, rep);  // host work; no engine needed
+        develop_host(em->em, *cfg, rep);  // no engine: the host restatement
     }
     if (report) *report = rep;
     return FG_OK;
 }
 
-// Developed-crystal BVH (transmittance.rs:82-89) in the threaded layout.
-static bool build_scan_bvh(const Emulsion &em, std::vector<Node> &nodes, std::vector<Crystal> &recs,
-                           std::string &msg)
+// Developed-crystal BVH (transmittance.rs:82-89) in the threaded layout, its records
+// and each record's centre rounding residual (f64 centre minus f32 centre; 0 in w).
+struct ScanBvh {
+    std::vector<Node> nodes;
+    std::vector<Crystal> recs;
+    std::vector<float4> lo;
+};
+
+static bool build_scan_bvh(const Emulsion &em, ScanBvh &sb, std::string &msg)
 {
+    std::vector<Node> &nodes = sb.nodes;
+    std::vector<Crystal> &recs = sb.recs;
     std::vector<uint32_t> subset;
     for (int64_t i = 0; i < em.n; i++)
         if (em.developed[(size_t)i]) subset.push_back((uint32_t)i);
@@ -581,6 +592,12 @@ static bool build_scan_bvh(const Emulsion &em, std::vector<Node> &nodes, std::ve
     RefBvh mapped = b;
     for (auto &k : mapped.idx) k = subset[k];
     make_records(em, mapped, em.developed.data(), em.fe.data(), recs);
+    sb.lo.resize(recs.size());
+    for (size_t k = 0; k < recs.size(); k++) {
+        const double *p = &em.pos[3 * (size_t)mapped.idx[k]];
+        sb.lo[k] = make_float4((float)(p[0] - (double)recs[k].c[0]), (float)(p[1] - (double)recs[k].c[1]),
+                               (float)(p[2] - (double)recs[k].c[2]), 0.0f);
+    }
     return true;
 }
 
@@ -597,10 +614,11 @@ bool valid_scan(const fg_scan_config *c, std::string &msg)
 // Enqueues the scan of an emulsion whose developed-crystal BVH (nodes, recs) was built
 // on the host by build_scan_bvh, timed by E->ev0 / E->ev1; *res receives the device
 // buffer of the result (E->out, or E->out2 after the OTF).  The caller holds E->mu.
-static fg_status scan_enqueue(fg_engine *E, const Emulsion &em, const std::vector<Node> &nodes,
-                              const std::vector<Crystal> &recs, const fg_scan_config &cfg, uint64_t seed,
-                              const float **res_out)
+static fg_status scan_enqueue(fg_engine *E, const Emulsion &em, const ScanBvh &sb, const fg_scan_config &cfg,
+                              uint64_t seed, const float **res_out)
 {
+    const std::vector<Node> &nodes = sb.nodes;
+    const std::vector<Crystal> &recs = sb.recs;
     HIP_TRY(E->scan_nodes.ensure(nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(E->scan_recs.ensure(recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
     if (!nodes.empty())
@@ -609,9 +627,13 @@ static fg_status scan_enqueue(fg_engine *E, const Emulsion &em, const std::vecto
     if (!recs.empty())
         HIP_TRY(hipMemcpyAsync(E->scan_recs.p, recs.data(), recs.size() * sizeof(Crystal),
                                hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
-    HIP_TRY(E->scan_consts.ensure(recs.size() * sizeof(float4)), FG_ERR_BUFFER_RANGE);
-    HIP_TRY(launch_down_constants(E->scan_recs.as<float4>(), (uint32_t)recs.size(), E->scan_consts.as<float4>(),
-                                  E->stream), FG_ERR_PIPELINE);
+    HIP_TRY(E->scan_lo.ensure(recs.size() * sizeof(float4)), FG_ERR_BUFFER_RANGE);
+    if (!recs.empty())
+        HIP_TRY(hipMemcpyAsync(E->scan_lo.p, sb.lo.data(), recs.size() * sizeof(float4), hipMemcpyHostToDevice,
+                               E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(E->scan_consts.ensure(2 * recs.size() * sizeof(float4)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(launch_down_constants(E->scan_recs.as<float4>(), E->scan_lo.as<float4>(), (uint32_t)recs.size(),
+                                  E->scan_consts.as<float4>(), E->stream), FG_ERR_PIPELINE);
     const size_t npx = (size_t)cfg.width_px * cfg.height_px;
     HIP_TRY(E->out.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
     ScanArgs a{};
@@ -649,12 +671,11 @@ static fg_status scan_enqueue(fg_engine *E, const Emulsion &em, const std::vecto
 }
 
 // scan_enqueue, then the result copied to the host array out (width_px * height_px).
-static fg_status scan_locked(fg_engine *E, const Emulsion &em, const std::vector<Node> &nodes,
-                             const std::vector<Crystal> &recs, const fg_scan_config &cfg, uint64_t seed,
-                             float *out, float *kernel_ms)
+static fg_status scan_locked(fg_engine *E, const Emulsion &em, const ScanBvh &sb, const fg_scan_config &cfg,
+                             uint64_t seed, float *out, float *kernel_ms)
 {
     const float *res = nullptr;
-    fg_status s = scan_enqueue(E, em, nodes, recs, cfg, seed, &res);
+    fg_status s = scan_enqueue(E, em, sb, cfg, seed, &res);
     if (s != FG_OK) return s;
     const size_t npx = (size_t)cfg.width_px * cfg.height_px;
     HIP_TRY(hipMemcpyAsync(out, res, npx * sizeof(float), hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
@@ -670,12 +691,11 @@ extern "C" fg_status fg_scan(fg_engine *E, const fg_emulsion *em, const fg_scan_
     if (!E || !em || !out) return fail(FG_ERR_INVALID_ARG, "fg_scan: null argument");
     if (!valid_scan(cfg, msg)) return fail(FG_ERR_INVALID_ARG, "fg_scan: " + msg);
     if (cfg->width_px == 0 || cfg->height_px == 0) return FG_OK;
-    std::vector<Node> nodes;
-    std::vector<Crystal> recs;
-    if (!build_scan_bvh(em->em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "fg_scan: " + msg);
+    ScanBvh sb;
+    if (!build_scan_bvh(em->em, sb, msg)) return fail(FG_ERR_INVALID_ARG, "fg_scan: " + msg);
     std::lock_guard<std::mutex> lk(E->mu);
     HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
-    return scan_locked(E, em->em, nodes, recs, *cfg, seed, out, nullptr);
+    return scan_locked(E, em->em, sb, *cfg, seed, out, nullptr);
 }
 
 static fg_status traverse(fg_engine *E, const fg_emulsion *emw, int64_t n_rays, const float *o,
@@ -826,6 +846,173 @@ static float ms_since(Clock::time_point t0)
     return std::chrono::duration<float, std::milli>(Clock::now() - t0).count();
 }
 
+// ------------------------------------------------------ device tile pipeline -----
+// Appends one tile to a batched input: its crystal slots from crystal_base on (the leaf
+// order of its transport BVH), its transport constants and its TileDev record.  The
+// input's pos64 / dims64 / orig arrays must already have room for the slots.
+static void append_tile(BatchedInput &in, const Emulsion &em, const fg_exposure_config &x, const fg_tuning &tn,
+                        uint32_t x0, uint32_t y0, uint32_t tw, uint32_t th, uint64_t tile_seed,
+                        uint32_t node_base, uint32_t n_nodes, int64_t crystal_base, uint64_t out_base,
+                        uint32_t out_stride)
+{
+    for (int64_t j = 0; j < em.n; j++) {
+        const size_t sl = (size_t)(crystal_base + j);
+        const uint32_t ci = em.bvh.idx[(size_t)j];
+        for (int a = 0; a < 3; a++) {
+            in.pos64[3 * sl + a] = em.pos[3 * (size_t)ci + a];
+            in.dims64[3 * sl + a] = em.dims[3 * (size_t)ci + a];
+        }
+        in.orig[sl] = ci;
+    }
+    TransportArgs a{};
+    fill_transport(a, em.cfg, x, tw, th, tile_seed + 1, em.reach, tn);
+    if (in.tiles.empty()) in.transport = a;
+    TileDev td{};
+    td.x0 = x0;
+    td.y0 = y0;
+    td.tw = tw;
+    td.th = th;
+    td.W = a.W;
+    td.H = a.H;
+    td.pw = a.pw;
+    td.ph = a.ph;
+    td.key0 = a.key0;
+    td.key1 = a.key1;
+    td.skey0 = (uint32_t)(tile_seed + 2);
+    td.skey1 = (uint32_t)((tile_seed + 2) >> 32);
+    td.out_base = out_base;
+    td.out_stride = out_stride;
+    td.node_base = node_base;
+    td.n_nodes = n_nodes;
+    td.crystal_base = (uint32_t)crystal_base;
+    td.n = (uint32_t)em.n;
+    td.n_photons = a.n_photons;
+    in.tiles.push_back(td);
+}
+
+// Development, scan and transport-occupancy settings shared by a batched input's tiles.
+static void batched_settings(BatchedInput &in, const fg_development_config &dc, const fg_scan_config &sc,
+                             const fg_tuning &tn, double depth)
+{
+    in.develop = DevelopParams{dc.threshold_photons, 0u, dc.filament_expansion, dc.clumping_proximity_factor};
+    const uint32_t spp = std::max(sc.samples_per_pixel, 1u);
+    in.scan.g = std::max((uint32_t)llround(sqrt((double)spp)), 1u);
+    in.scan.ss = std::max(sc.supersample, 1u);
+    in.scan.mode = sc.mode;
+    in.scan.rays_per_lane = tn.scan_rays_per_thread;
+    in.scan.z0 = (float)(depth + 0.1);
+    in.scan.sigma = (float)sc.silver_opacity;
+    in.transport_waves = tn.transport_waves;
+}
+
+// Index widths of the batched kernels: 32-bit node ids (two nodes per crystal slot) and
+// 32-bit byte offsets into the node and record arrays.
+static bool batched_fits(uint64_t n_crystals, uint64_t n_nodes)
+{
+    return n_crystals * 2 < 0xFFFFFFFFull && offsets_fit_32(n_nodes, n_crystals);
+}
+
+// One tile's device arrays, and where its exposure and D_neg live.
+struct TileArrays {
+    const Node *nodes;
+    const Crystal *recs;
+    uint32_t n_nodes;
+    uint32_t *counts;
+    const float *emap;         // exposure map (not NULL when the transport runs), row stride
+    uint32_t estride;
+    float *dneg;               // D_neg from out_base, row stride out_stride
+    uint64_t out_base;
+    uint32_t out_stride;
+    unsigned long long *stats; // 8 transport counters
+};
+
+// A one-tile batched frame (batched.hip) over one tile's device arrays: the per-tile
+// paths develop, build the developed-crystal BVH and scan with the batched frame's
+// kernels, launched directly (no graph).  *out is NULL on failure.
+static fg_status one_tile_batch(fg_engine *E, const Emulsion &em, const TileArrays &d, uint32_t x0, uint32_t y0,
+                                uint32_t tw, uint32_t th, uint64_t tile_seed, const fg_exposure_config &x,
+                                const fg_development_config &dc, const fg_scan_config &sc, BatchedFrame **out)
+{
+    *out = nullptr;
+    if (!batched_fits((uint64_t)em.n, d.n_nodes))
+        return fail(FG_ERR_INVALID_ARG, "tile: too many crystals for the device pipeline");
+    BatchedInput in;
+    in.nodes = d.nodes;
+    in.recs = d.recs;
+    in.counts = d.counts;
+    in.emap = d.emap;
+    in.dneg = d.dneg;
+    in.stats = d.stats;
+    in.frame_w = d.estride;
+    in.frame_h = th;
+    in.n_crystals = em.n;
+    in.pos64.resize((size_t)em.n * 3);
+    in.dims64.resize((size_t)em.n * 3);
+    in.orig.resize((size_t)em.n);
+    append_tile(in, em, x, E->tuning, x0, y0, tw, th, tile_seed, 0u, d.n_nodes, 0, d.out_base, d.out_stride);
+    batched_settings(in, dc, sc, E->tuning, em.cfg.depth);
+    BatchedFrame *b = nullptr;
+    const hipError_t e = batched_create(in, E->stream, &b);
+    if (e != hipSuccess) {
+        batched_destroy(b);
+        return fail(FG_ERR_BUFFER_RANGE, std::string("tile pipeline: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return FG_OK;
+}
+
+struct BatchGuard {
+    BatchedFrame *b = nullptr;
+    ~BatchGuard() { batched_destroy(b); }
+};
+
+// develop_emulsion (development.rs:59-188) of one emulsion on the device: the batched
+// frame's threshold, union-find clumping and boost kernels over a one-tile frame, from
+// the emulsion's latent counts; flags, expansions and clump ids come back to the host.
+static fg_status develop_device(fg_engine *E, Emulsion &em, const fg_development_config &cfg,
+                                fg_development_report &rep)
+{
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    fg_status s = ensure_device(E, em);
+    if (s != FG_OK) return s;
+    const size_t n = (size_t)em.n;
+    std::vector<uint32_t> slots(n);
+    for (size_t k = 0; k < n; k++) slots[k] = em.counts[em.bvh.idx[k]];
+    const DeviceEmulsion &d = *em.dev;
+    HIP_TRY(hipMemcpyAsync(d.counts.p, slots.data(), n * 4, hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    const TileArrays ta{d.nodes.as<Node>(), d.recs.as<Crystal>(), d.n_nodes, d.counts.as<uint32_t>(), nullptr, 1,
+                        nullptr, 0, 1, nullptr};
+    const fg_exposure_config x{};
+    const fg_scan_config sc{1, 1, 1, FG_SCAN_DNEG, 0.0, 1, 0};
+    BatchGuard bg;
+    s = one_tile_batch(E, em, ta, 0, 0, 1, 1, 0, x, cfg, sc, &bg.b);
+    if (s != FG_OK) return s;
+    HIP_TRY(batched_develop(bg.b, E->stream), FG_ERR_PIPELINE);
+    std::vector<TileDev> tiles;
+    uint32_t err[4];
+    HIP_TRY(batched_read_tiles(bg.b, E->stream, tiles, err), FG_ERR_BUFFER_MAP);
+    std::vector<uint8_t> dv;
+    std::vector<double> fe;
+    std::vector<int32_t> cl;
+    HIP_TRY(batched_read_develop(bg.b, E->stream, dv, fe, cl), FG_ERR_BUFFER_MAP);
+    for (size_t k = 0; k < n; k++) {
+        const uint32_t i = em.bvh.idx[k];
+        em.developed[i] = dv[k];
+        em.fe[i] = fe[k];
+        em.clump[i] = cl[k];
+    }
+    const TileDev &t = tiles[0];
+    memset(&rep, 0, sizeof rep);
+    rep.total_crystals = n;
+    rep.developed_crystals = t.n_dev;
+    rep.fixed_crystals = n - t.n_dev;
+    rep.total_clumps = t.clumps;
+    rep.max_clump_size = t.max_clump;
+    rep.avg_clump_size = t.clumps > 0 ? (double)t.n_dev / (double)t.clumps : 0.0;
+    return FG_OK;
+}
+
 extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img, const fg_tile_params *p,
                                             float *out, fg_tile_stats *stats)
 {
@@ -835,9 +1022,10 @@ extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img,
     const fg_emulsion_config ec = tile_emulsion_config(*p);
     std::string msg;
     if (!valid_cfg(&ec, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: " + msg);
-    // The engine lock covers the GPU phases only (the reference's dispatch_lock,
-    // gpu.rs:169, 503, serialises GPU work): synthesis, development and the scan's BVH
-    // build run on the calling thread, so callers on several threads overlap them.
+    // Synthesis runs on the calling thread outside the engine lock (the reference's
+    // dispatch_lock, gpu.rs:169, 503, serialises GPU work only), so callers on several
+    // threads overlap it; transport, development, the developed-crystal BVH and the
+    // scan run on the device as a one-tile batched frame.
     const auto t0 = Clock::now();
     fg_emulsion em;
     synthesize(ec, p->seed, em.em);
@@ -848,39 +1036,46 @@ extern "C" fg_status fg_render_tile_all_gpu(fg_engine *E, const float *tile_img,
     xc.crystal_absorption_coeff = p->crystal_absorption_coeff;
     xc.max_bounces = p->max_bounces;
     xc.photons_per_pixel = 0;
-    fg_expose_stats xs;
-    const auto t1 = Clock::now();
-    fg_status s;
-    {
-        std::lock_guard<std::mutex> lk(E->mu);
-        HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
-        s = expose_locked(E, em.em, tile_img, p->width_px, p->height_px, xc, p->seed + 1, &xs);
-        free_device(em.em);
-    }
+    const fg_development_config dc{p->threshold_photons, 0, p->filament_expansion, p->clumping_proximity_factor, 0.85};
+    const fg_scan_config sc{p->width_px, p->height_px, p->samples_per_pixel, FG_SCAN_DNEG, p->silver_opacity, 1, 0};
+    const uint32_t tw = p->width_px, th = p->height_px;
+    const size_t npx = (size_t)tw * th;
+    std::lock_guard<std::mutex> lk(E->mu);
+    struct DeviceGuard {
+        Emulsion &em;
+        ~DeviceGuard() { free_device(em); }
+    } dguard{em.em};
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    fg_status s = ensure_device(E, em.em);
     if (s != FG_OK) return s;
-    const auto t2 = Clock::now();
-    fg_development_config dc{p->threshold_photons, 0, p->filament_expansion, p->clumping_proximity_factor, 0.85};
-    fg_development_report rep;
-    develop_host(em.em, dc, rep);
-    const auto t3 = Clock::now();
-    fg_scan_config sc{p->width_px, p->height_px, p->samples_per_pixel, FG_SCAN_DNEG, p->silver_opacity, 1, 0};
-    std::vector<Node> nodes;
-    std::vector<Crystal> recs;
-    if (!build_scan_bvh(em.em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_all_gpu: " + msg);
-    {
-        std::lock_guard<std::mutex> lk(E->mu);
-        HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
-        s = scan_locked(E, em.em, nodes, recs, sc, p->seed + 2, out, nullptr);
-    }
+    HIP_TRY(E->emap.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->out.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->stats.ensure(8 * sizeof(unsigned long long)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipMemcpyAsync(E->emap.p, tile_img, npx * sizeof(float), hipMemcpyHostToDevice, E->stream),
+            FG_ERR_BUFFER_MAP);
+    const DeviceEmulsion &d = *em.em.dev;
+    const TileArrays ta{d.nodes.as<Node>(), d.recs.as<Crystal>(), d.n_nodes, d.counts.as<uint32_t>(),
+                        E->emap.as<float>(), tw, E->out.as<float>(), 0, tw, E->stats.as<unsigned long long>()};
+    BatchGuard bg;
+    s = one_tile_batch(E, em.em, ta, 0, 0, tw, th, p->seed, xc, dc, sc, &bg.b);
     if (s != FG_OK) return s;
+    HIP_TRY(batched_render(bg.b, E->stream, false), FG_ERR_PIPELINE);
+    HIP_TRY(hipMemcpyAsync(out, E->out.p, npx * sizeof(float), hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    std::vector<TileDev> tiles;
+    uint32_t err[4];
+    HIP_TRY(batched_read_tiles(bg.b, E->stream, tiles, err), FG_ERR_BUFFER_MAP);
+    if (err[0] || err[1] || err[2])
+        return fail(FG_ERR_PIPELINE, "fg_render_tile_all_gpu: developed-crystal BVH exceeds the reference stack depth or leaf size");
     if (stats) {
+        float tk = 0, dk = 0, sk = 0;
+        HIP_TRY(batched_kernel_ms(bg.b, &tk, &dk, &sk), FG_ERR_CHANNEL);
         stats->crystals = (uint64_t)em.em.n;
-        stats->developed = rep.developed_crystals;
-        stats->clumps = rep.total_clumps;
-        stats->photons = xs.launched;
-        stats->ms_transport = std::chrono::duration<float, std::milli>(t2 - t1).count();
-        stats->ms_develop = std::chrono::duration<float, std::milli>(t3 - t2).count();
-        stats->ms_scan = ms_since(t3);
+        stats->developed = tiles[0].n_dev;
+        stats->clumps = tiles[0].clumps;
+        stats->photons = tiles[0].n_photons;
+        stats->ms_transport = tk;
+        stats->ms_develop = dk;
+        stats->ms_scan = sk;
         stats->ms_total = ms_since(t0);
     }
     return FG_OK;
@@ -920,13 +1115,12 @@ extern "C" fg_status fg_render_tile_delta_d(fg_engine *E, int64_t n, const doubl
         if (!(fe[i] > 0) || !std::isfinite(fe[i]))
             return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: filament expansion must be positive");
     }
-    std::vector<Node> nodes;
-    std::vector<Crystal> recs;
-    if (!build_scan_bvh(em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: " + msg);
+    ScanBvh sb;
+    if (!build_scan_bvh(em, sb, msg)) return fail(FG_ERR_INVALID_ARG, "fg_render_tile_delta_d: " + msg);
     std::lock_guard<std::mutex> lk(E->mu);
     HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
     const float *dneg = nullptr;
-    fg_status s = scan_enqueue(E, em, nodes, recs, sc, p->seed, &dneg);
+    fg_status s = scan_enqueue(E, em, sb, sc, p->seed, &dneg);
     if (s != FG_OK) return s;
     const size_t npx = (size_t)p->width_px * p->height_px;
     HIP_TRY(E->hp.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
@@ -960,13 +1154,15 @@ struct fg_frame {
     DevBuf d_layout;
     std::vector<Emulsion> ems;
     int64_t n_crystals = 0, n_nodes = 0;
-    DevBuf d_nodes, d_recs, d_counts, d_emap, d_dneg, d_dneg2, d_stats, d_scan_nodes, d_scan_recs, d_scan_consts;
+    DevBuf d_nodes, d_recs, d_counts, d_emap, d_dneg, d_dneg2, d_stats;
     bool rendered = false;
     bool otf_applied = false;
     uint32_t last_mode = FG_FRAME_PLAIN;
     BatchedFrame *batched = nullptr;
     bool graph_failed = false;
     fg_tuning batched_tuning{};
+    std::vector<BatchedFrame *> plain;   // one-tile frames of the plain path, per tile
+    fg_tuning plain_tuning{};
 };
 
 extern "C" int64_t fg_frame_crystal_count(const fg_frame *F) { return F ? F->n_crystals : 0; }
@@ -1132,116 +1328,76 @@ extern "C" void fg_frame_destroy(fg_frame *F)
 {
     if (!F) return;
     if (F->batched) batched_destroy(F->batched);
+    for (BatchedFrame *b : F->plain) batched_destroy(b);
     for (DevBuf *b : {&F->d_nodes, &F->d_recs, &F->d_counts, &F->d_emap, &F->d_dneg, &F->d_dneg2,
-                      &F->d_stats, &F->d_scan_nodes, &F->d_scan_recs, &F->d_scan_consts, &F->d_layout})
+                      &F->d_stats, &F->d_layout})
         b->release();
     delete F;
 }
 
-// Plain per-tile path: per tile one transport launch, a synchronisation and readback of
-// the latent counts, development and the developed-crystal BVH on the host, one scan
-// launch and a synchronisation -- the shape of render_tile_all_gpu called per tile.
+// Plain per-tile path: per tile one launch sequence (transport; development and the
+// developed-crystal BVH; scan -- a one-tile batched frame launched directly, no graph)
+// and a synchronisation with the readback of the tile's counters: the shape of
+// render_tile_all_gpu called per tile, with the frame's inputs resident.  The one-tile
+// frames are built on the first plain render and kept with the frame.
 static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *st)
 {
     const fg_frame_desc &D = F->desc;
-    std::vector<uint32_t> c;
-    float ms_tk = 0, ms_sk = 0;
-    double t_tr = 0, t_dev = 0, t_sc = 0;
-    HIP_TRY(hipMemsetAsync(F->d_stats.p, 0, 64, E->stream), FG_ERR_BUFFER_MAP);
-    for (size_t k = 0; k < F->tiles.size(); k++) {
-        const TileRec &t = F->tiles[k];
-        Emulsion &em = F->ems[k];
-        auto a0 = Clock::now();
-        TransportArgs a{};
-        a.nodes = F->d_nodes.as<Node>() + t.node_base;
-        a.recs = reinterpret_cast<const float4 *>(F->d_recs.as<Crystal>() + t.crystal_base);
-        a.n_nodes = t.n_nodes;
-        fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach, E->tuning);
-        a.emap = F->d_emap.as<float>() + (size_t)t.y0 * D.width + t.x0;
-        a.estride = D.width;
-        a.counts = F->d_counts.as<uint32_t>() + t.crystal_base;
-        a.stats = F->d_stats.as<unsigned long long>();
-        if (t.n) HIP_TRY(hipMemsetAsync(a.counts, 0, (size_t)t.n * 4, E->stream), FG_ERR_BUFFER_MAP);
-        HIP_TRY(hipMemsetAsync(F->d_stats.as<unsigned long long>() + 6, 0, 8, E->stream), FG_ERR_BUFFER_MAP);
-        HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
-        HIP_TRY(launch_transport_any(E, a, F->d_stats.as<unsigned long long>() + 6), FG_ERR_PIPELINE);
-        HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
-        c.resize((size_t)t.n);
-        if (t.n)
-            HIP_TRY(hipMemcpyAsync(c.data(), a.counts, (size_t)t.n * 4, hipMemcpyDeviceToHost, E->stream),
-                    FG_ERR_BUFFER_MAP);
-        HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
-        ms_tk += ms;
-        st->photons += a.n_photons;
-        auto a1 = Clock::now();
-        for (size_t j = 0; j < c.size(); j++) em.counts[em.bvh.idx[j]] = c[j];
-        fg_development_report rep;
-        develop_host(em, D.development, rep);
-        st->developed += rep.developed_crystals;
-        st->clumps += rep.total_clumps;
-        auto a2 = Clock::now();
-        std::vector<Node> nodes;
-        std::vector<Crystal> recs;
-        std::string msg;
-        if (!build_scan_bvh(em, nodes, recs, msg)) return fail(FG_ERR_INVALID_ARG, "scan: " + msg);
-        HIP_TRY(F->d_scan_nodes.ensure(nodes.size() * sizeof(Node)), FG_ERR_BUFFER_RANGE);
-        HIP_TRY(F->d_scan_recs.ensure(recs.size() * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
-        if (!nodes.empty())
-            HIP_TRY(hipMemcpyAsync(F->d_scan_nodes.p, nodes.data(), nodes.size() * sizeof(Node),
-                                   hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
-        if (!recs.empty())
-            HIP_TRY(hipMemcpyAsync(F->d_scan_recs.p, recs.data(), recs.size() * sizeof(Crystal),
-                                   hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
-        HIP_TRY(F->d_scan_consts.ensure(recs.size() * sizeof(float4)), FG_ERR_BUFFER_RANGE);
-        HIP_TRY(launch_down_constants(F->d_scan_recs.as<float4>(), (uint32_t)recs.size(),
-                                      F->d_scan_consts.as<float4>(), E->stream), FG_ERR_PIPELINE);
-        ScanArgs s{};
-        s.nodes = F->d_scan_nodes.as<Node>();
-        s.recs = F->d_scan_recs.as<float4>();
-        s.consts = F->d_scan_consts.as<float4>();
-        s.n_nodes = (uint32_t)nodes.size();
-        s.w_px = t.tw;
-        s.h_px = t.th;
-        const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
-        s.g = std::max((uint32_t)llround(sqrt((double)spp)), 1u);
-        s.ss = std::max(D.scan.supersample, 1u);
-        s.mode = D.scan.mode;
-        s.pw = (float)(em.cfg.width / (double)t.tw);
-        s.ph = (float)(em.cfg.height / (double)t.th);
-        s.z0 = (float)(em.cfg.depth + 0.1);
-        s.sigma = (float)D.scan.silver_opacity;
-        s.key0 = (uint32_t)(t.seed + 2);
-        s.key1 = (uint32_t)((t.seed + 2) >> 32);
-        s.out = F->d_dneg.as<float>() + t.out_base;
-        s.ostride = t.out_stride;
-        s.rays_per_lane = E->tuning.scan_rays_per_thread;
-        HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
-        HIP_TRY(launch_scan(s, E->stream), FG_ERR_PIPELINE);
-        HIP_TRY(hipEventRecord(E->ev1, E->stream), FG_ERR_CHANNEL);
-        HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
-        (void)hipEventElapsedTime(&ms, E->ev0, E->ev1);
-        ms_sk += ms;
-        st->rays += (uint64_t)t.tw * t.th * s.g * s.g * s.ss * s.ss;
-        auto a3 = Clock::now();
-        t_tr += std::chrono::duration<double, std::milli>(a1 - a0).count();
-        t_dev += std::chrono::duration<double, std::milli>(a2 - a1).count();
-        t_sc += std::chrono::duration<double, std::milli>(a3 - a2).count();
+    if (!F->plain.empty() && memcmp(&F->plain_tuning, &E->tuning, sizeof(fg_tuning)) != 0) {
+        for (BatchedFrame *b : F->plain) batched_destroy(b);
+        F->plain.clear();
     }
-    unsigned long long cnt[8];
-    HIP_TRY(hipMemcpy(cnt, F->d_stats.p, 64, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
-    st->bounces = cnt[0];
-    st->nodes_visited = cnt[1];
-    st->prims_tested = cnt[2];
-    st->abs_events = cnt[3];
-    st->atomics = cnt[5];
-    st->packets_run = cnt[7];
+    if (F->plain.empty()) {
+        F->plain_tuning = E->tuning;
+        F->plain.assign(F->tiles.size(), nullptr);
+        for (size_t k = 0; k < F->tiles.size(); k++) {
+            const TileRec &t = F->tiles[k];
+            const TileArrays ta{F->d_nodes.as<Node>() + t.node_base, F->d_recs.as<Crystal>() + t.crystal_base,
+                                t.n_nodes, F->d_counts.as<uint32_t>() + t.crystal_base, F->d_emap.as<float>(),
+                                D.width, F->d_dneg.as<float>(), (uint64_t)t.out_base, t.out_stride,
+                                F->d_stats.as<unsigned long long>()};
+            fg_status s = one_tile_batch(E, F->ems[k], ta, t.x0, t.y0, t.tw, t.th, t.seed, D.exposure,
+                                         D.development, D.scan, &F->plain[k]);
+            if (s != FG_OK) return s;
+        }
+    }
+    const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
+    const uint64_t g = std::max((uint32_t)llround(sqrt((double)spp)), 1u), ss = std::max(D.scan.supersample, 1u);
+    unsigned long long tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float ms_tk = 0, ms_dk = 0, ms_sk = 0;
+    for (size_t k = 0; k < F->tiles.size(); k++) {
+        BatchedFrame *b = F->plain[k];
+        HIP_TRY(batched_render(b, E->stream, false), FG_ERR_PIPELINE);
+        unsigned long long cnt[8];
+        HIP_TRY(hipMemcpyAsync(cnt, F->d_stats.p, sizeof cnt, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+        std::vector<TileDev> tiles;
+        uint32_t err[4];
+        HIP_TRY(batched_read_tiles(b, E->stream, tiles, err), FG_ERR_BUFFER_MAP);   // synchronises
+        if (err[0] || err[1] || err[2])
+            return fail(FG_ERR_PIPELINE, "plain frame: developed-crystal BVH exceeds the reference stack depth or leaf size");
+        for (int i = 0; i < 8; i++) tot[i] += cnt[i];
+        float tk = 0, dk = 0, sk = 0;
+        HIP_TRY(batched_kernel_ms(b, &tk, &dk, &sk), FG_ERR_CHANNEL);
+        ms_tk += tk;
+        ms_dk += dk;
+        ms_sk += sk;
+        const TileDev &t = tiles[0];
+        st->photons += t.n_photons;
+        st->developed += t.n_dev;
+        st->clumps += t.clumps;
+        st->rays += (uint64_t)t.tw * t.th * g * g * ss * ss;
+    }
+    st->bounces = tot[0];
+    st->nodes_visited = tot[1];
+    st->prims_tested = tot[2];
+    st->abs_events = tot[3];
+    st->atomics = tot[5];
+    st->packets_run = tot[7];
     st->ms_kernel_transport = ms_tk;
     st->ms_kernel_scan = ms_sk;
-    st->ms_transport = (float)t_tr;
-    st->ms_develop = (float)t_dev;
-    st->ms_scan = (float)t_sc;
+    st->ms_transport = ms_tk;
+    st->ms_develop = ms_dk;
+    st->ms_scan = ms_sk;
     return FG_OK;
 }
 
@@ -1270,55 +1426,12 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
         in.orig.resize((size_t)F->n_crystals);
         for (size_t k = 0; k < F->tiles.size(); k++) {
             const TileRec &t = F->tiles[k];
-            const Emulsion &em = F->ems[k];
-            for (int64_t j = 0; j < t.n; j++) {
-                const size_t sl = (size_t)(t.crystal_base + j);
-                const uint32_t ci = em.bvh.idx[(size_t)j];
-                for (int a = 0; a < 3; a++) {
-                    in.pos64[3 * sl + a] = em.pos[3 * (size_t)ci + a];
-                    in.dims64[3 * sl + a] = em.dims[3 * (size_t)ci + a];
-                }
-                in.orig[sl] = ci;
-            }
-            TransportArgs a{};
-            fill_transport(a, em.cfg, D.exposure, t.tw, t.th, t.seed + 1, em.reach, E->tuning);
-            if (k == 0) in.transport = a;
-            TileDev td{};
-            td.x0 = t.x0;
-            td.y0 = t.y0;
-            td.tw = t.tw;
-            td.th = t.th;
-            td.W = a.W;
-            td.H = a.H;
-            td.pw = a.pw;
-            td.ph = a.ph;
-            td.key0 = a.key0;
-            td.key1 = a.key1;
-            td.skey0 = (uint32_t)(t.seed + 2);
-            td.skey1 = (uint32_t)((t.seed + 2) >> 32);
-            td.out_base = (uint64_t)t.out_base;
-            td.out_stride = t.out_stride;
-            td.node_base = (uint32_t)t.node_base;
-            td.n_nodes = t.n_nodes;
-            td.crystal_base = (uint32_t)t.crystal_base;
-            td.n = (uint32_t)t.n;
-            td.n_photons = a.n_photons;
-            in.tiles.push_back(td);
+            append_tile(in, F->ems[k], D.exposure, E->tuning, t.x0, t.y0, t.tw, t.th, t.seed, (uint32_t)t.node_base,
+                        t.n_nodes, t.crystal_base, (uint64_t)t.out_base, t.out_stride);
         }
-        in.develop = DevelopParams{D.development.threshold_photons, 0u, D.development.filament_expansion,
-                                   D.development.clumping_proximity_factor};
-        const uint32_t spp = std::max(D.scan.samples_per_pixel, 1u);
-        in.scan.g = std::max((uint32_t)llround(sqrt((double)spp)), 1u);
-        in.scan.ss = std::max(D.scan.supersample, 1u);
-        in.scan.mode = D.scan.mode;
-        in.scan.rays_per_lane = E->tuning.scan_rays_per_thread;
-        in.transport_waves = E->tuning.transport_waves;
-        in.scan.z0 = (float)(D.emulsion.depth + 0.1);
-        in.scan.sigma = (float)D.scan.silver_opacity;
-        if ((uint64_t)F->n_crystals * 2 >= 0xFFFFFFFFull)
-            return fail(FG_ERR_INVALID_ARG, "batched frame: too many crystals for 32-bit node ids");
-        if (!offsets_fit_32((uint64_t)F->n_nodes, (uint64_t)F->n_crystals))
-            return fail(FG_ERR_INVALID_ARG, "batched frame: node or crystal arrays of 4 GiB or more");
+        batched_settings(in, D.development, D.scan, E->tuning, D.emulsion.depth);
+        if (!batched_fits((uint64_t)F->n_crystals, (uint64_t)F->n_nodes))
+            return fail(FG_ERR_INVALID_ARG, "batched frame: too many crystals for 32-bit node ids and offsets");
         HIP_TRY(batched_create(in, E->stream, &F->batched), FG_ERR_BUFFER_RANGE);
     }
     HIP_TRY(hipEventRecord(E->ev0, E->stream), FG_ERR_CHANNEL);
@@ -1473,29 +1586,20 @@ extern "C" fg_status fg_frame_read_development(fg_engine *E, const fg_frame *F, 
     if (!E || !F || n != F->n_crystals) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_development: bad argument");
     if (!F->rendered) return fail(FG_ERR_INVALID_ARG, "fg_frame_read_development: frame not rendered");
     std::lock_guard<std::mutex> lk(E->mu);
-    if (F->last_mode == FG_FRAME_PLAIN) {
-        for (size_t k = 0; k < F->tiles.size(); k++) {
-            const TileRec &t = F->tiles[k];
-            const Emulsion &em = F->ems[k];
-            for (int64_t j = 0; j < t.n; j++) {
-                const size_t o = (size_t)(t.crystal_base + j);
-                if (developed) developed[o] = em.developed[(size_t)j];
-                if (fe) fe[o] = em.fe[(size_t)j];
-                if (clump_id) clump_id[o] = em.clump[(size_t)j];
-            }
-        }
-        return FG_OK;
-    }
     HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
     std::vector<uint8_t> d;
     std::vector<double> f;
     std::vector<int32_t> c;
-    HIP_TRY(batched_read_develop(F->batched, E->stream, d, f, c), FG_ERR_BUFFER_MAP);
+    const bool plain = F->last_mode == FG_FRAME_PLAIN;
+    if (!plain) HIP_TRY(batched_read_develop(F->batched, E->stream, d, f, c), FG_ERR_BUFFER_MAP);
     for (size_t k = 0; k < F->tiles.size(); k++) {
         const TileRec &t = F->tiles[k];
         const Emulsion &em = F->ems[k];
+        // slots of the frame's batched pass, or of the tile's own one-tile frame
+        if (plain) HIP_TRY(batched_read_develop(F->plain[k], E->stream, d, f, c), FG_ERR_BUFFER_MAP);
+        const size_t base = plain ? 0 : (size_t)t.crystal_base;
         for (int64_t j = 0; j < t.n; j++) {
-            const size_t sl = (size_t)(t.crystal_base + j);
+            const size_t sl = base + (size_t)j;
             const size_t o = (size_t)t.crystal_base + em.bvh.idx[(size_t)j];
             if (developed) developed[o] = d[sl];
             if (fe) fe[o] = f[sl];

@@ -154,8 +154,7 @@ FG_HD bool crystal_hit(const float c[3], const float q[4], const float h[3], uin
 // Constants of a crystal for rays along -z (the scan): the direction (0, 0, -1) rotated
 // into the crystal frame depends on the crystal only, so its inverse (cube) or its
 // components over the semi-axes (other shapes) are computed once per crystal; k[3] is the
-// bounding-sphere radius of the scan's pre-test.  crystal_hit_down with these constants
-// performs exactly crystal_hit's operations for the direction (0, 0, -1).
+// bounding-sphere radius of the scan's pre-test.
 FG_HD void down_constants(const float q[4], const float h[3], uint32_t morph, float k[4])
 {
     float ex, ey, ez;
@@ -174,42 +173,49 @@ FG_HD void down_constants(const float q[4], const float h[3], uint32_t morph, fl
     k[3] = sqrtf(r2);
 }
 
-FG_HD bool crystal_hit_down(const float c[3], const float q[4], const float h[3], uint32_t morph,
-                            const float k[4], float ox, float oy, float oz, float &t0, float &t1)
+// Chord max(t_exit - max(t_enter, 0), 0) of a ray (ox, oy, oz) + t (0, 0, -1) through a
+// crystal (HalideCrystal::intersect_ray, crystal.rs:140-217, as transmittance.rs:
+// 141-150 uses it), 0 on a miss; k = down_constants, lo = the centre's rounding residual
+// (f64 centre minus the record's f32 centre), so the ray's offset from the centre is
+// (o - c) - lo, near the f64 offset (a record's f32 centre alone is off by up to half
+// an ulp of its coordinate, 4e-6 at 40 units, which moves a ray grazing a rounded
+// crystal across its rim).  The ray is solved from the point level with the crystal's
+// centre, t = t' + (oz - c_z): scan rays start above the slab, tens of semi-axes away,
+// and solving from there in f32 loses the chord of a thin crystal to cancellation (the
+// ellipsoid's b^2 - 4ac cancels to 1e-3 of the chord for an 8:1 platelet 20 units below
+// the origin).  From the centre's level the ellipsoid is solved about the ray's closest
+// approach in the unit frame, m = p + t_c q with t_c = -(p.q) / (q.q), chord
+// 2 sqrt((1 - m.m) / (q.q)); the box by its slabs.
+FG_HD float crystal_chord_down(const float c[3], const float q[4], const float h[3], uint32_t morph,
+                               const float k[4], const float lo[3], float ox, float oy, float oz)
 {
+    const float shift = (oz - c[2]) - lo[2];
     float lx, ly, lz;
-    qinv_rotate(q, ox - c[0], oy - c[1], oz - c[2], lx, ly, lz);
+    qinv_rotate(q, (ox - c[0]) - lo[0], (oy - c[1]) - lo[1], 0.0f, lx, ly, lz);
+    float te, tx;
     if (morph == 0u) {
         const float ix = k[0], iy = k[1], iz = k[2];
         const float a1 = (-h[0] - lx) * ix, a2 = (h[0] - lx) * ix;
         const float a3 = (-h[1] - ly) * iy, a4 = (h[1] - ly) * iy;
         const float a5 = (-h[2] - lz) * iz, a6 = (h[2] - lz) * iz;
-        const float tmin = fmaxf(fmaxf(fminf(a1, a2), fminf(a3, a4)), fminf(a5, a6));
-        const float tmax = fminf(fminf(fmaxf(a1, a2), fmaxf(a3, a4)), fmaxf(a5, a6));
-        if (tmax >= tmin && tmax > 0.0f) {
-            t0 = fmaxf(tmin, 0.0f);
-            t1 = tmax;
-            return true;
-        }
-        return false;
+        te = fmaxf(fmaxf(fminf(a1, a2), fminf(a3, a4)), fminf(a5, a6));
+        tx = fminf(fminf(fmaxf(a1, a2), fmaxf(a3, a4)), fmaxf(a5, a6));
+        if (!(tx >= te)) return 0.0f;
+    } else {
+        const float px = lx / h[0], py = ly / h[1], pz = lz / h[2];
+        const float qx = k[0], qy = k[1], qz = k[2];
+        const float qq = qx * qx + qy * qy + qz * qz;
+        const float tc = -(px * qx + py * qy + pz * qz) / qq;
+        const float mx = px + tc * qx, my = py + tc * qy, mz = pz + tc * qz;
+        const float rem = 1.0f - (mx * mx + my * my + mz * mz);
+        if (!(rem >= 0.0f)) return 0.0f;
+        const float half = sqrtf(rem / qq);
+        te = tc - half;
+        tx = tc + half;
     }
-    const float px = lx / h[0], py = ly / h[1], pz = lz / h[2];
-    const float qx = k[0], qy = k[1], qz = k[2];
-    const float a = qx * qx + qy * qy + qz * qz;
-    const float b = 2.0f * (px * qx + py * qy + pz * qz);
-    const float cc = (px * px + py * py + pz * pz) - 1.0f;
-    const float disc = b * b - 4.0f * a * cc;
-    if (disc < 0.0f) return false;
-    const float s = sqrtf(disc);
-    const float r0 = (-b - s) / (2.0f * a);
-    const float r1 = (-b + s) / (2.0f * a);
-    const float te = fminf(r0, r1), tx = fmaxf(r0, r1);
-    if (tx > 0.0f) {
-        t0 = fmaxf(te, 0.0f);
-        t1 = tx;
-        return true;
-    }
-    return false;
+    // t = t' + shift: exit beyond the origin, entry clamped at the origin
+    if (!(tx > -shift)) return 0.0f;
+    return te >= -shift ? tx - te : tx + shift;
 }
 
 // Chord of a crystal whose semi-axes are scaled by s (about its centre) and the chord's

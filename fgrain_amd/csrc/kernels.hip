@@ -54,14 +54,16 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
     if (valid && threadIdx.x % L == 0u) A.out[(size_t)py * A.ostride + px] = scan_value(T, A.mode);
 }
 
-__global__ void __launch_bounds__(256) down_constants_kernel(const float4 *recs, uint32_t n, float4 *consts)
+__global__ void __launch_bounds__(256) down_constants_kernel(const float4 *recs, const float4 *lo, uint32_t n,
+                                                             float4 *consts)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const Rec r = load_rec(recs, k);
     float c[4];
     down_constants(r.q, r.h, r.morph, c);
-    consts[k] = make_float4(c[0], c[1], c[2], c[3]);
+    consts[2 * k] = make_float4(c[0], c[1], c[2], c[3]);
+    consts[2 * k + 1] = lo[k];
 }
 
 // ------------------------------------------------------- traversal for tests ---
@@ -212,11 +214,11 @@ hipError_t launch_transport(const TransportArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_down_constants(const float4 *recs, uint32_t n, float4 *consts, hipStream_t s)
+hipError_t launch_down_constants(const float4 *recs, const float4 *lo, uint32_t n, float4 *consts, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launcher's launches
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(down_constants_kernel, dim3((n + 255) / 256), dim3(256), 0, s, recs, n, consts);
+    hipLaunchKernelGGL(down_constants_kernel, dim3((n + 255) / 256), dim3(256), 0, s, recs, lo, n, consts);
     return hipGetLastError();
 }
 

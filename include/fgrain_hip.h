@@ -244,7 +244,9 @@ fg_status fg_expose(fg_engine *engine, fg_emulsion *em, const float *exposure_ma
                     uint32_t tw, uint32_t th, const fg_exposure_config *cfg, uint64_t seed,
                     fg_expose_stats *stats);
 /* develop_emulsion (development.rs:59-188): integer threshold (exact), clumping as
- * connected components, clump boost.  Runs on the host; engine may be NULL. */
+ * connected components, clump boost, from the emulsion's latent counts.  With an engine
+ * it runs on the device (the batched frame's development kernels); engine NULL runs the
+ * same rules on the host. */
 fg_status fg_develop(fg_engine *engine, fg_emulsion *em, const fg_development_config *cfg,
                      fg_development_report *report);
 /* scan_emulsion (transmittance.rs:65-171) over the developed crystals; out has

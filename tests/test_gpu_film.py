@@ -12,11 +12,11 @@ pytestmark = pytest.mark.gpu
 
 # The compositing kernels (Hann blend, Gaussian separation, statistics, grain, output)
 # are switched off in the library: their first launch on the MI355X ended in a device
-# memory fault whose cause is not found yet (the host code and the kernels built for the
-# CPU run clean under AddressSanitizer and match the oracle bit for bit).  The tests that
-# launch them stay here, skipped, until the cause is fixed.
+# memory fault whose cause is not found yet (the kernels' source built for the CPU
+# against a HIP shim matches the oracle bit for bit).  The tests that launch them stay
+# here, skipped, until the cause is fixed.
 COMPOSITE_OFF = ("compositing kernels switched off: device memory fault on their first GPU launch, "
-                 "cause not found yet (CPU build of host code and kernels is ASan-clean and bit-exact)")
+                 "cause not found yet")
 
 
 @pytest.fixture(scope="module")

@@ -362,6 +362,9 @@ def test_c1_tile_scan_matches_fp64_oracle(engine, ss, scale):
 
 
 def test_development_exact(engine):
+    """develop_emulsion with an engine runs the device kernels (threshold, union-find
+    clumping over the tile's BVH, boost; development.rs:59-188): exact against the
+    oracle."""
     cfg = trix_config(96, 96)
     em = F.VirtualEmulsionLayer.synthesize(cfg, 4)
     rng = np.random.default_rng(0)
@@ -380,9 +383,13 @@ def test_development_exact(engine):
     assert rep.avg_clump_size == pytest.approx(orep["avg_clump"], rel=1e-12)
 
 
-@pytest.mark.parametrize("ss", [1, 2])
-def test_scan_matches_fp64_oracle(engine, ss):
-    cfg = trix_config(64, 48)
+@pytest.mark.parametrize("morph,ss", [(0, 1), (0, 2), (1, 1), (1, 2), (2, 1)])
+def test_scan_matches_fp64_oracle(engine, morph, ss):
+    """Every morphology (cubes, 8:1 tabular platelets, octahedral ellipsoids): the scan
+    solves each crystal from the level of its centre, so thin platelets far below the
+    ray origin keep their chords in f32 (from the origin, b^2 - 4ac cancels to 1e-3 of
+    the chord)."""
+    cfg = trix_config(64, 48, morph=morph, aspect=8.0 if morph == 1 else 1.0)
     em = F.VirtualEmulsionLayer.synthesize(cfg, 8)
     rng = np.random.default_rng(1)
     counts = rng.poisson(3.2, len(em)).astype(np.uint32)
@@ -481,23 +488,32 @@ def test_reference_render_tile_all_gpu(engine, exp_val):
 
 def test_render_tile_matches_stage_pipeline(engine):
     """render_tile_all_gpu = synthesize(seed) -> expose(seed+1) -> develop -> scan(seed+2)
-    (image_pipeline.rs:924-1014) through the stage API, bit for bit."""
-    p = F.GpuTileParams.for_preset("tri-x", 2.4, 40, 32, exposure=1.0, tile_seed=1234)
-    img = np.linspace(0.0, 1.0, 40 * 32, dtype=np.float32)
-    D = engine.render_tile_all_gpu(img, p)
-    cfg = F.EmulsionConfig.tri_x_400(p.c.film_w, p.c.film_h)
-    em = F.VirtualEmulsionLayer.synthesize(cfg, 1234)
-    F.expose_emulsion(em, img.reshape(32, 40), F.ExposureConfig(6.0, 0.12, 1.3, 48), 1235, engine)
-    F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
-    D2, _ = F.scan_emulsion(em, F.ScanConfig(40, 32, 4, F.ScanMode.NegativeDensity, 0.25), 1236, engine)
-    assert (D.reshape(32, 40).view(np.uint32) == D2.view(np.uint32)).all()
+    (image_pipeline.rs:924-1014): the drop-in runs all four on the device (development
+    and the developed-crystal BVH included); the stage API builds the scan's BVH on the
+    host.  Same photons and development; D_neg equal up to the order of the chord sum
+    inside a BVH leaf (the device build keeps the reference's leaf sets, not their
+    internal order), and the same scan within 1e-5 of the oracle's T."""
+    for seed, (w, h) in ((1234, (40, 32)), (77, (97, 61))):
+        p = F.GpuTileParams.for_preset("tri-x", 2.4, w, h, exposure=1.0, tile_seed=seed)
+        img = np.linspace(0.0, 1.0, w * h, dtype=np.float32)
+        D = engine.render_tile_all_gpu(img, p).reshape(h, w)
+        st = engine.last_tile_stats
+        cfg = F.EmulsionConfig.tri_x_400(p.c.film_w, p.c.film_h)
+        em = F.VirtualEmulsionLayer.synthesize(cfg, seed)
+        F.expose_emulsion(em, img.reshape(h, w), F.ExposureConfig(6.0, 0.12, 1.3, 48), seed + 1, engine)
+        rep = F.develop_emulsion(em, F.DevelopmentConfig(4, 1.2, 1.25), engine)
+        D2, _ = F.scan_emulsion(em, F.ScanConfig(w, h, 4, F.ScanMode.NegativeDensity, 0.25), seed + 2, engine)
+        assert st["crystals"] == len(em) and st["developed"] == rep.developed_crystals
+        assert st["clumps"] == rep.total_clumps
+        assert 0 < rep.developed_crystals < len(em)
+        T, T2 = 10.0 ** (0.05 - D.astype(np.float64)), 10.0 ** (0.05 - D2.astype(np.float64))
+        assert np.abs(T - T2).max() <= 2e-6
 
 
 def test_render_tile_all_gpu_from_threads_equals_sequential(engine):
     """Callers on several threads (the reference's Rayon tile loop, image_pipeline.rs:
     427-591) get exactly the results of sequential calls: the engine lock covers the GPU
-    phases while synthesis, development and the scan's BVH build run on each caller's
-    thread."""
+    phases while synthesis runs on each caller's thread."""
     from concurrent.futures import ThreadPoolExecutor
     jobs = []
     for k in range(8):

@@ -108,6 +108,24 @@ def _threaded_walk_hits(nodes, recs_idx, cs, o, d):
     return out
 
 
+def test_host_development_equals_oracle():
+    """fg_develop without an engine runs the host restatement (development.rs:59-188):
+    flags, clump ids, boosted expansions and the report equal the oracle's exactly."""
+    import ctypes as C
+    from fgrain_amd import _lib as L
+    em = F.VirtualEmulsionLayer.synthesize(trix_config(96, 96), 4)
+    counts = np.random.default_rng(0).poisson(3.0, len(em)).astype(np.uint32)
+    em.set_state(counts=counts)
+    cfg = F.DevelopmentConfig(4, 1.2, 1.25).c()
+    rep = L.DevelopmentReportC()
+    L.check(L.lib().fg_develop(None, em._h, C.byref(cfg), C.byref(rep)))
+    st = em.state()
+    dev, fe, cid, orep = O.develop(oracle_crystals(em), counts, 4, 1.2, 1.25)
+    assert (st["developed"] == dev).all() and (st["clump_id"] == cid).all() and (st["fe"] == fe).all()
+    assert rep.developed_crystals == orep["developed"] and rep.total_clumps == orep["clumps"]
+    assert rep.max_clump_size == orep["max_clump"]
+
+
 def test_threaded_layout_reproduces_dfs_hit_order():
     em = F.VirtualEmulsionLayer.synthesize(trix_config(40, 40), 9)
     nodes, recs = em.device_layout()
