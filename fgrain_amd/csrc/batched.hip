@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "batched.h"
+#include "partition.h"
 #include "device_common.h"
 
 namespace fg {
@@ -55,6 +56,8 @@ struct Bufs {
     // per slot
     double *pos64, *dims64, *mdim, *fe, *cen;
     uint32_t *orig, *tile_of, *devflag, *parent, *root, *csize, *root_flag, *prefix, *devpos;
+    uint32_t *devflag_o;   // developed flags by crystal index (crystal_base + orig)
+    uint32_t *prank, *rpos;  // partition scratch of k_bvh_level (per element)
     int32_t *clump;
     uint32_t *elem[2], *final_elem;
     float *elo, *ehi;
@@ -204,6 +207,7 @@ __global__ void __launch_bounds__(256) k_threshold(Bufs B, const uint32_t *count
     const uint32_t d = (s < n && counts[s] >= P.threshold) ? 1u : 0u;
     if (s < n) {
         B.devflag[s] = d;
+        B.devflag_o[B.tiles[B.tile_of[s]].crystal_base + B.orig[s]] = d;
         B.fe[s] = d ? P.fe : 0.0;
         B.parent[s] = (uint32_t)s;
         B.clump[s] = -1;
@@ -346,7 +350,9 @@ __global__ void __launch_bounds__(256) k_tile_scans(Bufs B)
 {
     TileDev &T = B.tiles[blockIdx.x];
     const uint32_t clumps = block_scan_flags(B.root_flag, B.prefix, T.crystal_base, T.n);
-    const uint32_t ndev = block_scan_flags(B.devflag, B.devpos, T.crystal_base, T.n);
+    // developed crystals numbered in crystal-index order: the developed-crystal BVH starts
+    // from the subset in the order transmittance.rs:82-89 clones it
+    const uint32_t ndev = block_scan_flags(B.devflag_o, B.devpos, T.crystal_base, T.n);
     if (threadIdx.x == 0) {
         T.clumps = clumps;
         T.n_dev = ndev;
@@ -369,7 +375,7 @@ __global__ void __launch_bounds__(256) k_boost(Bufs B, uint64_t n, DevelopParams
         B.clump[s] = (int32_t)B.prefix[crystal_base + B.orig[r]];
         if (r == s) key = t;
         // developed-crystal box (crystal.rs:106-133 with the boosted expansion) and centroid
-        const uint32_t e = crystal_base + B.devpos[s];
+        const uint32_t e = crystal_base + B.devpos[crystal_base + B.orig[s]];
         B.elem[0][e] = (uint32_t)s;
         const double m = B.mdim[s] * fe;
         for (int a = 0; a < 3; a++) {
@@ -425,6 +431,31 @@ struct MaxF { __device__ float operator()(float a, float b) const { return fmaxf
 struct MinD { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
 struct MaxD { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 struct SumU { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; } };
+
+// Destination of the element at position p of a segment of `count` in the in-place
+// partition of bvh.rs:243-256 (i from the start, j from the end; a small element -- its
+// centroid below the midpoint -- advances i, a large one is swapped to j).  That loop
+// swaps the k-th large element from the start with the k-th small one from the end
+// while the first precedes the second; every large element lands at the end of the
+// segment in the order the loop meets it.  Solved per element (checked against the loop
+// itself on 20 000 random segments): with r the element's rank among its kind counted
+// from the start and nl the number of small elements,
+//   small: k = nl - r is its rank from the end; it trades places with the k-th large
+//          element when p - r (large elements before it) >= k, else it stays;
+//   large: when the smalls after it, nl - (p - r), number at least r, it lands at
+//          count - 1 (r = 0) or one before the position of the r-th small from the end;
+//          else it moves down one place.
+// pos lists the smalls' positions by rank, then the larges'.
+__device__ __forceinline__ uint32_t partition_dest(bool small, uint32_t p, uint32_t r, uint32_t nl, uint32_t count,
+                                                   const uint32_t *pos)
+{
+    if (small) {
+        const uint32_t k = nl - r;
+        return (p - r >= k) ? pos[nl + k - 1] : p;
+    }
+    if (nl - (p - r) >= r) return r == 0 ? count - 1 : pos[nl - r] - 1;
+    return p - 1;
+}
 
 // Node ids of a split and its two child segments.  One lane does this per split (a block's
 // thread 0, a wave's lane 0); k_bvh_level_tiny aggregates the counters over its wave.
@@ -525,11 +556,14 @@ __global__ void __launch_bounds__(kLevelThreads) k_bvh_level(Bufs B, int level)
             for (uint32_t e = sg.start + threadIdx.x; e < sg.start + sg.count; e += kLevelThreads) dst[e] = src[e];
             nl = sg.count / 2;
         } else {
+            // pass 1: each element's rank among the small (centroid < mid) or the large ones,
+            // counted from the segment's start; prank lists the smalls' positions, then the
+            // larges', by rank
             uint32_t base_l = 0, base_r = 0;
             for (uint32_t c = 0; c < sg.count; c += kLevelThreads) {
-                const uint32_t e = sg.start + c + threadIdx.x;
-                const bool valid = c + threadIdx.x < sg.count;
-                const uint32_t s = valid ? src[e] : 0u;
+                const uint32_t p = c + threadIdx.x;
+                const bool valid = p < sg.count;
+                const uint32_t s = valid ? src[sg.start + p] : 0u;
                 const bool left = valid && B.cen[3 * s + axis] < mid;
                 const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
                 const unsigned long long below = (1ull << lane) - 1ull;
@@ -548,12 +582,21 @@ __global__ void __launch_bounds__(kLevelThreads) k_bvh_level(Bufs B, int level)
                     tr += wr[w];
                 }
                 if (valid) {
-                    if (left) dst[sg.start + base_l + ol + (uint32_t)__popcll(ml & below)] = s;
-                    else dst[sg.start + nl + base_r + orr + (uint32_t)__popcll(mr & below)] = s;
+                    const uint32_t r = left ? base_l + ol + (uint32_t)__popcll(ml & below)
+                                            : base_r + orr + (uint32_t)__popcll(mr & below);
+                    B.prank[sg.start + (left ? r : nl + r)] = p;
+                    B.rpos[sg.start + p] = r;
                 }
                 base_l += tl;
                 base_r += tr;
                 __syncthreads();
+            }
+            // pass 2: every element to its place in the reference's in-place partition
+            for (uint32_t p = threadIdx.x; p < sg.count; p += kLevelThreads) {
+                const uint32_t s = src[sg.start + p];
+                const uint32_t d = partition_dest(B.cen[3 * s + axis] < mid, p, B.rpos[sg.start + p], nl, sg.count,
+                                                  B.prank + sg.start);
+                dst[sg.start + d] = s;
             }
         }
         if (threadIdx.x == 0) emit_split(B, level, sg, nl, nd);
@@ -570,7 +613,7 @@ __device__ __forceinline__ V wave_reduce(V v, Op op)
 
 // The same level step for segments of kLaneSegment + 1 .. kWaveSegment crystals, one
 // wave per segment (a lane per crystal): the box, centroid extent, split count and the
-// stable left/right partition are those of k_bvh_level.  A block takes four segments per
+// partition are those of k_bvh_level.  A block takes four segments per
 // step (block-uniform trip count) and reserves their node ids and child segments with one
 // atomic per tile and per size class.
 __global__ void __launch_bounds__(256) k_bvh_level_small(Bufs B, int level)
@@ -635,8 +678,7 @@ __global__ void __launch_bounds__(256) k_bvh_level_small(Bufs B, int level)
                     if (valid) dst[e] = s;
                     nl = sg.count / 2;
                 } else if (valid) {
-                    if (left) dst[sg.start + (uint32_t)__popcll(ml & below)] = s;
-                    else dst[sg.start + nl + (uint32_t)__popcll(mr & below)] = s;
+                    dst[sg.start + partition_dest64(left, lane, ml, mr, sg.count)] = s;
                 }
                 if (lane == 0) {
                     w_seg[wave] = sg;
@@ -697,8 +739,8 @@ __global__ void __launch_bounds__(256) k_bvh_level_small(Bufs B, int level)
 }
 
 // The level step for segments of at most kLaneSegment crystals, one lane per segment:
-// sequential box and extent (min/max are exact in any order), then the same stable
-// left/right partition as the block and wave forms.  The children of such a segment are
+// sequential box and extent (min/max are exact in any order), then the reference's
+// in-place partition loop itself.  The children of such a segment are
 // again at most kLaneSegment long; a wave reserves its splits' node ids (per tile) and
 // child segments with one atomic each.
 __global__ void __launch_bounds__(256) k_bvh_level_tiny(Bufs B, int level)
@@ -759,11 +801,19 @@ __global__ void __launch_bounds__(256) k_bvh_level_tiny(Bufs B, int level)
                     for (uint32_t e = sg.start; e < sg.start + sg.count; e++) dst[e] = src[e];
                     nl = sg.count / 2;
                 } else {
-                    uint32_t il = sg.start, ir = sg.start + nl;
-                    for (uint32_t e = sg.start; e < sg.start + sg.count; e++) {
-                        const uint32_t s = src[e];
-                        if (B.cen[3 * s + axis] < mid) dst[il++] = s;
-                        else dst[ir++] = s;
+                    // the in-place partition of bvh.rs:243-256 itself, on the copy in dst
+                    for (uint32_t e = sg.start; e < sg.start + sg.count; e++) dst[e] = src[e];
+                    uint32_t i = sg.start, j = sg.start + sg.count - 1;
+                    while (i <= j) {
+                        const uint32_t s = dst[i];
+                        if (B.cen[3 * s + axis] < mid) {
+                            i++;
+                        } else {
+                            dst[i] = dst[j];
+                            dst[j] = s;
+                            if (j == sg.start) break;   // j would pass i (bvh.rs:251-253)
+                            j--;
+                        }
                     }
                 }
                 if (sg.depth > 61u) atomicOr(&B.err[0], 1u);
@@ -975,6 +1025,9 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.root_flag, n));
     TRY(alloc(b, B.prefix, n));
     TRY(alloc(b, B.devpos, n));
+    TRY(alloc(b, B.devflag_o, n));
+    TRY(alloc(b, B.prank, n));
+    TRY(alloc(b, B.rpos, n));
     TRY(alloc(b, B.clump, n));
     TRY(alloc(b, B.elem[0], n));
     TRY(alloc(b, B.elem[1], n));
