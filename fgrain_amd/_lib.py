@@ -213,6 +213,7 @@ SIGNATURES = {
     "fg_frame_read_tiles": (_st, [_vp, _vp, C.c_int64, _f32p, _i64p]),
     "fg_frame_read_counts": (_st, [_vp, _vp, C.c_int64, _u32p]),
     "fg_frame_read_development": (_st, [_vp, _vp, C.c_int64, _u8p, _f64p, _i64p]),
+    "fg_frame_get_tile": (_st, [_vp, _vp, C.c_int64, _i64p, _i64p, _f64p, _f64p, _f64p, _i32p, _f64p, _vp, _vp]),
     "fg_frame_crystal_count": (C.c_int64, [_vp]),
     "fg_expected_forward": (_st, [_vp, _vp, C.POINTER(ExpectedModelC), _f32p, C.c_uint32,
                                   C.c_uint32, _f32p, _f32p, _f32p, C.POINTER(AdjointStatsC)]),

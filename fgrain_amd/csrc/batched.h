@@ -60,4 +60,20 @@ hipError_t batched_read_develop(BatchedFrame *b, hipStream_t s, std::vector<uint
 hipError_t batched_kernel_ms(BatchedFrame *b, float *transport, float *develop, float *scan);
 void batched_destroy(BatchedFrame *b);
 
+// Transport BVHs (BVH::build, bvh.rs:157-295) of many tiles' crystals on the device, with
+// the batched frame's level kernels: tile t's crystals are slots base[t] .. base[t] +
+// count[t] in crystal order (pos, dims 3 per slot; quat 4; sens 1; one morphology).
+// Writes tile t's threaded nodes from nodes + 2 base[t] (node_count[t] of them), its
+// records in leaf order from recs + base[t], and leaf_slot[base[t] + k] = the slot of
+// leaf position k.  err: [0] a node deeper than the reference's stack, [1] a leaf of more
+// than 15 crystals, [2] an unfinished build.  Synchronises s.
+struct TransportBuildIn {
+    const double *pos, *dims, *quat, *sens;
+    uint32_t morph;
+    int64_t n;
+    std::vector<uint32_t> base, count;
+};
+hipError_t build_transport_bvhs(const TransportBuildIn &in, Node *nodes, Crystal *recs, uint32_t *leaf_slot,
+                                std::vector<uint32_t> &node_count, uint32_t err[4], hipStream_t s);
+
 }  // namespace fg

@@ -967,6 +967,63 @@ __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, 
         out[T.out_base + (size_t)py * T.out_stride + px] = scan_value(Tm, S.mode);
 }
 
+// ------------------------------------------------- transport BVHs on the device
+// BVH::build (bvh.rs:157-195) of every crystal of each tile: element boxes by
+// HalideCrystal::aabb of an undeveloped crystal (crystal.rs:106-133, expansion 1),
+// rounded outward to f32, f64 centroids; slots are in crystal order within a tile, so
+// the build starts from the index order 0..n like the reference.
+__global__ void __launch_bounds__(256) k_tb_init(Bufs B, uint64_t n, uint32_t n_tiles)
+{
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 3 * kSegClasses) B.seg_count[threadIdx.x] = 0u;
+        if (threadIdx.x < 4) B.err[threadIdx.x] = 0u;
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < max(2 * n, (uint64_t)n_tiles);
+         i += stride) {
+        if (i < 2 * n) B.leafcnt[i] = 0u;
+        if (i < n_tiles) B.node_count[i] = 0u;
+        if (i >= n) continue;
+        uint32_t lo = 0, hi = n_tiles;   // last tile with crystal_base <= i
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (B.tiles[mid].crystal_base <= i) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t s = (uint32_t)i;
+        B.tile_of[s] = lo;
+        B.orig[s] = s - B.tiles[lo].crystal_base;
+        const double *d = &B.dims64[3 * (size_t)s];
+        const double m = fmax(fmax(d[0], d[1]), d[2]);
+        B.mdim[s] = m;
+        for (int a = 0; a < 3; a++) {
+            const double l = B.pos64[3 * (size_t)s + a] - m, h = B.pos64[3 * (size_t)s + a] + m;
+            B.elo[3 * (size_t)s + a] = __double2float_rd(l);
+            B.ehi[3 * (size_t)s + a] = __double2float_ru(h);
+            B.cen[3 * (size_t)s + a] = 0.5 * (l + h);
+        }
+        B.elem[0][s] = s;
+    }
+}
+
+// transport records in leaf order (HalideCrystal fields as f32, make_records in host.cpp)
+__global__ void __launch_bounds__(256) k_tb_records(Bufs B, const double *quat, const double *sens, uint32_t morph,
+                                                    Crystal *recs, uint64_t n)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t s = B.final_elem[e];
+    Crystal r;
+    for (int a = 0; a < 3; a++) {
+        r.c[a] = (float)B.pos64[3 * (size_t)s + a];
+        r.h[a] = (float)B.dims64[3 * (size_t)s + a];
+    }
+    for (int a = 0; a < 4; a++) r.q[a] = (float)quat[4 * (size_t)s + a];
+    r.sens = (float)sens[s];
+    r.morph = morph;
+    recs[e] = r;
+}
+
 }  // namespace
 
 // --------------------------------------------------------------- host side -------
@@ -1267,6 +1324,80 @@ hipError_t batched_kernel_ms(BatchedFrame *b, float *transport, float *develop, 
     TRY(hipEventElapsedTime(transport, b->ev[0], b->ev[1]));
     TRY(hipEventElapsedTime(develop, b->ev[1], b->ev[2]));
     return hipEventElapsedTime(scan, b->ev[2], b->ev[3]);
+}
+
+hipError_t build_transport_bvhs(const TransportBuildIn &in, Node *nodes, Crystal *recs, uint32_t *leaf_slot,
+                                std::vector<uint32_t> &node_count, uint32_t err[4], hipStream_t s)
+{
+    const size_t n = (size_t)in.n;
+    const uint32_t n_tiles = (uint32_t)in.base.size();
+    node_count.assign(n_tiles, 0u);
+    for (int k = 0; k < 4; k++) err[k] = 0u;
+    if (n == 0 || n_tiles == 0) return hipSuccess;
+    BatchedFrame tmp;   // holds the build's scratch allocations (freed below)
+    Bufs &B = tmp.B;
+    B.pos64 = const_cast<double *>(in.pos);
+    B.dims64 = const_cast<double *>(in.dims);
+    B.dnodes = nodes;
+    B.final_elem = leaf_slot;
+    hipError_t e = hipSuccess;
+    auto A = [&](auto *&p, size_t count) {
+        if (e == hipSuccess) e = alloc(&tmp, p, count);
+    };
+    A(B.mdim, n);
+    A(B.cen, 3 * n);
+    A(B.orig, n);
+    A(B.tile_of, n);
+    A(B.elo, 3 * n);
+    A(B.ehi, 3 * n);
+    A(B.elem[0], n);
+    A(B.elem[1], n);
+    A(B.bn, 2 * n);
+    A(B.leafcnt, 2 * n);
+    A(B.prank, n);
+    A(B.rpos, n);
+    for (int p = 0; p < 2; p++) {
+        A(B.seg[p][kSegBlock], n / (kWaveSegment + 1) + 1);
+        A(B.seg[p][kSegWave], n / (kLaneSegment + 1) + 1);
+        A(B.seg[p][kSegLane], n);
+    }
+    A(B.seg_count, 3 * kSegClasses);
+    A(B.node_count, n_tiles);
+    A(B.err, 4);
+    A(B.tiles, n_tiles);
+    std::vector<TileDev> tiles(n_tiles);
+    for (uint32_t t = 0; t < n_tiles; t++) {
+        tiles[t] = TileDev{};
+        tiles[t].crystal_base = in.base[t];
+        tiles[t].n = tiles[t].n_dev = in.count[t];
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(B.tiles, tiles.data(), n_tiles * sizeof(TileDev), hipMemcpyHostToDevice, s);
+    const uint32_t nb = (uint32_t)((n + 255) / 256), nb2 = (uint32_t)((2 * n + 255) / 256);
+    (void)hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_tb_init, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nb2, 8192))), dim3(256), 0, s,
+                           B, (uint64_t)n, n_tiles);
+        hipLaunchKernelGGL(k_root_segments, dim3((n_tiles + 255) / 256), dim3(256), 0, s, B, n_tiles);
+        for (int level = 0; level < kMaxLevels; level++) {
+            hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(kLevelThreads), 0, s, B, level);
+            hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
+            hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
+        }
+        hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B);
+        hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, (uint64_t)(2 * n));
+        hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, (uint64_t)(2 * n));
+        hipLaunchKernelGGL(k_tb_records, dim3(nb), dim3(256), 0, s, B, in.quat, in.sens, in.morph, recs, (uint64_t)n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(node_count.data(), B.node_count, n_tiles * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(err, B.err, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = e2;
+    for (void *p : tmp.allocs) (void)hipFree(p);
+    tmp.allocs.clear();
+    return e;
 }
 
 void batched_destroy(BatchedFrame *b)

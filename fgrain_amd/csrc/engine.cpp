@@ -211,10 +211,11 @@ static void compact_cells(const float4 *cells, size_t n, std::vector<double> &pt
         }
 }
 
-// Poisson-disk centres of many emulsions in one pass of device launches; cells come
-// back to the host in one buffer (tile k's grid at offsets[k]).  Caller holds E->mu.
+// Poisson-disk centres of many emulsions in one pass of device launches into
+// E->synth_cells (tile k's grid at offsets[k]); with cells given, they come back to the
+// host in one buffer.  Caller holds E->mu.
 static fg_status device_cells_tiles(fg_engine *E, const std::vector<fg_emulsion_config> &cfgs,
-                                    const std::vector<uint64_t> &seeds, std::vector<float4> &cells,
+                                    const std::vector<uint64_t> &seeds, std::vector<float4> *cells,
                                     std::vector<size_t> &offsets)
 {
     const size_t n = cfgs.size();
@@ -244,9 +245,12 @@ static fg_status device_cells_tiles(fg_engine *E, const std::vector<fg_emulsion_
     const uint32_t rounds = (n && cfgs[0].k_samples) ? cfgs[0].k_samples : 30u;
     fg_status st = FG_OK;
     hipError_t e = launch_darts_tiles(dargs.as<DartArgs>(), (uint32_t)n, max_phase, rounds, E->stream);
-    cells.resize(total);
-    if (e == hipSuccess && total)
-        e = hipMemcpyAsync(cells.data(), E->synth_cells.p, total * sizeof(float4), hipMemcpyDeviceToHost, E->stream);
+    if (cells) {
+        cells->resize(total);
+        if (e == hipSuccess && total)
+            e = hipMemcpyAsync(cells->data(), E->synth_cells.p, total * sizeof(float4), hipMemcpyDeviceToHost,
+                               E->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
     if (e != hipSuccess) st = fail(FG_ERR_PIPELINE, std::string("device synthesis: ") + hipGetErrorString(e));
     dargs.release();
@@ -259,7 +263,7 @@ static fg_status device_points(fg_engine *E, const fg_emulsion_config &cfg, uint
 {
     std::vector<float4> cells;
     std::vector<size_t> off;
-    fg_status s = device_cells_tiles(E, {cfg}, {seed}, cells, off);
+    fg_status s = device_cells_tiles(E, {cfg}, {seed}, &cells, off);
     if (s != FG_OK) return s;
     compact_cells(cells.data(), cells.size(), pts);
     return FG_OK;
@@ -1167,6 +1171,125 @@ struct fg_frame {
 
 extern "C" int64_t fg_frame_crystal_count(const fg_frame *F) { return F ? F->n_crystals : 0; }
 
+// Emulsions of a device-synthesised frame built on the device: dart-thrown centres of
+// every tile at once (synth.hip), compacted in cell order, each crystal's attributes
+// from its own Philox stream (emulsion.rs:160-207), and every tile's transport BVH
+// (BVH::build, bvh.rs:157-295) and leaf-order records written straight into the frame's
+// node and record arrays (tile k's nodes from 2 crystal_base).  The host keeps each
+// tile's crystals and leaf order (F->ems) for the batched input and the read-backs; no
+// BVH is built on the host.  Caller holds E->mu.
+static fg_status device_frame_emulsions(fg_engine *E, fg_frame *F, const std::vector<fg_emulsion_config> &cfgs,
+                                        const std::vector<uint64_t> &seeds)
+{
+    const size_t nt = F->tiles.size();
+    std::vector<size_t> off;
+    fg_status st = device_cells_tiles(E, cfgs, seeds, nullptr, off);
+    if (st != FG_OK) return st;
+    std::vector<SynthTile> tiles(nt);
+    for (size_t k = 0; k < nt; k++) {
+        tiles[k] = SynthTile{};
+        tiles[k].cell_off = off[k];
+        tiles[k].n_cells = (uint32_t)(off[k + 1] - off[k]);
+        tiles[k].seed = seeds[k];
+        tiles[k].rmax = cfgs[k].r_min * 0.48;
+    }
+    DevBuf dtiles, dcounts;
+    HIP_TRY(dtiles.ensure(nt * sizeof(SynthTile)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(dcounts.ensure(nt * sizeof(uint32_t)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(hipMemcpyAsync(dtiles.p, tiles.data(), nt * sizeof(SynthTile), hipMemcpyHostToDevice, E->stream),
+            FG_ERR_BUFFER_MAP);
+    HIP_TRY(launch_synth_count(E->synth_cells.as<float4>(), dtiles.as<SynthTile>(), (uint32_t)nt,
+                               dcounts.as<uint32_t>(), E->stream), FG_ERR_PIPELINE);
+    std::vector<uint32_t> counts(nt);
+    HIP_TRY(hipMemcpyAsync(counts.data(), dcounts.p, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, E->stream),
+            FG_ERR_BUFFER_MAP);
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    uint64_t n = 0;
+    TransportBuildIn in;
+    for (size_t k = 0; k < nt; k++) {
+        if (n + counts[k] >= (1ull << 31)) return fail(FG_ERR_INVALID_ARG, "device synthesis: too many crystals");
+        tiles[k].crystal_base = (uint32_t)n;
+        in.base.push_back((uint32_t)n);
+        in.count.push_back(counts[k]);
+        n += counts[k];
+    }
+    if (!batched_fits(n, 2 * n)) return fail(FG_ERR_INVALID_ARG, "device synthesis: node or record arrays of 4 GiB or more");
+    HIP_TRY(hipMemcpyAsync(dtiles.p, tiles.data(), nt * sizeof(SynthTile), hipMemcpyHostToDevice, E->stream),
+            FG_ERR_BUFFER_MAP);
+    DevBuf pos, dims, quat, sens, leaf;
+    HIP_TRY(pos.ensure(3 * n * sizeof(double)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(dims.ensure(3 * n * sizeof(double)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(quat.ensure(4 * n * sizeof(double)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(sens.ensure(n * sizeof(double)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(leaf.ensure(n * sizeof(uint32_t)), FG_ERR_BUFFER_RANGE);
+    const fg_emulsion_config &c0 = cfgs.empty() ? F->desc.emulsion : cfgs[0];
+    SynthAttrs A{};
+    A.mu = log(c0.mean_crystal_radius);
+    A.size_sd = c0.size_variance;
+    A.min_radius = c0.min_radius;
+    A.base_sens = c0.base_sensitivity;
+    A.sens_sd = c0.sensitivity_sigma;
+    A.aspect = c0.tabular_aspect_ratio;
+    A.min_half_thickness = c0.min_half_thickness;
+    A.morph = c0.morphology;
+    HIP_TRY(launch_synth_fill(E->synth_cells.as<float4>(), dtiles.as<SynthTile>(), (uint32_t)nt, A, pos.as<double>(),
+                              dims.as<double>(), quat.as<double>(), sens.as<double>(), E->stream), FG_ERR_PIPELINE);
+    HIP_TRY(F->d_nodes.ensure(2 * n * sizeof(Node)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(F->d_recs.ensure(n * sizeof(Crystal)), FG_ERR_BUFFER_RANGE);
+    in.pos = pos.as<double>();
+    in.dims = dims.as<double>();
+    in.quat = quat.as<double>();
+    in.sens = sens.as<double>();
+    in.morph = c0.morphology;
+    in.n = (int64_t)n;
+    std::vector<uint32_t> node_count;
+    uint32_t err[4];
+    HIP_TRY(build_transport_bvhs(in, F->d_nodes.as<Node>(), F->d_recs.as<Crystal>(), leaf.as<uint32_t>(), node_count,
+                                 err, E->stream), FG_ERR_PIPELINE);
+    if (err[0] || err[1] || err[2])
+        return fail(FG_ERR_PIPELINE, "device synthesis: transport BVH exceeds the reference stack depth or leaf size");
+    std::vector<double> hp(3 * n), hd(3 * n), hq(4 * n), hs(n);
+    std::vector<uint32_t> hl(n);
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(hp.data(), pos.p, 3 * n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipMemcpyAsync(hd.data(), dims.p, 3 * n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipMemcpyAsync(hq.data(), quat.p, 4 * n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipMemcpyAsync(hs.data(), sens.p, n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipMemcpyAsync(hl.data(), leaf.p, n * 4, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+    }
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    for (size_t k = 0; k < nt; k++) {
+        TileRec &t = F->tiles[k];
+        Emulsion &em = F->ems[k];
+        const size_t b = in.base[k], m = in.count[k];
+        em.cfg = cfgs[k];
+        em.n = (int64_t)m;
+        em.pos.assign(hp.begin() + 3 * b, hp.begin() + 3 * (b + m));
+        em.dims.assign(hd.begin() + 3 * b, hd.begin() + 3 * (b + m));
+        em.quat.assign(hq.begin() + 4 * b, hq.begin() + 4 * (b + m));
+        em.sens.assign(hs.begin() + b, hs.begin() + b + m);
+        em.morph.assign(m, (int32_t)c0.morphology);
+        em.counts.assign(m, 0u);
+        em.developed.assign(m, 0u);
+        em.fe.assign(m, 1.0);
+        em.clump.assign(m, -1);
+        em.bvh = RefBvh{};
+        em.bvh.idx.resize(m);
+        for (size_t j = 0; j < m; j++) em.bvh.idx[j] = hl[b + j] - (uint32_t)b;
+        double mx = 0.0;
+        if (c0.morphology == 0)
+            for (size_t j = 0; j < 3 * m; j++) mx = std::max(mx, em.dims[j]);
+        em.reach = (sqrt(3.0) - 1.0) * mx;
+        t.crystal_base = (int64_t)b;
+        t.n = (int64_t)m;
+        t.node_base = 2 * (int64_t)b;
+        t.n_nodes = node_count[k];
+    }
+    F->n_crystals = (int64_t)n;
+    F->n_nodes = 2 * (int64_t)n;
+    return FG_OK;
+}
+
 extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg_frame **out)
 {
     if (!E || !desc || !out || !desc->exposure_map)
@@ -1224,11 +1347,9 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
         delete F;
         return fail(FG_ERR_INVALID_ARG, "fg_frame_create: synthesis must be 0 (host) or 1 (device)");
     }
-    std::vector<float4> dcells;
-    std::vector<size_t> doff;
-    if (desc->synthesis == 1) {
-        // Poisson-disk centres of every tile on the device at once; compaction,
-        // attributes and BVHs on the host threads below
+    const bool device_built = desc->synthesis == 1;
+    if (device_built) {
+        // every tile's emulsion and transport BVH on the device (device_frame_emulsions)
         std::vector<fg_emulsion_config> cfgs;
         std::vector<uint64_t> seeds;
         for (size_t k = 0; k < F->tiles.size(); k++) {
@@ -1241,56 +1362,49 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
             delete F;
             return fail(FG_ERR_DEVICE_REQUEST, std::string("hipSetDevice: ") + hipGetErrorString(e));
         }
-        fg_status s = device_cells_tiles(E, cfgs, seeds, dcells, doff);
+        fg_status s = device_frame_emulsions(E, F, cfgs, seeds);
         if (s != FG_OK) {
-            delete F;
+            fg_frame_destroy(F);
             return s;
         }
     }
-    std::atomic<size_t> next{0};
-    std::atomic<bool> bad{false};
-    std::string err;
-    std::mutex emu;
-    unsigned nt = desc->threads ? desc->threads : std::max(1u, std::thread::hardware_concurrency());
-    nt = std::min<unsigned>(nt, (unsigned)F->tiles.size());
-    auto work = [&]() {
-        for (;;) {
-            const size_t k = next.fetch_add(1);
-            if (k >= F->tiles.size()) break;
-            const TileRec &t = F->tiles[k];
-            const fg_emulsion_config c = tile_cfg(k);
-            if (desc->synthesis == 1) {
-                Emulsion &em = F->ems[k];
-                compact_cells(dcells.data() + doff[k], doff[k + 1] - doff[k], em.pos);
-                em.cfg = c;
-                em.n = (int64_t)(em.pos.size() / 3);
-                attributes_per_point(c, t.seed, em);
-            } else {
-                synthesize(c, t.seed, F->ems[k]);
+    if (!device_built) {
+        // the reference's Bridson sampler and BVH per tile on the host threads
+        std::atomic<size_t> next{0};
+        std::atomic<bool> bad{false};
+        std::string err;
+        std::mutex emu;
+        unsigned nt = desc->threads ? desc->threads : std::max(1u, std::thread::hardware_concurrency());
+        nt = std::min<unsigned>(nt, (unsigned)F->tiles.size());
+        auto work = [&]() {
+            for (;;) {
+                const size_t k = next.fetch_add(1);
+                if (k >= F->tiles.size()) break;
+                synthesize(tile_cfg(k), F->tiles[k].seed, F->ems[k]);
+                std::string m;
+                if (!finalize_emulsion(F->ems[k], m)) {
+                    std::lock_guard<std::mutex> g(emu);
+                    bad = true;
+                    err = m;
+                }
             }
-            std::string m;
-            if (!finalize_emulsion(F->ems[k], m)) {
-                std::lock_guard<std::mutex> g(emu);
-                bad = true;
-                err = m;
-            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned i = 0; i < nt; i++) pool.emplace_back(work);
+        for (auto &th : pool) th.join();
+        if (bad) {
+            delete F;
+            return fail(FG_ERR_INVALID_ARG, "fg_frame_create: " + err);
         }
-    };
-    std::vector<std::thread> pool;
-    for (unsigned i = 0; i < nt; i++) pool.emplace_back(work);
-    for (auto &th : pool) th.join();
-    if (bad) {
-        delete F;
-        return fail(FG_ERR_INVALID_ARG, "fg_frame_create: " + err);
-    }
-    for (size_t k = 0; k < F->tiles.size(); k++) {
-        TileRec &t = F->tiles[k];
-        t.crystal_base = F->n_crystals;
-        t.node_base = F->n_nodes;
-        t.n = F->ems[k].n;
-        t.n_nodes = (uint32_t)F->ems[k].nodes.size();
-        F->n_crystals += t.n;
-        F->n_nodes += t.n_nodes;
+        for (size_t k = 0; k < F->tiles.size(); k++) {
+            TileRec &t = F->tiles[k];
+            t.crystal_base = F->n_crystals;
+            t.node_base = F->n_nodes;
+            t.n = F->ems[k].n;
+            t.n_nodes = (uint32_t)F->ems[k].nodes.size();
+            F->n_crystals += t.n;
+            F->n_nodes += t.n_nodes;
+        }
     }
     std::lock_guard<std::mutex> lk(E->mu);
     auto cleanup = [&](fg_status s) { fg_frame_destroy(F); return s; };
@@ -1305,7 +1419,7 @@ extern "C" fg_status fg_frame_create(fg_engine *E, const fg_frame_desc *desc, fg
         e = fg::film_layout_upload(desc->width, desc->height, desc->tile, ov, F->d_layout, F->layout, E->stream);
     if (e == hipSuccess) e = F->d_stats.ensure(64);
     if (e != hipSuccess) return cleanup(fail(FG_ERR_BUFFER_RANGE, std::string("frame allocation: ") + hipGetErrorString(e)));
-    for (size_t k = 0; k < F->tiles.size() && e == hipSuccess; k++) {
+    for (size_t k = 0; k < F->tiles.size() && e == hipSuccess && !device_built; k++) {
         const TileRec &t = F->tiles[k];
         if (t.n_nodes)
             e = hipMemcpy(F->d_nodes.as<Node>() + t.node_base, F->ems[k].nodes.data(), t.n_nodes * sizeof(Node),
@@ -1577,6 +1691,34 @@ extern "C" fg_status fg_frame_read_counts(fg_engine *E, const fg_frame *F, int64
         const Emulsion &em = F->ems[k];
         for (int64_t j = 0; j < t.n; j++) counts[t.crystal_base + em.bvh.idx[(size_t)j]] = c[(size_t)(t.crystal_base + j)];
     }
+    return FG_OK;
+}
+
+extern "C" fg_status fg_frame_get_tile(fg_engine *E, const fg_frame *F, int64_t tile, int64_t *n, int64_t *n_nodes,
+                                     double *pos, double *dims, double *quat, int32_t *morph, double *sens,
+                                     void *nodes, void *records)
+{
+    if (!E || !F || tile < 0 || tile >= (int64_t)F->tiles.size())
+        return fail(FG_ERR_INVALID_ARG, "fg_frame_get_tile: bad argument");
+    const TileRec &t = F->tiles[(size_t)tile];
+    const Emulsion &em = F->ems[(size_t)tile];
+    if (n) *n = t.n;
+    if (n_nodes) *n_nodes = t.n_nodes;
+    const size_t m = (size_t)t.n;
+    if (pos) memcpy(pos, em.pos.data(), 3 * m * sizeof(double));
+    if (dims) memcpy(dims, em.dims.data(), 3 * m * sizeof(double));
+    if (quat) memcpy(quat, em.quat.data(), 4 * m * sizeof(double));
+    if (morph) memcpy(morph, em.morph.data(), m * sizeof(int32_t));
+    if (sens) memcpy(sens, em.sens.data(), m * sizeof(double));
+    if (!nodes && !records) return FG_OK;
+    std::lock_guard<std::mutex> lk(E->mu);
+    HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    if (nodes && t.n_nodes)
+        HIP_TRY(hipMemcpy(nodes, F->d_nodes.as<Node>() + t.node_base, (size_t)t.n_nodes * sizeof(Node),
+                          hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+    if (records && m)
+        HIP_TRY(hipMemcpy(records, F->d_recs.as<Crystal>() + t.crystal_base, m * sizeof(Crystal),
+                          hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
     return FG_OK;
 }
 

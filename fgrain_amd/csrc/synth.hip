@@ -57,7 +57,142 @@ __global__ void __launch_bounds__(256) darts_tiles_kernel(const DartArgs *tiles,
     dart(A, i0, j0, k0, round, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// ------------------------------------------------- compaction and crystal attributes
+constexpr double kTauD = 6.283185307179586, kPiD = 3.141592653589793;
+
+// The per-point stream of device synthesis: ctr = (lo32(point), hi32(point), block,
+// kTagAttr), words consumed in order (as the host form in host.cpp draws them).
+struct PointStream {
+    uint32_t k0, k1, p0, p1, block = 0, buf[4];
+    int pos = 4;
+    __device__ PointStream(uint64_t seed, uint64_t p)
+        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), p0((uint32_t)p), p1((uint32_t)(p >> 32)) {}
+    __device__ uint32_t u32()
+    {
+        if (pos == 4) {
+            buf[0] = p0;
+            buf[1] = p1;
+            buf[2] = block++;
+            buf[3] = kTagAttr;
+            philox(buf, k0, k1);
+            pos = 0;
+        }
+        return buf[pos++];
+    }
+    __device__ double uniform()
+    {
+        const uint64_t lo = u32(), hi = u32();
+        return (double)((lo | (hi << 32)) >> 11) * 0x1p-53;
+    }
+    __device__ double normal(double mean, double sd)
+    {
+        const double u1 = 1.0 - uniform(), u2 = uniform();
+        return mean + sd * (sqrt(-2.0 * log(u1)) * cos(kTauD * u2));
+    }
+};
+
+// emulsion.rs:160-207 for one point: radius, sensitivity, orientation, semi-axes
+__device__ void point_attributes(const SynthAttrs &A, double rmax, uint64_t seed, uint64_t p, double *d,
+                                 double *q, double &sens)
+{
+    PointStream rng(seed, p);
+    double rad = exp(rng.normal(A.mu, A.size_sd));
+    rad = fmin(fmax(rad, A.min_radius), rmax);
+    sens = fmax(rng.normal(A.base_sens, A.sens_sd), 0.1);
+    if (A.morph == 0u || A.morph == 2u) {
+        const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+        // UnitQuaternion::from_euler_angles(2 pi u1, pi u2, 2 pi u3)
+        const double ro = u1 * kTauD * 0.5, pi_ = u2 * kPiD * 0.5, ya = u3 * kTauD * 0.5;
+        const double sr = sin(ro), cr = cos(ro), sp = sin(pi_), cp = cos(pi_), sy = sin(ya), cy = cos(ya);
+        q[0] = cr * cp * cy + sr * sp * sy;
+        q[1] = sr * cp * cy - cr * sp * sy;
+        q[2] = cr * sp * cy + sr * cp * sy;
+        q[3] = cr * cp * sy - sr * sp * cy;
+        d[0] = d[1] = d[2] = rad;
+    } else {
+        const double ht = fmax(rad / A.aspect, A.min_half_thickness);
+        const double tilt = -0.17 + 0.34 * rng.uniform();
+        double ax = -1.0 + 2.0 * rng.uniform(), ay = -1.0 + 2.0 * rng.uniform(), az = 0.0;
+        const double nn = sqrt(ax * ax + ay * ay + az * az);
+        ax /= nn;
+        ay /= nn;
+        az /= nn;
+        const double sh = sin(tilt * 0.5), ch = cos(tilt * 0.5);
+        q[0] = ch;
+        q[1] = ax * sh;
+        q[2] = ay * sh;
+        q[3] = az * sh;
+        d[0] = rad;
+        d[1] = rad;
+        d[2] = ht;
+    }
+}
+
+__global__ void __launch_bounds__(256) synth_count_kernel(const float4 *cells, const SynthTile *tiles, uint32_t *counts)
+{
+    __shared__ uint32_t wc[4];
+    const SynthTile T = tiles[blockIdx.x];
+    uint32_t c = 0;
+    for (uint32_t i = threadIdx.x; i < T.n_cells; i += 256) c += cells[T.cell_off + i].w != 0.0f ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o, 64);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ void __launch_bounds__(256) synth_fill_kernel(const float4 *cells, const SynthTile *tiles, SynthAttrs A,
+                                                         double *pos, double *dims, double *quat, double *sens)
+{
+    __shared__ uint32_t wc[4];
+    const SynthTile T = tiles[blockIdx.x];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t running = 0;
+    for (uint32_t c0 = 0; c0 < T.n_cells; c0 += 256) {
+        const uint32_t i = c0 + threadIdx.x;
+        const float4 cl = i < T.n_cells ? cells[T.cell_off + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool has = cl.w != 0.0f;
+        const unsigned long long m = __ballot(has);
+        if (lane == 0) wc[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (uint32_t w = 0; w < 4; w++) {
+            if (w < wave) before += wc[w];
+            tot += wc[w];
+        }
+        if (has) {
+            const uint32_t k = running + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            const size_t sl = (size_t)T.crystal_base + k;
+            pos[3 * sl] = (double)cl.x;
+            pos[3 * sl + 1] = (double)cl.y;
+            pos[3 * sl + 2] = (double)cl.z;
+            double sv;
+            point_attributes(A, T.rmax, T.seed, k, &dims[3 * sl], &quat[4 * sl], sv);
+            sens[sl] = sv;
+        }
+        running += tot;
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+hipError_t launch_synth_count(const float4 *cells, const SynthTile *tiles, uint32_t n_tiles, uint32_t *counts,
+                              hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launcher's launches
+    if (n_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(synth_count_kernel, dim3(n_tiles), dim3(256), 0, s, cells, tiles, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_synth_fill(const float4 *cells, const SynthTile *tiles, uint32_t n_tiles, const SynthAttrs &A,
+                             double *pos, double *dims, double *quat, double *sens, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launcher's launches
+    if (n_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(synth_fill_kernel, dim3(n_tiles), dim3(256), 0, s, cells, tiles, A, pos, dims, quat, sens);
+    return hipGetLastError();
+}
 
 hipError_t launch_darts_tiles(const DartArgs *tiles, uint32_t n_tiles, uint32_t max_phase_cells,
                               uint32_t rounds, hipStream_t s)

@@ -210,6 +210,23 @@ class Frame:
                                                 ptr(f, L._f64p), ptr(c, L._i64p)))
         return d, f, c
 
+    def tile_data(self, k: int) -> dict:
+        """Tile k's crystals (crystal order) and the device layout its transport reads:
+        "nodes" (n_nodes x 8 u32 words, 32-byte threaded nodes), "records" (n x 12 words,
+        48-byte records in leaf order)."""
+        n, nn = C.c_int64(), C.c_int64()
+        check(L.lib().fg_frame_get_tile(self.engine._h, self._h, k, C.byref(n), C.byref(nn), None, None, None, None,
+                                        None, None, None))
+        m = n.value
+        out = dict(pos=np.zeros((m, 3)), dims=np.zeros((m, 3)), quat=np.zeros((m, 4)), morph=np.zeros(m, np.int32),
+                   sens=np.zeros(m), nodes=np.zeros((nn.value, 8), np.uint32), records=np.zeros((m, 12), np.uint32))
+        check(L.lib().fg_frame_get_tile(self.engine._h, self._h, k, None, None, ptr(out["pos"], L._f64p),
+                                        ptr(out["dims"], L._f64p), ptr(out["quat"], L._f64p),
+                                        ptr(out["morph"], L._i32p), ptr(out["sens"], L._f64p),
+                                        out["nodes"].ctypes.data_as(C.c_void_p),
+                                        out["records"].ctypes.data_as(C.c_void_p)))
+        return out
+
     def tile_ranges(self):
         """(x0, y0, tw, th, seed) per tile, in frame order (image_pipeline.rs:397-434)."""
         out = []

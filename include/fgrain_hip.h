@@ -381,6 +381,15 @@ fg_status fg_frame_read_counts(fg_engine *engine, const fg_frame *frame, int64_t
                                uint32_t *counts);
 /* Development of the last render (developed flag, filament expansion, clump id or -1),
  * crystals in tile order then the tile's crystal order. */
+/* Tile `tile` of a frame (row-major tile order): its crystal count n, its transport BVH's
+ * node count n_nodes, its crystals in crystal order (pos, dims = semi-axes, quat = (w,i,j,k),
+ * morph, sens; NULL to skip) and the device layout the transport reads (nodes: n_nodes
+ * 32-byte threaded nodes; records: n 48-byte records in leaf order; NULL to skip), as
+ * fg_emulsion_get_device_layout gives them for an emulsion.  For frames synthesised on
+ * the device these come from the device build. */
+fg_status fg_frame_get_tile(fg_engine *engine, const fg_frame *frame, int64_t tile, int64_t *n,
+                            int64_t *n_nodes, double *pos, double *dims, double *quat, int32_t *morph,
+                            double *sens, void *nodes, void *records);
 fg_status fg_frame_read_development(fg_engine *engine, const fg_frame *frame, int64_t n,
                                     uint8_t *developed, double *fe, int64_t *clump_id);
 int64_t fg_frame_crystal_count(const fg_frame *frame);

@@ -56,3 +56,40 @@ def test_device_synthesised_frame_renders_identically_on_both_paths(engine):
     assert np.array_equal(cp, fd.counts())
     assert np.abs(Dp - fd.read()).max() <= 1e-5
     assert sp["developed"] == sb["developed"] > 0
+
+
+@pytest.mark.parametrize("workload", ["frame24mp_trix_cubic", "frame24mp_tmax_tabular"])
+def test_device_built_frame_tiles_equal_host_builds(engine, workload):
+    """Frames synthesised on the device get every tile's crystals, transport BVH and
+    records from the device (compaction, attributes, BVH::build of bvh.rs:157-295 with the
+    reference's partition order): per tile, the threaded nodes and leaf-order records equal
+    word for word the host build over the same crystals (build_ref_bvh and its layout;
+    T-Max tiles with uneven densities and tabular crystals), and the crystals are the oracle
+    twin's darts with the attribute draws of emulsion.rs:160-207 (f64 transcendentals of
+    the device's libm: within 1e-13 relative of the host's)."""
+    cfg = copy.deepcopy(W.load_config(workload))
+    cfg["exposure"]["photons_per_pixel"] = 4
+    w, h = 700, 420   # 3 x 2 tiles, partial edge tiles
+    fr = W.Frame(engine, cfg, W.blob_exposure(w, h, 3, sigma=(40, 160)), frame=(w, h), synthesis="device")
+    mult = fr._mult
+    e = cfg["emulsion"]
+    for k, (x0, y0, tw, th, seed) in enumerate(fr.tile_ranges()):
+        t = fr.tile_data(k)
+        assert len(t["pos"]) > 1000
+        ec = W.emulsion_config(cfg, float(tw), float(th))
+        em = F.VirtualEmulsionLayer.from_crystals(ec, t["pos"], t["dims"], t["quat"], t["morph"], t["sens"])
+        nodes, recs = em.device_layout()
+        assert nodes.shape == t["nodes"].shape and np.array_equal(nodes, t["nodes"]), k
+        assert np.array_equal(recs, t["records"]), k
+        r_min = e["r_min"] / np.cbrt(max(float(mult[k]), 1e-3)) if mult is not None else e["r_min"]
+        ref = O.synthesize_darts(W=float(tw), H=float(th), D=e["depth"], r_min=r_min,
+                                 mean_radius=e["mean_crystal_radius"], size_sigma=e["size_variance"],
+                                 morph=e["morphology"], aspect=e["tabular_aspect_ratio"],
+                                 base_sens=e["base_sensitivity"], sens_sigma=e["sensitivity_sigma"],
+                                 min_radius=e["min_radius"], min_half_thickness=e["min_half_thickness"],
+                                 k_samples=e["k_samples"], seed=seed)
+        assert np.array_equal(t["pos"], ref.pos), k
+        for key, v in (("dims", ref.dims), ("quat", ref.quat), ("sens", ref.sens)):
+            np.testing.assert_allclose(t[key], v, rtol=1e-13, atol=1e-15, err_msg=f"{key} tile {k}")
+    st = fr.render(W.Frame.BATCHED)
+    assert st["graph_used"] == 1 and 0 < st["developed"] < st["crystals"]
