@@ -2,6 +2,7 @@
 // layout of overlapping tiles, the optics and compositing kernels' launchers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace fg {
@@ -32,9 +33,21 @@ struct FilmParams {
     const float *kern;    // 2 rad + 1 normalised Gaussian weights
 };
 
+// Both structs are kernel arguments built by host code and read by device code, which
+// are compiled apart: their layout is pinned here for both compiles.
+static_assert(sizeof(FilmLayout) == 88 && offsetof(FilmLayout, wext) == 24 && offsetof(FilmLayout, xs) == 32 &&
+                  offsetof(FilmLayout, ramp) == 80,
+              "FilmLayout layout");
+static_assert(sizeof(FilmParams) == 32 && offsetof(FilmParams, rad) == 16 && offsetof(FilmParams, kern) == 24,
+              "FilmParams layout");
+
 // The Hann blend kernel belongs to the compositing kernels that are switched off (a
-// device memory fault on their first launch, cause not yet found).
-constexpr bool kFilmBlendEnabled = false;
+// device memory fault on their first launch, cause not yet found); FG_FILM_COMPOSITE=1
+// at compile time switches it on.
+#ifndef FG_FILM_COMPOSITE
+#define FG_FILM_COMPOSITE 0
+#endif
+constexpr bool kFilmBlendEnabled = FG_FILM_COMPOSITE != 0;
 
 // span of tile t along one axis (image_pipeline.rs:400-410)
 inline void film_span(uint32_t t, uint32_t step, uint32_t pad, uint32_t size, uint32_t &start, uint32_t &len)

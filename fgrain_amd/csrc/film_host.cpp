@@ -19,11 +19,11 @@ using Clock = std::chrono::steady_clock;
 
 // The compositing kernels (Hann blend, Gaussian separation, statistics, grain and
 // output) are switched off: their first launch on the MI355X ended in a device memory
-// fault whose cause is not yet found.  The kernels' source built for the host against a
-// HIP shim matches the oracle bit for bit; that build does not cover this file's host
-// code.  The entry points that launch them return FG_ERR_PIPELINE until the cause is
-// fixed.
-constexpr bool kCompositeEnabled = false;
+// fault whose cause is not yet found.  The entry points that launch them return
+// FG_ERR_PIPELINE until the cause is fixed.  FG_FILM_COMPOSITE=1 at compile time switches
+// them on (film.h); the CPU harness in tests/ builds this file and the kernels that way
+// for the host, under AddressSanitizer.
+constexpr bool kCompositeEnabled = FG_FILM_COMPOSITE != 0;
 constexpr const char *kCompositeOff =
     "the compositing kernels are switched off in this build (device memory fault under investigation)";
 
