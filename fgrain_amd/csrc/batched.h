@@ -36,6 +36,11 @@ struct BatchedInput {
     // host arrays in slot order (per tile: slot k holds crystal bvh.idx[k])
     std::vector<double> pos64, dims64;
     std::vector<uint32_t> orig;
+    // or, for crystals already on the device (crystal order, tile t's from its
+    // crystal_base), their attributes and each slot's crystal index within its tile;
+    // then the tiles' maxb are taken as given (an upper bound of their largest semi-axis)
+    const double *dev_pos = nullptr, *dev_dims = nullptr;
+    const uint32_t *dev_leaf = nullptr;
     TransportArgs transport;        // optics and shared fields; per-tile fields overwritten
     uint32_t transport_waves = 6;   // occupancy of the transport kernel: 6 or 7 waves per SIMD
     DevelopParams develop;
@@ -64,8 +69,8 @@ void batched_destroy(BatchedFrame *b);
 // the batched frame's level kernels: tile t's crystals are slots base[t] .. base[t] +
 // count[t] in crystal order (pos, dims 3 per slot; quat 4; sens 1; one morphology).
 // Writes tile t's threaded nodes from nodes + 2 base[t] (node_count[t] of them), its
-// records in leaf order from recs + base[t], and leaf_slot[base[t] + k] = the slot of
-// leaf position k.  err: [0] a node deeper than the reference's stack, [1] a leaf of more
+// records in leaf order from recs + base[t], and leaf_slot[base[t] + k] = the index
+// within the tile of the crystal at leaf position k.  err: [0] a node deeper than the reference's stack, [1] a leaf of more
 // than 15 crystals, [2] an unfinished build.  Synchronises s.
 struct TransportBuildIn {
     const double *pos, *dims, *quat, *sens;

@@ -1006,9 +1006,36 @@ __global__ void __launch_bounds__(256) k_tb_init(Bufs B, uint64_t n, uint32_t n_
     }
 }
 
+// The batched input's slot arrays from device-resident crystals: slot sl of tile t holds
+// the tile's crystal leaf[sl] (an index within the tile), whose attributes sit at
+// crystal_base + leaf[sl] of pos / dims (crystal order).
+__global__ void __launch_bounds__(256) k_gather_slots(Bufs B, const double *pos, const double *dims,
+                                                      const uint32_t *leaf, uint64_t n, uint32_t n_tiles)
+{
+    const uint64_t sl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= n) return;
+    uint32_t lo = 0, hi = n_tiles;   // last tile with crystal_base <= sl
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (B.tiles[mid].crystal_base <= sl) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t local = leaf[sl];
+    const size_t cs = (size_t)B.tiles[lo].crystal_base + local;
+    double m = 0.0;
+    for (int a = 0; a < 3; a++) {
+        B.pos64[3 * sl + a] = pos[3 * cs + a];
+        B.dims64[3 * sl + a] = dims[3 * cs + a];
+        m = fmax(m, dims[3 * cs + a]);
+    }
+    B.mdim[sl] = m;
+    B.orig[sl] = local;
+    B.tile_of[sl] = lo;
+}
+
 // transport records in leaf order (HalideCrystal fields as f32, make_records in host.cpp)
 __global__ void __launch_bounds__(256) k_tb_records(Bufs B, const double *quat, const double *sens, uint32_t morph,
-                                                    Crystal *recs, uint64_t n)
+                                                    Crystal *recs, uint32_t *leaf_local, uint64_t n)
 {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -1022,6 +1049,7 @@ __global__ void __launch_bounds__(256) k_tb_records(Bufs B, const double *quat, 
     r.sens = (float)sens[s];
     r.morph = morph;
     recs[e] = r;
+    leaf_local[e] = B.orig[s];   // the crystal's index within its tile
 }
 
 }  // namespace
@@ -1128,6 +1156,7 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
         T.blocks = (uint32_t)(((uint64_t)T.tw * T.th + ppb - 1) / ppb);
         T.block_base = blocks;
         blocks += T.blocks;
+        if (in.dev_pos) continue;   // T.maxb as given (gathered on the device below)
         double mx = 0.0;
         for (uint32_t k = 0; k < T.n; k++) {
             const size_t sl = (size_t)T.crystal_base + k;
@@ -1141,12 +1170,19 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     b->total_items = items;
     b->total_blocks = blocks;
     b->in.tiles = tiles;
-    TRY(hipMemcpyAsync(B.pos64, in.pos64.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
-    TRY(hipMemcpyAsync(B.dims64, in.dims64.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
-    TRY(hipMemcpyAsync(B.mdim, mdim.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-    TRY(hipMemcpyAsync(B.orig, in.orig.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    TRY(hipMemcpyAsync(B.tile_of, tile_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     TRY(hipMemcpyAsync(B.tiles, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, s));
+    if (in.dev_pos) {
+        (void)hipGetLastError();
+        hipLaunchKernelGGL(k_gather_slots, dim3(std::max<uint32_t>(1u, (uint32_t)((n + 255) / 256))), dim3(256), 0, s,
+                           B, in.dev_pos, in.dev_dims, in.dev_leaf, (uint64_t)n, b->n_tiles);
+        TRY(hipGetLastError());
+    } else {
+        TRY(hipMemcpyAsync(B.pos64, in.pos64.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        TRY(hipMemcpyAsync(B.dims64, in.dims64.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        TRY(hipMemcpyAsync(B.mdim, mdim.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        TRY(hipMemcpyAsync(B.orig, in.orig.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        TRY(hipMemcpyAsync(B.tile_of, tile_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    }
     TRY(hipStreamSynchronize(s));
     int dev = 0, cus = 0, per_cu = 0;
     TRY(hipGetDevice(&dev));
@@ -1339,7 +1375,6 @@ hipError_t build_transport_bvhs(const TransportBuildIn &in, Node *nodes, Crystal
     B.pos64 = const_cast<double *>(in.pos);
     B.dims64 = const_cast<double *>(in.dims);
     B.dnodes = nodes;
-    B.final_elem = leaf_slot;
     hipError_t e = hipSuccess;
     auto A = [&](auto *&p, size_t count) {
         if (e == hipSuccess) e = alloc(&tmp, p, count);
@@ -1352,6 +1387,7 @@ hipError_t build_transport_bvhs(const TransportBuildIn &in, Node *nodes, Crystal
     A(B.ehi, 3 * n);
     A(B.elem[0], n);
     A(B.elem[1], n);
+    A(B.final_elem, n);
     A(B.bn, 2 * n);
     A(B.leafcnt, 2 * n);
     A(B.prank, n);
@@ -1387,7 +1423,8 @@ hipError_t build_transport_bvhs(const TransportBuildIn &in, Node *nodes, Crystal
         hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B);
         hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, (uint64_t)(2 * n));
         hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, (uint64_t)(2 * n));
-        hipLaunchKernelGGL(k_tb_records, dim3(nb), dim3(256), 0, s, B, in.quat, in.sens, in.morph, recs, (uint64_t)n);
+        hipLaunchKernelGGL(k_tb_records, dim3(nb), dim3(256), 0, s, B, in.quat, in.sens, in.morph, recs, leaf_slot,
+                           (uint64_t)n);
         e = hipGetLastError();
     }
     if (e == hipSuccess)

@@ -859,18 +859,22 @@ static void append_tile(BatchedInput &in, const Emulsion &em, const fg_exposure_
                         uint32_t node_base, uint32_t n_nodes, int64_t crystal_base, uint64_t out_base,
                         uint32_t out_stride)
 {
-    for (int64_t j = 0; j < em.n; j++) {
-        const size_t sl = (size_t)(crystal_base + j);
-        const uint32_t ci = em.bvh.idx[(size_t)j];
-        for (int a = 0; a < 3; a++) {
-            in.pos64[3 * sl + a] = em.pos[3 * (size_t)ci + a];
-            in.dims64[3 * sl + a] = em.dims[3 * (size_t)ci + a];
+    if (!em.on_device)
+        for (int64_t j = 0; j < em.n; j++) {
+            const size_t sl = (size_t)(crystal_base + j);
+            const uint32_t ci = em.bvh.idx[(size_t)j];
+            for (int a = 0; a < 3; a++) {
+                in.pos64[3 * sl + a] = em.pos[3 * (size_t)ci + a];
+                in.dims64[3 * sl + a] = em.dims[3 * (size_t)ci + a];
+            }
+            in.orig[sl] = ci;
         }
-        in.orig[sl] = ci;
-    }
     TransportArgs a{};
     fill_transport(a, em.cfg, x, tw, th, tile_seed + 1, em.reach, tn);
+    // one launch prunes every tile with one reach: the largest of the tiles' (a larger
+    // reach only prunes less, so every tile stays exact); negative = pruning off
     if (in.tiles.empty()) in.transport = a;
+    else if (in.transport.reach >= 0.0f) in.transport.reach = std::max(in.transport.reach, a.reach);
     TileDev td{};
     td.x0 = x0;
     td.y0 = y0;
@@ -891,6 +895,7 @@ static void append_tile(BatchedInput &in, const Emulsion &em, const fg_exposure_
     td.crystal_base = (uint32_t)crystal_base;
     td.n = (uint32_t)em.n;
     td.n_photons = a.n_photons;
+    td.maxb = em.rmax;   // used as given for crystals on the device only
     in.tiles.push_back(td);
 }
 
@@ -928,6 +933,10 @@ struct TileArrays {
     uint64_t out_base;
     uint32_t out_stride;
     unsigned long long *stats; // 8 transport counters
+    // crystals on the device (frames built there): attributes in crystal order and each
+    // leaf slot's crystal index; NULL = the emulsion's host arrays
+    const double *dev_pos = nullptr, *dev_dims = nullptr;
+    const uint32_t *dev_leaf = nullptr;
 };
 
 // A one-tile batched frame (batched.hip) over one tile's device arrays: the per-tile
@@ -950,9 +959,14 @@ static fg_status one_tile_batch(fg_engine *E, const Emulsion &em, const TileArra
     in.frame_w = d.estride;
     in.frame_h = th;
     in.n_crystals = em.n;
-    in.pos64.resize((size_t)em.n * 3);
-    in.dims64.resize((size_t)em.n * 3);
-    in.orig.resize((size_t)em.n);
+    in.dev_pos = d.dev_pos;
+    in.dev_dims = d.dev_dims;
+    in.dev_leaf = d.dev_leaf;
+    if (!in.dev_pos) {
+        in.pos64.resize((size_t)em.n * 3);
+        in.dims64.resize((size_t)em.n * 3);
+        in.orig.resize((size_t)em.n);
+    }
     append_tile(in, em, x, E->tuning, x0, y0, tw, th, tile_seed, 0u, d.n_nodes, 0, d.out_base, d.out_stride);
     batched_settings(in, dc, sc, E->tuning, em.cfg.depth);
     BatchedFrame *b = nullptr;
@@ -1167,6 +1181,10 @@ struct fg_frame {
     fg_tuning batched_tuning{};
     std::vector<BatchedFrame *> plain;   // one-tile frames of the plain path, per tile
     fg_tuning plain_tuning{};
+    // frames built on the device: crystal attributes (crystal order, tile t's from its
+    // crystal_base) and each leaf slot's crystal index within its tile
+    bool device_built = false;
+    DevBuf d_pos, d_dims, d_quat, d_sens, d_leaf;
 };
 
 extern "C" int64_t fg_frame_crystal_count(const fg_frame *F) { return F ? F->n_crystals : 0; }
@@ -1216,7 +1234,7 @@ static fg_status device_frame_emulsions(fg_engine *E, fg_frame *F, const std::ve
     if (!batched_fits(n, 2 * n)) return fail(FG_ERR_INVALID_ARG, "device synthesis: node or record arrays of 4 GiB or more");
     HIP_TRY(hipMemcpyAsync(dtiles.p, tiles.data(), nt * sizeof(SynthTile), hipMemcpyHostToDevice, E->stream),
             FG_ERR_BUFFER_MAP);
-    DevBuf pos, dims, quat, sens, leaf;
+    DevBuf &pos = F->d_pos, &dims = F->d_dims, &quat = F->d_quat, &sens = F->d_sens, &leaf = F->d_leaf;
     HIP_TRY(pos.ensure(3 * n * sizeof(double)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(dims.ensure(3 * n * sizeof(double)), FG_ERR_BUFFER_RANGE);
     HIP_TRY(quat.ensure(4 * n * sizeof(double)), FG_ERR_BUFFER_RANGE);
@@ -1248,26 +1266,15 @@ static fg_status device_frame_emulsions(fg_engine *E, fg_frame *F, const std::ve
                                  err, E->stream), FG_ERR_PIPELINE);
     if (err[0] || err[1] || err[2])
         return fail(FG_ERR_PIPELINE, "device synthesis: transport BVH exceeds the reference stack depth or leaf size");
-    std::vector<double> hp(3 * n), hd(3 * n), hq(4 * n), hs(n);
-    std::vector<uint32_t> hl(n);
-    if (n) {
-        HIP_TRY(hipMemcpyAsync(hp.data(), pos.p, 3 * n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
-        HIP_TRY(hipMemcpyAsync(hd.data(), dims.p, 3 * n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
-        HIP_TRY(hipMemcpyAsync(hq.data(), quat.p, 4 * n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
-        HIP_TRY(hipMemcpyAsync(hs.data(), sens.p, n * 8, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
-        HIP_TRY(hipMemcpyAsync(hl.data(), leaf.p, n * 4, hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
-    }
-    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    // the host keeps each tile's leaf order (for the read-backs); the attributes stay on
+    // the device, where the batched inputs gather them
     for (size_t k = 0; k < nt; k++) {
         TileRec &t = F->tiles[k];
         Emulsion &em = F->ems[k];
         const size_t b = in.base[k], m = in.count[k];
         em.cfg = cfgs[k];
         em.n = (int64_t)m;
-        em.pos.assign(hp.begin() + 3 * b, hp.begin() + 3 * (b + m));
-        em.dims.assign(hd.begin() + 3 * b, hd.begin() + 3 * (b + m));
-        em.quat.assign(hq.begin() + 4 * b, hq.begin() + 4 * (b + m));
-        em.sens.assign(hs.begin() + b, hs.begin() + b + m);
+        em.on_device = true;
         em.morph.assign(m, (int32_t)c0.morphology);
         em.counts.assign(m, 0u);
         em.developed.assign(m, 0u);
@@ -1275,16 +1282,20 @@ static fg_status device_frame_emulsions(fg_engine *E, fg_frame *F, const std::ve
         em.clump.assign(m, -1);
         em.bvh = RefBvh{};
         em.bvh.idx.resize(m);
-        for (size_t j = 0; j < m; j++) em.bvh.idx[j] = hl[b + j] - (uint32_t)b;
-        double mx = 0.0;
-        if (c0.morphology == 0)
-            for (size_t j = 0; j < 3 * m; j++) mx = std::max(mx, em.dims[j]);
-        em.reach = (sqrt(3.0) - 1.0) * mx;
+        if (m)
+            HIP_TRY(hipMemcpyAsync(em.bvh.idx.data(), leaf.as<uint32_t>() + b, m * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, E->stream), FG_ERR_BUFFER_MAP);
+        // the radius clamp r_min * 0.48 (emulsion.rs:166) bounds every semi-axis: with it
+        // the transport prunes by a reach no smaller than the crystals' own, so exactly
+        em.rmax = tiles[k].rmax;
+        em.reach = c0.morphology == 0 ? (sqrt(3.0) - 1.0) * em.rmax : 0.0;
         t.crystal_base = (int64_t)b;
         t.n = (int64_t)m;
         t.node_base = 2 * (int64_t)b;
         t.n_nodes = node_count[k];
     }
+    HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);
+    F->device_built = true;
     F->n_crystals = (int64_t)n;
     F->n_nodes = 2 * (int64_t)n;
     return FG_OK;
@@ -1444,7 +1455,7 @@ extern "C" void fg_frame_destroy(fg_frame *F)
     if (F->batched) batched_destroy(F->batched);
     for (BatchedFrame *b : F->plain) batched_destroy(b);
     for (DevBuf *b : {&F->d_nodes, &F->d_recs, &F->d_counts, &F->d_emap, &F->d_dneg, &F->d_dneg2,
-                      &F->d_stats, &F->d_layout})
+                      &F->d_stats, &F->d_layout, &F->d_pos, &F->d_dims, &F->d_quat, &F->d_sens, &F->d_leaf})
         b->release();
     delete F;
 }
@@ -1466,10 +1477,15 @@ static fg_status frame_render_plain(fg_engine *E, fg_frame *F, fg_frame_stats *s
         F->plain.assign(F->tiles.size(), nullptr);
         for (size_t k = 0; k < F->tiles.size(); k++) {
             const TileRec &t = F->tiles[k];
-            const TileArrays ta{F->d_nodes.as<Node>() + t.node_base, F->d_recs.as<Crystal>() + t.crystal_base,
-                                t.n_nodes, F->d_counts.as<uint32_t>() + t.crystal_base, F->d_emap.as<float>(),
-                                D.width, F->d_dneg.as<float>(), (uint64_t)t.out_base, t.out_stride,
-                                F->d_stats.as<unsigned long long>()};
+            TileArrays ta{F->d_nodes.as<Node>() + t.node_base, F->d_recs.as<Crystal>() + t.crystal_base,
+                          t.n_nodes, F->d_counts.as<uint32_t>() + t.crystal_base, F->d_emap.as<float>(),
+                          D.width, F->d_dneg.as<float>(), (uint64_t)t.out_base, t.out_stride,
+                          F->d_stats.as<unsigned long long>()};
+            if (F->device_built) {
+                ta.dev_pos = F->d_pos.as<double>() + 3 * t.crystal_base;
+                ta.dev_dims = F->d_dims.as<double>() + 3 * t.crystal_base;
+                ta.dev_leaf = F->d_leaf.as<uint32_t>() + t.crystal_base;
+            }
             fg_status s = one_tile_batch(E, F->ems[k], ta, t.x0, t.y0, t.tw, t.th, t.seed, D.exposure,
                                          D.development, D.scan, &F->plain[k]);
             if (s != FG_OK) return s;
@@ -1535,9 +1551,15 @@ static fg_status frame_render_batched(fg_engine *E, fg_frame *F, fg_frame_stats 
         in.frame_w = D.width;
         in.frame_h = D.height;
         in.n_crystals = F->n_crystals;
-        in.pos64.resize((size_t)F->n_crystals * 3);
-        in.dims64.resize((size_t)F->n_crystals * 3);
-        in.orig.resize((size_t)F->n_crystals);
+        if (F->device_built) {
+            in.dev_pos = F->d_pos.as<double>();
+            in.dev_dims = F->d_dims.as<double>();
+            in.dev_leaf = F->d_leaf.as<uint32_t>();
+        } else {
+            in.pos64.resize((size_t)F->n_crystals * 3);
+            in.dims64.resize((size_t)F->n_crystals * 3);
+            in.orig.resize((size_t)F->n_crystals);
+        }
         for (size_t k = 0; k < F->tiles.size(); k++) {
             const TileRec &t = F->tiles[k];
             append_tile(in, F->ems[k], D.exposure, E->tuning, t.x0, t.y0, t.tw, t.th, t.seed, (uint32_t)t.node_base,
@@ -1705,14 +1727,23 @@ extern "C" fg_status fg_frame_get_tile(fg_engine *E, const fg_frame *F, int64_t 
     if (n) *n = t.n;
     if (n_nodes) *n_nodes = t.n_nodes;
     const size_t m = (size_t)t.n;
-    if (pos) memcpy(pos, em.pos.data(), 3 * m * sizeof(double));
-    if (dims) memcpy(dims, em.dims.data(), 3 * m * sizeof(double));
-    if (quat) memcpy(quat, em.quat.data(), 4 * m * sizeof(double));
     if (morph) memcpy(morph, em.morph.data(), m * sizeof(int32_t));
-    if (sens) memcpy(sens, em.sens.data(), m * sizeof(double));
-    if (!nodes && !records) return FG_OK;
+    if (!em.on_device) {
+        if (pos) memcpy(pos, em.pos.data(), 3 * m * sizeof(double));
+        if (dims) memcpy(dims, em.dims.data(), 3 * m * sizeof(double));
+        if (quat) memcpy(quat, em.quat.data(), 4 * m * sizeof(double));
+        if (sens) memcpy(sens, em.sens.data(), m * sizeof(double));
+    }
+    if (!nodes && !records && !em.on_device) return FG_OK;
     std::lock_guard<std::mutex> lk(E->mu);
     HIP_TRY(hipSetDevice(E->device), FG_ERR_DEVICE_REQUEST);
+    if (em.on_device && m) {
+        const size_t b = (size_t)t.crystal_base;
+        if (pos) HIP_TRY(hipMemcpy(pos, F->d_pos.as<double>() + 3 * b, 3 * m * 8, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+        if (dims) HIP_TRY(hipMemcpy(dims, F->d_dims.as<double>() + 3 * b, 3 * m * 8, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+        if (quat) HIP_TRY(hipMemcpy(quat, F->d_quat.as<double>() + 4 * b, 4 * m * 8, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+        if (sens) HIP_TRY(hipMemcpy(sens, F->d_sens.as<double>() + b, m * 8, hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);
+    }
     if (nodes && t.n_nodes)
         HIP_TRY(hipMemcpy(nodes, F->d_nodes.as<Node>() + t.node_base, (size_t)t.n_nodes * sizeof(Node),
                           hipMemcpyDeviceToHost), FG_ERR_BUFFER_MAP);

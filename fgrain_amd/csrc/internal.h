@@ -38,6 +38,10 @@ struct Emulsion {
     DeviceEmulsion *dev = nullptr;
     bool dev_state_dirty = true;  // counts changed on the host since the last upload
     double reach = 0.0;           // (sqrt(3) - 1) * largest semi-axis of a cubic crystal
+    // a tile of a frame built on the device: pos / dims / quat / sens stay in the frame's
+    // device arrays (empty here), rmax bounds the largest semi-axis
+    bool on_device = false;
+    double rmax = 0.0;
 };
 
 // Philox sequential stream with the layout documented in fg_layout.h.
