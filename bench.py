@@ -341,6 +341,8 @@ def main():
                     help="emulsion synthesis of the frame (default: the workload's, host Bridson)")
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore fgrain_amd/configs/tuning/<workload>.json (autotuner output)")
+    ap.add_argument("--set-tuning", action="append", default=[], metavar="KEY=VALUE",
+                    help="override one tuning field after the workload's tuning is loaded")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (counter calibration)")
     ap.add_argument("--valu-probe", type=int, default=0, metavar="ITERS",
@@ -380,6 +382,15 @@ def main():
     if tuned:
         engine.set_tuning(**tuned["tuning"])
         log(f"tuning ({tuned['measured']}): {tuned['tuning']}")
+    if args.set_tuning:
+        t = engine.tuning()
+        for kv in args.set_tuning:
+            k, _, v = kv.partition("=")
+            if k not in t or not v:
+                raise SystemExit(f"bench.py: --set-tuning {kv!r}: expected one of {sorted(t)}=<int>")
+            t[k] = int(v)
+        engine.set_tuning(**t)
+        log(f"tuning overridden: {t}")
     if "fit" in cfg:
         return fit_main(args, engine, cfg)
     if "film" in cfg:
