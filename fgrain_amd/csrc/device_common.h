@@ -499,6 +499,26 @@ enum : uint32_t { kNeed = 0, kStart = 1, kWalk = 2, kDone = 3 };
 // queue 1961 / 1626 / 1514 / 1460 with depth 1 / 2 / 3 / 4 (30 VGPRs spilled at
 // depth 4); LDS queue 1415 / 1378 / 1363 / 1366 / 1369 with depth 4 / 8 / 12 / 16 / 24
 // (T-Max frame: 4027 / 3848 / 3809 / 3842 / 3865).  Every launch is 256 lanes per block.
+// The node test of the transport walk as one floating-point comparison: the node box
+// overlaps the segment box [s, h] (lo <= h and hi >= s on every axis) and box_hit holds
+// (tx >= te and tx > 0).  With denormals kept, a - b <= 0 exactly when a <= b for finite
+// a and b (and for the unbounded segment box of an unpruned walk), and tx >= max(te,
+// smallest denormal) exactly when tx >= te and tx > 0, so the maximum of the seven
+// differences is <= 0 exactly when both tests hold.
+__device__ __forceinline__ bool near_box_hit(const NodeView &nd, float s0, float s1, float s2, float h0, float h1,
+                                             float h2, float ox, float oy, float oz, float ix, float iy, float iz)
+{
+    const float t1x = (nd.lo[0] - ox) * ix, t2x = (nd.hi[0] - ox) * ix;
+    const float t1y = (nd.lo[1] - oy) * iy, t2y = (nd.hi[1] - oy) * iy;
+    const float t1z = (nd.lo[2] - oz) * iz, t2z = (nd.hi[2] - oz) * iz;
+    const float te = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+    const float tx = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+    const float a = fmaxf(fmaxf(nd.lo[0] - h0, nd.lo[1] - h1), nd.lo[2] - h2);
+    const float b = fmaxf(fmaxf(s0 - nd.hi[0], s1 - nd.hi[1]), s2 - nd.hi[2]);
+    const float c = fmaxf(te, __int_as_float(1)) - tx;
+    return fmaxf(fmaxf(a, b), c) <= 0.0f;
+}
+
 constexpr int kLeafQueue = 12;
 constexpr uint32_t kQueueLanes = 256;
 
@@ -673,7 +693,9 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 iy = 1.0f / dy;
                 iz = 1.0f / dz;
                 const float ex = ox + dx * step, ey = oy + dy * step, ez = oz + dz * step;
-                const float mg = C.reach + 1e-3f + 1e-5f * (fabsf(ox) + fabsf(oy) + fabsf(oz) + step);
+                // without pruning the segment box is unbounded and every node is near
+                const float mg = prune ? C.reach + 1e-3f + 1e-5f * (fabsf(ox) + fabsf(oy) + fabsf(oz) + step)
+                                       : __builtin_inff();
                 s0 = fminf(ox, ex) - mg;
                 h0 = fmaxf(ox, ex) + mg;
                 s1 = fminf(oy, ey) - mg;
@@ -695,25 +717,28 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             const uint32_t stop =
                 __builtin_amdgcn_readfirstlane((walking * (100u - min(C.wait_pct, 100u))) / 100u);
             uint32_t lim = phase == kWalk ? T.n_nodes : 0u;
-            for (uint32_t it = 0;; it++) {
-                const bool stepping = i < lim;
-                const uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(stepping));
-                if (ns == 0u || (it > 0u && ns <= stop)) break;
-                cnt.nodes += ns;
-                if (stepping) {
+            // the first step runs whenever a lane can step, later ones while more than stop
+            // lanes can (0 lanes is at most stop)
+            uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(i < lim));
+            uint32_t steps = 0;
+            while (ns != 0u) {
+                steps += ns;
+                if (i < lim) {
                     const NodeView nd = load_node_buf(nodes_r, T.node_base + i);
-                    const bool near = !prune || (nd.lo[0] <= h0 && nd.hi[0] >= s0 && nd.lo[1] <= h1 &&
-                                                 nd.hi[1] >= s1 && nd.lo[2] <= h2 && nd.hi[2] >= s2);
-                    const bool hit = near && box_hit(nd.lo, nd.hi, ox, oy, oz, ix, iy, iz);
-                    if (hit && nd.prim != kInterior) {
-                        lq[kQueueLanes * nq++] = nd.prim;
-                        if (nq == (uint32_t)kLeafQueue) lim = 0u;
-                    }
+                    const bool hit = near_box_hit(nd, s0, s1, s2, h0, h1, h2, ox, oy, oz, ix, iy, iz);
+                    // a stepping lane has room in its queue (lim drops to 0 when it fills),
+                    // so the slot is written unconditionally and taken only by a hit leaf
+                    lq[kQueueLanes * nq] = nd.prim;
+                    nq += (hit & (nd.prim != kInterior)) ? 1u : 0u;
+                    lim = nq == (uint32_t)kLeafQueue ? 0u : lim;
                     // a leaf's skip link is its own index + 1 in the threaded layout, so a
                     // hit node of either kind continues at i + 1
                     i = hit ? i + 1 : nd.skip;
                 }
+                ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(i < lim));
+                if (ns <= stop) break;
             }
+            cnt.nodes += steps;
         }
         // ---- the queued leaves, in the order found: Beer-Lambert on each crystal hit
         // before the step end
