@@ -14,7 +14,7 @@ each candidate renders a crop of the workload's frame (whole tiles, edge tiles i
 on the batched path and is scored by the median HIP-event time of its kernel.  Transport
 candidates must reproduce the default's latent counts exactly (they only reorder work).
 
-    python -m fgrain_amd.autotune --workload frame24mp_trix_cubic [--tiles 4x3] [--reps 3]
+    python -m fgrain_amd.autotune --workload frame24mp_trix_cubic [--tiles 8x6] [--reps 3]
 
 writes fgrain_amd/configs/tuning/<workload>.json, which bench.py applies.
 """
@@ -58,7 +58,7 @@ def _median_ms(frame, key, reps):
     return float(np.median([frame.render(Frame.BATCHED)[key] for _ in range(reps)]))
 
 
-def tune(engine, workload: str, tiles=(4, 3), reps=3, log=print) -> dict:
+def tune(engine, workload: str, tiles=(8, 6), reps=3, log=print) -> dict:
     from . import workloads as W
 
     cfg = W.load_config(workload)
@@ -113,7 +113,7 @@ def main(argv=None):
 
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="frame24mp_trix_cubic")
-    ap.add_argument("--tiles", default="4x3")
+    ap.add_argument("--tiles", default="8x6")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=None, help="output JSON (default: the tuning directory)")
     a = ap.parse_args(argv)

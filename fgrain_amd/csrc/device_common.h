@@ -520,6 +520,8 @@ __device__ __forceinline__ bool near_box_hit(const NodeView &nd, float s0, float
 }
 
 constexpr int kLeafQueue = 12;
+// Candidates of the absorption pre-test kept per lane and round (LDS column).
+constexpr uint32_t kCandidates = 8;
 constexpr uint32_t kQueueLanes = 256;
 
 // kMode 0: latent counts (the transport); 1: expected-dose replay; 2: gradient replay
@@ -550,11 +552,16 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     if (!kFrame) lane_tile_single(C, T, S0);
     PhotonRng rng;
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f, w = 0.f;
-    float step = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, h0 = 0.f, h1 = 0.f, h2 = 0.f;
+    float step = 0.f;
     uint32_t b = 0, i = 0, nq = 0;   // nq: leaves queued in this lane's column of lq
     __shared__ uint32_t lq_all[kLeafQueue][kQueueLanes];
     uint32_t *lq = &lq_all[0][threadIdx.x];
+    __shared__ uint32_t cq_all[kCandidates][kQueueLanes];
+    uint32_t *cq = &cq_all[0][threadIdx.x];
+    // leaf cursor of the lane's queue: kc the next crystal of the open leaf, qk = (next
+    // queue slot) << 28 | (the open leaf's end); a tile's records are addressed by 32-bit
+    // byte offsets, so its crystal indices (and leaf ends) stay below 2^32 / 48 < 2^28
+    uint32_t kc = 0u, qk = 0u;
     // per-wave words of the deposit pre-reduction (wave_aggregated_add)
     __shared__ uint32_t agg_u32[kQueueLanes / 64];
     __shared__ double agg_f64[kQueueLanes / 64];
@@ -689,19 +696,6 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 if (C.mu_t <= 0.0f) s = 1e6f;
                 else s = -__logf(1.0f - rng.u()) / C.mu_t;
                 step = fminf(s, C.step_cap);
-                ix = 1.0f / dx;
-                iy = 1.0f / dy;
-                iz = 1.0f / dz;
-                const float ex = ox + dx * step, ey = oy + dy * step, ez = oz + dz * step;
-                // without pruning the segment box is unbounded and every node is near
-                const float mg = prune ? C.reach + 1e-3f + 1e-5f * (fabsf(ox) + fabsf(oy) + fabsf(oz) + step)
-                                       : __builtin_inff();
-                s0 = fminf(ox, ex) - mg;
-                h0 = fmaxf(ox, ex) + mg;
-                s1 = fminf(oy, ey) - mg;
-                h1 = fmaxf(oy, ey) + mg;
-                s2 = fminf(oz, ez) - mg;
-                h2 = fmaxf(oz, ez) + mg;
                 i = 0;
                 nq = 0u;
                 phase = kWalk;
@@ -716,11 +710,29 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             const uint32_t walking = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(phase == kWalk));
             const uint32_t stop =
                 __builtin_amdgcn_readfirstlane((walking * (100u - min(C.wait_pct, 100u))) / 100u);
-            uint32_t lim = phase == kWalk ? T.n_nodes : 0u;
+            uint32_t lim = phase == kWalk && nq == 0u ? T.n_nodes : 0u;
             // the first step runs whenever a lane can step, later ones while more than stop
             // lanes can (0 lanes is at most stop)
             uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(i < lim));
             uint32_t steps = 0;
+            // the walk's inverse direction and segment box, from the bounce's origin,
+            // direction and step on every pass (not kept live through the leaf phase)
+            float ix = 0.f, iy = 0.f, iz = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f, h0 = 0.f, h1 = 0.f, h2 = 0.f;
+            if (ns != 0u) {
+                ix = 1.0f / dx;
+                iy = 1.0f / dy;
+                iz = 1.0f / dz;
+                const float ex = ox + dx * step, ey = oy + dy * step, ez = oz + dz * step;
+                // without pruning the segment box is unbounded and every node is near
+                const float mg = prune ? C.reach + 1e-3f + 1e-5f * (fabsf(ox) + fabsf(oy) + fabsf(oz) + step)
+                                       : __builtin_inff();
+                s0 = fminf(ox, ex) - mg;
+                h0 = fmaxf(ox, ex) + mg;
+                s1 = fminf(oy, ey) - mg;
+                h1 = fmaxf(oy, ey) + mg;
+                s2 = fminf(oz, ez) - mg;
+                h2 = fmaxf(oz, ez) + mg;
+            }
             while (ns != 0u) {
                 steps += ns;
                 if (i < lim) {
@@ -741,65 +753,100 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             cnt.nodes += steps;
         }
         // ---- the queued leaves, in the order found: Beer-Lambert on each crystal hit
-        // before the step end
-#pragma unroll 1
-        for (int qd = 0; qd < kLeafQueue; qd++) {
-            if (!(phase == kWalk && (uint32_t)qd < nq)) continue;
-            const uint32_t lf = lq[kQueueLanes * qd];
-            const uint32_t first = lf >> 4, nprim = lf & 15u;
-            for (uint32_t k = first; k < first + nprim; k++) {
-                cnt.prims++;
-                const Rec r = load_rec_buf(recs_r, T.crystal_base + k);
-                float t0, t1;
-                if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
-                if (!(t0 < step)) continue;
-                const float L = fmaxf(fminf(t1, step) - t0, 0.0f);
-                if (!(L > 0.0f)) continue;
-                const float pa = 1.0f - __expf(-C.mu_abs * L);
-                if constexpr (kMode == kCounts) {
-                    const float lam = w * pa * r.sens * 5.0f;
-                    const float fl = fminf(floorf(lam), 4294967040.0f);
-                    const uint32_t whole = fl > 0.0f ? (uint32_t)fl : 0u;
-                    const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
-                    const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
-                    if constexpr (kPre) {
-                        cnt.atomics += wave_aggregated_add(counts_b, T.crystal_base + k, k_lat, k_lat > 0u,
-                                                           &agg_u32[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
-                    } else if (k_lat > 0u) {
-                        atomicAdd(&counts_b[T.crystal_base + k], k_lat);
-                    }
-                    if (k_lat > 0u) {
-                        cnt.abs++;
-                        cnt.tot += k_lat;
-                    }
-                    w *= fmaxf(1.0f - pa * 0.9f, 0.05f);
-                } else {
-                    cnt.abs++;
-                    if constexpr (kMode == kReplayGrad) {
-                        const float base = 5.0f * r.sens * R->gdose[k];
-                        const float dep = w * pa;
-                        a_e += base * dep;
-                        a_mu += base * E * (w_mu * pa + w * L * (1.0f - pa));
-                        a_r += base * E * (w_r * pa);
-                        a_g += base * E * dep * score;
+        // before the step end, in rounds of two phases.  A: the lane runs the cheap
+        // conservative pre-test (crystal_may_absorb) over its leaves' crystals in order
+        // from its cursor (the open leaf's crystals left, then the next queue slots),
+        // keeping the candidates in an LDS column; a full column ends its phase A.
+        // B: it tests the candidates in the same order with crystal_hit.  A crystal the
+        // pre-test drops cannot record an event, and no draw or weight depends on the
+        // dropped ones, so every result is that of testing all of them.  A lane whose
+        // leaves are not all done keeps its queue and cursor, takes no node steps in the
+        // next pass of the loop and continues there.
+        uint32_t nc = 0u;
+        bool more = phase == kWalk && nq > 0u;
+        while (__builtin_amdgcn_ballot_w64(more && nc < kCandidates) != 0ull) {
+            if (more && nc < kCandidates) {
+                // open the next leaf with crystals, or end at the queue's end
+                while (kc == (qk & 0x0FFFFFFFu) && more) {
+                    const uint32_t qd = qk >> 28;
+                    if (qd < nq) {
+                        const uint32_t lf = lq[kQueueLanes * qd];
+                        kc = lf >> 4;
+                        qk = ((qd + 1u) << 28) | (kc + (lf & 15u));
                     } else {
-                        const double dep = (double)(((E * w) * pa) * r.sens * 5.0f);
-                        if constexpr (kPre) {
-                            cnt.atomics += wave_aggregated_add(R->dose, k, dep, true,
-                                                               &agg_f64[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
-                        } else {
-                            atomicAdd(&R->dose[k], dep);
-                        }
+                        more = false;
                     }
-                    const float m = 1.0f - pa * 0.9f;
-                    const float dm = -0.9f * L * (1.0f - pa);
-                    w_mu = w_mu * m + w * dm;
-                    w_r = w_r * m;
-                    w = w * m;
+                }
+                if (more) {
+                    cnt.prims++;
+                    const uint32_t o = (T.crystal_base + kc) * 48u;
+                    const float4 ca = buf_load4(recs_r, o), ch = buf_load4(recs_r, o + 32u);
+                    const float cc[3] = {ca.x, ca.y, ca.z}, hh[3] = {ch.x, ch.y, ch.z};
+                    if (crystal_may_absorb(cc, hh, __float_as_uint(ch.w), ox, oy, oz, dx, dy, dz, step)) {
+                        cq[kQueueLanes * nc] = kc;
+                        nc++;
+                    }
+                    kc++;
                 }
             }
         }
-        nq = 0u;
+        if (phase == kWalk && nq > 0u && !more) {
+            nq = 0u;   // every queued leaf done: the walk goes on
+            kc = qk = 0u;
+        }
+#pragma unroll 1
+        for (uint32_t j = 0; j < kCandidates; j++) {
+            if (j >= nc) continue;
+            const uint32_t k = cq[kQueueLanes * j];
+            const Rec r = load_rec_buf(recs_r, T.crystal_base + k);
+            float t0, t1;
+            if (!crystal_hit(r.c, r.q, r.h, r.morph, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
+            if (!(t0 < step)) continue;
+            const float L = fmaxf(fminf(t1, step) - t0, 0.0f);
+            if (!(L > 0.0f)) continue;
+            const float pa = 1.0f - __expf(-C.mu_abs * L);
+            if constexpr (kMode == kCounts) {
+                const float lam = w * pa * r.sens * 5.0f;
+                const float fl = fminf(floorf(lam), 4294967040.0f);
+                const uint32_t whole = fl > 0.0f ? (uint32_t)fl : 0u;
+                const float frac = fminf(fmaxf(lam - (float)whole, 0.0f), 1.0f);
+                const uint32_t k_lat = whole + (rng.u() < frac ? 1u : 0u);
+                if constexpr (kPre) {
+                    cnt.atomics += wave_aggregated_add(counts_b, T.crystal_base + k, k_lat, k_lat > 0u,
+                                                       &agg_u32[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+                } else if (k_lat > 0u) {
+                    atomicAdd(&counts_b[T.crystal_base + k], k_lat);
+                }
+                if (k_lat > 0u) {
+                    cnt.abs++;
+                    cnt.tot += k_lat;
+                }
+                w *= fmaxf(1.0f - pa * 0.9f, 0.05f);
+            } else {
+                cnt.abs++;
+                if constexpr (kMode == kReplayGrad) {
+                    const float base = 5.0f * r.sens * R->gdose[k];
+                    const float dep = w * pa;
+                    a_e += base * dep;
+                    a_mu += base * E * (w_mu * pa + w * L * (1.0f - pa));
+                    a_r += base * E * (w_r * pa);
+                    a_g += base * E * dep * score;
+                } else {
+                    const double dep = (double)(((E * w) * pa) * r.sens * 5.0f);
+                    if constexpr (kPre) {
+                        cnt.atomics += wave_aggregated_add(R->dose, k, dep, true,
+                                                           &agg_f64[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+                    } else {
+                        atomicAdd(&R->dose[k], dep);
+                    }
+                }
+                const float m = 1.0f - pa * 0.9f;
+                const float dm = -0.9f * L * (1.0f - pa);
+                w_mu = w_mu * m + w * dm;
+                w_r = w_r * m;
+                w = w * m;
+            }
+        }
         // ---- end of the bounce: move, boundaries, halation or scattering, roulette
         if (phase == kWalk && nq == 0u && i >= T.n_nodes) {
             const float qx = ox + dx * step, qy = oy + dy * step, qz = oz + dz * step;

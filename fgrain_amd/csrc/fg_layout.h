@@ -151,6 +151,62 @@ FG_HD bool crystal_hit(const float c[3], const float q[4], const float h[3], uin
     return false;
 }
 
+// Reciprocal for error margins: one hardware instruction on the device (1 ulp), the
+// division on the host; either is far inside the margins it scales.
+FG_HD float margin_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+// Conservative pre-test of an absorption event on the segment o + t d, t in [0, step]
+// (|d| = 1): false only when crystal_hit, evaluated in float on the same inputs, cannot
+// return a hit with t0 < step and a chord L = min(t1, step) - t0 > 0.  The crystal lies
+// in its bounding sphere (radius sqrt(3) hmax for the cube, hmax for the ellipsoid), and
+// the test rejects a segment that misses that sphere by more than crystal_hit's float
+// error can move its answer:
+//  * cube: the slab times are quotients of differences that carry a few ulps of
+//    |o - c| + R, so the hit decision and t0, t1 are those of a geometry perturbed by a
+//    few ulps of that size; margin 1e-5 (|o - c|_1 + R), about 100 times that;
+//  * ellipsoid: the discriminant b^2 - 4 a (|p|^2 - 1) in semi-axis units (p the origin,
+//    |p| <= rho = (|o - c|_1 + R) / hmin) loses about 10 ulps of b^2 = 4 |p|^2 a to
+//    cancellation, so d_perp^2 - 1 (d_perp: distance of the line from the centre in those
+//    units) is decided within 6e-7 rho^2; the test keeps every segment with physical
+//    distance^2 <= R^2 (1 + 1e-5 rho + 6e-6 rho^2), ten times that (and 1e-5 rho for the
+//    rotation and division errors).  Near tangency the roots spread by at most
+//    sqrt(error) / 2a <= 7.7e-4 rho hmax along the ray; the ends of the segment get 8e-3
+//    rho hmax.
+// Comparisons are written so that a NaN keeps the crystal.
+FG_HD bool crystal_may_absorb(const float c[3], const float h[3], uint32_t morph, float ox, float oy, float oz,
+                              float dx, float dy, float dz, float step)
+{
+    const float px = c[0] - ox, py = c[1] - oy, pz = c[2] - oz;
+    const float hmax = fmaxf(fmaxf(h[0], h[1]), h[2]);
+    const float pn = fabsf(px) + fabsf(py) + fabsf(pz);
+    float R, r2, mt;
+    if (morph == 0u) {
+        R = hmax * 1.7320525f;   // sqrt(3) (1 + 1e-6)
+        const float m = 1e-5f * (pn + R);
+        r2 = (R + m) * (R + m);
+        mt = m;
+    } else {
+        R = hmax * 1.000001f;
+        const float hmin = fminf(fminf(h[0], h[1]), h[2]);
+        const float rho = (pn + R) * margin_rcp(hmin);
+        r2 = (R * R) * (1.0f + rho * (1e-5f + 6e-6f * rho));
+        mt = hmax * (8e-3f * rho) + 1e-5f * (pn + R);
+    }
+    const float ax = py * dz - pz * dy, ay = pz * dx - px * dz, az = px * dy - py * dx;
+    const float tp = px * dx + py * dy + pz * dz;
+    if (ax * ax + ay * ay + az * az > r2) return false;   // the line passes outside
+    if (tp + (R + mt) <= 0.0f) return false;               // behind the origin
+    if (tp - (R + mt) >= step) return false;               // beyond the segment's end
+    return true;
+}
+
 // Constants of a crystal for rays along -z (the scan): the direction (0, 0, -1) rotated
 // into the crystal frame depends on the crystal only, so its inverse (cube) or its
 // components over the semi-axes (other shapes) are computed once per crystal; k[3] is the
