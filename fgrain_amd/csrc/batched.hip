@@ -158,16 +158,17 @@ __global__ void __launch_bounds__(256) k_reset(Bufs B, uint32_t *counts, unsigne
 }
 
 // persistent transport over every tile's packets, built for 6 waves per SIMD (80 VGPRs)
-// and for 7 (72 VGPRs, more spills): which is faster depends on the frame (Tri-X frame
-// 1363 / 1285 ms, T-Max frame 3806 / 3877 ms), so the tuning chooses
-template <bool kPre>
+// and for 7 (72 VGPRs, a few spills, its own LDS queue sizes): which is faster depends on
+// the frame (Tri-X frame 1063 / 996 ms, T-Max frame 2921 / 3251 ms with the default queue
+// sizes), so the tuning chooses
+template <bool kPre, int kLQ, uint32_t kCand>
 __device__ __forceinline__ void transport_frame(const TransportArgs &C, const Bufs &B, const Node *nodes,
                                                 const float4 *recs, uint32_t *counts, const float *emap,
                                                 uint32_t n_tiles, uint64_t total_photons)
 {
     LaneCounters cnt;
-    transport_persistent<true, kCounts, kPre>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work,
-                                              total_photons, cnt);
+    transport_persistent<true, kCounts, kPre, kLQ, kCand>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work,
+                                                          total_photons, cnt);
     flush_lane_counters(cnt, C.stats);
 }
 
@@ -179,7 +180,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))
                                                    const float *emap, uint32_t n_tiles,
                                                    uint64_t total_photons)
 {
-    transport_frame<kPre>(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
+    transport_frame<kPre, kLeafQueue, kCandidates>(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
 }
 
 template <bool kPre>
@@ -188,7 +189,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))
                                                    const float *emap, uint32_t n_tiles,
                                                    uint64_t total_photons)
 {
-    transport_frame<kPre>(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
+    transport_frame<kPre, kLeafQueueW7, kCandidatesW7>(C, B, nodes, recs, counts, emap, n_tiles, total_photons);
 }
 
 using TransportKernel = void (*)(TransportArgs, Bufs, const Node *, const float4 *, uint32_t *, const float *,

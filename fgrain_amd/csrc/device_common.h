@@ -490,15 +490,6 @@ __device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32
 
 enum : uint32_t { kNeed = 0, kStart = 1, kWalk = 2, kDone = 3 };
 
-// Hit leaves of one walk are queued, up to kLeafQueue, and processed together in the
-// order found.  A bounce's ray is fixed and the walk does not read the packet weight, so
-// deferring a leaf changes no result; lanes holding a leaf keep stepping instead of
-// idling until their neighbours find one.  The queue lives in LDS, one column per lane
-// (slot q of lane t at [q][t]: 64 consecutive words per wave, no bank conflicts), so
-// its depth costs no registers.  Measured on the Tri-X frame (transport ms): register
-// queue 1961 / 1626 / 1514 / 1460 with depth 1 / 2 / 3 / 4 (30 VGPRs spilled at
-// depth 4); LDS queue 1415 / 1378 / 1363 / 1366 / 1369 with depth 4 / 8 / 12 / 16 / 24
-// (T-Max frame: 4027 / 3848 / 3809 / 3842 / 3865).  Every launch is 256 lanes per block.
 // The node test of the transport walk as one floating-point comparison: the node box
 // overlaps the segment box [s, h] (lo <= h and hi >= s on every axis) and box_hit holds
 // (tx >= te and tx > 0).  With denormals kept, a - b <= 0 exactly when a <= b for finite
@@ -519,16 +510,32 @@ __device__ __forceinline__ bool near_box_hit(const NodeView &nd, float s0, float
     return fmaxf(fmaxf(a, b), c) <= 0.0f;
 }
 
+// Hit leaves of one walk are queued, up to kLeafQueue, and processed together in the
+// order found.  A bounce's ray is fixed and the walk does not read the packet weight, so
+// deferring a leaf changes no result; lanes holding a leaf keep stepping instead of
+// idling until their neighbours find one.  The queue lives in LDS, one column per lane
+// (slot q of lane t at [q][t]: 64 consecutive words per wave, no bank conflicts), so
+// its depth costs no registers.  Measured on the Tri-X frame (transport ms): register
+// queue 1961 / 1626 / 1514 / 1460 with depth 1 / 2 / 3 / 4 (30 VGPRs spilled at
+// depth 4); LDS queue 1415 / 1378 / 1363 / 1366 / 1369 with depth 4 / 8 / 12 / 16 / 24
+// (T-Max frame: 4027 / 3848 / 3809 / 3842 / 3865).  Every launch is 256 lanes per block.
+// The leaf queue and the candidates of the absorption pre-test (below) are LDS columns,
+// 1 KB per slot and 256-lane block.  Defaults (six waves per SIMD, the per-tile and
+// replay kernels): 12 leaves and 8 candidates, 20 KB; the seven-wave frame transport: 10
+// and 12, 22 KB, the most that seven blocks fit in a CU's 160 KB (Tri-X frame transport
+// 996 ms with 12 and 8, 954 with 12 and 10, 946 with 10 and 12, 954 with 8 and 14; T-Max
+// at six waves 2912, 2908, 2928 and 2974 ms, and 2911 with 12 and 12).
 constexpr int kLeafQueue = 12;
-// Candidates of the absorption pre-test kept per lane and round (LDS column).
 constexpr uint32_t kCandidates = 8;
+constexpr int kLeafQueueW7 = 10;
+constexpr uint32_t kCandidatesW7 = 12;
 constexpr uint32_t kQueueLanes = 256;
 
 // kMode 0: latent counts (the transport); 1: expected-dose replay; 2: gradient replay
 // (single tile only), with R the replay outputs and sums the lane's parameter sums.
 enum : int { kCounts = 0, kReplayDose = 1, kReplayGrad = 2 };
 
-template <bool kFrame, int kMode = kCounts, bool kPre = false>
+template <bool kFrame, int kMode = kCounts, bool kPre = false, int kLQ = kLeafQueue, uint32_t kCand = kCandidates>
 __device__ __forceinline__ void transport_persistent(const TransportArgs &C, const TileDev *tiles,
                                                      uint32_t n_tiles, const Node *nodes,
                                                      const float4 *recs, uint32_t *counts,
@@ -554,9 +561,9 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f, w = 0.f;
     float step = 0.f;
     uint32_t b = 0, i = 0, nq = 0;   // nq: leaves queued in this lane's column of lq
-    __shared__ uint32_t lq_all[kLeafQueue][kQueueLanes];
+    __shared__ uint32_t lq_all[kLQ][kQueueLanes];
     uint32_t *lq = &lq_all[0][threadIdx.x];
-    __shared__ uint32_t cq_all[kCandidates][kQueueLanes];
+    __shared__ uint32_t cq_all[kCand][kQueueLanes];
     uint32_t *cq = &cq_all[0][threadIdx.x];
     // leaf cursor of the lane's queue: kc the next crystal of the open leaf, qk = (next
     // queue slot) << 28 | (the open leaf's end); a tile's records are addressed by 32-bit
@@ -742,7 +749,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     // so the slot is written unconditionally and taken only by a hit leaf
                     lq[kQueueLanes * nq] = nd.prim;
                     nq += (hit & (nd.prim != kInterior)) ? 1u : 0u;
-                    lim = nq == (uint32_t)kLeafQueue ? 0u : lim;
+                    lim = nq == (uint32_t)kLQ ? 0u : lim;
                     // a leaf's skip link is its own index + 1 in the threaded layout, so a
                     // hit node of either kind continues at i + 1
                     i = hit ? i + 1 : nd.skip;
@@ -764,8 +771,8 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
         // next pass of the loop and continues there.
         uint32_t nc = 0u;
         bool more = phase == kWalk && nq > 0u;
-        while (__builtin_amdgcn_ballot_w64(more && nc < kCandidates) != 0ull) {
-            if (more && nc < kCandidates) {
+        while (__builtin_amdgcn_ballot_w64(more && nc < kCand) != 0ull) {
+            if (more && nc < kCand) {
                 // open the next leaf with crystals, or end at the queue's end
                 while (kc == (qk & 0x0FFFFFFFu) && more) {
                     const uint32_t qd = qk >> 28;
@@ -795,7 +802,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
             kc = qk = 0u;
         }
 #pragma unroll 1
-        for (uint32_t j = 0; j < kCandidates; j++) {
+        for (uint32_t j = 0; j < kCand; j++) {
             if (j >= nc) continue;
             const uint32_t k = cq[kQueueLanes * j];
             const Rec r = load_rec_buf(recs_r, T.crystal_base + k);
