@@ -545,6 +545,8 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                                                      ReplaySums *sums = nullptr)
 {
     static_assert(kMode == kCounts || !kFrame, "replay runs on one tile");
+    // the leaf cursor keeps the next queue slot (up to kLQ) in the top 4 bits of qk
+    static_assert(kLQ >= 1 && kLQ <= 15, "leaf queue depth must fit the cursor's 4-bit slot field");
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t chunk = max(C.chunk, 64u);
     unsigned long long pool = 0;
