@@ -461,10 +461,8 @@ __device__ __forceinline__ void lane_tile_single(const TransportArgs &C, LaneTil
     S.ph = C.ph;
 }
 
-// frame packet id g -> tile (last tile with photon_base <= g) and its constants
-__device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32_t n_tiles, uint64_t g,
-                                                    const float *emap, uint32_t estride, LaneTile &T,
-                                                    PacketStart &S)
+// index of the tile of frame packet id g (last tile with photon_base <= g)
+__device__ __forceinline__ uint32_t tile_of_packet(const TileDev *tiles, uint32_t n_tiles, uint64_t g)
 {
     uint32_t lo = 0, hi = n_tiles;
     while (hi - lo > 1) {
@@ -472,7 +470,12 @@ __device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32
         if (tiles[mid].photon_base <= g) lo = mid;
         else hi = mid;
     }
-    const TileDev &t = tiles[lo];
+    return lo;
+}
+
+__device__ __forceinline__ void lane_tile_fill(const TileDev &t, const float *emap, uint32_t estride, LaneTile &T,
+                                               PacketStart &S)
+{
     T.node_base = t.node_base;
     T.crystal_base = t.crystal_base;
     T.n_nodes = t.n_nodes;
@@ -485,7 +488,6 @@ __device__ __forceinline__ uint64_t lane_tile_frame(const TileDev *tiles, uint32
     S.key1 = t.key1;
     S.pw = t.pw;
     S.ph = t.ph;
-    return g - t.photon_base;
 }
 
 enum : uint32_t { kNeed = 0, kStart = 1, kWalk = 2, kDone = 3 };
@@ -551,6 +553,17 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     const uint32_t chunk = max(C.chunk, 64u);
     unsigned long long pool = 0;
     uint32_t pool_left = 0;
+    // per wave: the tile of its latest chunk and that tile's packet range (LDS, not live
+    // in registers between packet starts)
+    __shared__ unsigned long long wrange[kQueueLanes / 64][2];
+    __shared__ uint32_t wtile[kQueueLanes / 64];
+    const uint32_t wv = threadIdx.x >> 6;
+    if constexpr (kFrame) {
+        wrange[wv][0] = 0ull;
+        wrange[wv][1] = 0ull;
+        wtile[wv] = 0u;
+        __builtin_amdgcn_wave_barrier();
+    }
     uint32_t phase = kNeed;
     const Node *nodes_b = kFrame ? nodes : C.nodes;
     const float4 *recs_b = kFrame ? recs : C.recs;
@@ -600,6 +613,9 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
         // ---- hand out packets to lanes without one
         const unsigned long long nm = __ballot(phase == kNeed);
         if (nm) {
+            // the wave's packet ids come in consecutive chunks, far smaller than a tile: the
+            // tile of the latest chunk's first id is kept wave-uniform (wt, [wbase, wend)),
+            // and a lane whose id lies in it skips the binary search
             // n lanes need a packet; r = how many of them sit below this lane (mbcnt)
             const uint32_t n = (uint32_t)__popcll(nm);
             const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(nm >> 32),
@@ -614,6 +630,15 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 const uint32_t nb_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(nb >> 32));
                 const uint32_t nb_lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)nb);
                 nb = ((unsigned long long)nb_hi << 32) | (unsigned long long)nb_lo;
+                if constexpr (kFrame) {
+                    if (nb < total) {
+                        const uint32_t wt = tile_of_packet(tiles, n_tiles, nb);
+                        wrange[wv][0] = tiles[wt].photon_base;
+                        wrange[wv][1] = wt + 1u < n_tiles ? tiles[wt + 1u].photon_base : total;
+                        wtile[wv] = wt;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
                 if (r >= pool_left) id = nb + (r - pool_left);
                 pool = nb + (n - pool_left);
                 pool_left = chunk - (n - pool_left);
@@ -628,7 +653,12 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                 } else {
                     uint64_t p = id;
                     PacketStart S = S0;
-                    if (kFrame) p = lane_tile_frame(tiles, n_tiles, id, emap, C.estride, T, S);
+                    if constexpr (kFrame) {
+                        const uint32_t ti = (id >= wrange[wv][0] && id < wrange[wv][1]) ? wtile[wv]
+                                                                                     : tile_of_packet(tiles, n_tiles, id);
+                        lane_tile_fill(tiles[ti], emap, C.estride, T, S);
+                        p = id - tiles[ti].photon_base;
+                    }
                     rng.init(p, S.key0, S.key1);
                     float x, y;
                     if (C.ppp > 0) {
