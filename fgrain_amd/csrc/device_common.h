@@ -817,23 +817,40 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                     }
                 }
                 if (more) {
-                    // two crystals of the open leaf at a time while the leaf and the column
-                    // have room for both (their loads in flight together)
-                    const bool two = kc + 1u < (qk & 0x0FFFFFFFu) && nc + 1u < kCand;
+                    // two crystals at a time while the column has room for both (their loads
+                    // in flight together): the open leaf's next two, or its last one and the
+                    // first of the next queued leaf
+                    bool two = nc + 1u < kCand;
+                    uint32_t k2 = kc + 1u, qk2 = qk;
+                    if (two && k2 == (qk & 0x0FFFFFFFu)) {
+                        const uint32_t qd = qk >> 28;
+                        two = qd < nq;
+                        if (two) {
+                            const uint32_t lf = lq[kQueueLanes * qd];
+                            k2 = lf >> 4;
+                            qk2 = ((qd + 1u) << 28) | (k2 + (lf & 15u));
+                            two = (lf & 15u) != 0u;   // a leaf of no crystals: one this step
+                        }
+                    }
                     const uint32_t o = (T.crystal_base + kc) * 48u;
-                    const uint32_t o2 = two ? o + 48u : o;
+                    const uint32_t o2 = two ? (T.crystal_base + k2) * 48u : o;
                     const float4 ca = buf_load4(recs_r, o), ch = buf_load4(recs_r, o + 32u);
                     const float4 ca2 = buf_load4(recs_r, o2), ch2 = buf_load4(recs_r, o2 + 32u);
                     const float cc[3] = {ca.x, ca.y, ca.z}, hh[3] = {ch.x, ch.y, ch.z};
                     const float cc2[3] = {ca2.x, ca2.y, ca2.z}, hh2[3] = {ch2.x, ch2.y, ch2.z};
                     const bool m1 = crystal_may_absorb(cc, hh, __float_as_uint(ch.w), ox, oy, oz, dx, dy, dz, step);
                     const bool m2 = two && crystal_may_absorb(cc2, hh2, __float_as_uint(ch2.w), ox, oy, oz, dx, dy, dz, step);
-                    cq[kQueueLanes * nc] = m1 ? kc : kc + 1u;
+                    cq[kQueueLanes * nc] = m1 ? kc : k2;
                     nc += m1 ? 1u : 0u;
-                    if (m1 && m2) cq[kQueueLanes * nc] = kc + 1u;
+                    if (m1 && m2) cq[kQueueLanes * nc] = k2;
                     nc += m2 ? 1u : 0u;
                     cnt.prims += two ? 2u : 1u;
-                    kc += two ? 2u : 1u;
+                    if (two) {
+                        kc = k2 + 1u;
+                        qk = qk2;
+                    } else {
+                        kc++;
+                    }
                 }
             }
         }
