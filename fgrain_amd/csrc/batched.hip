@@ -856,9 +856,11 @@ __global__ void __launch_bounds__(256) k_bvh_level_tiny(Bufs B, int level)
     }
 }
 
-__global__ void k_build_check(Bufs B)
+// slot: the segment-count slot the level after the last launched one reads
+// ((last + 1) % 3, which is kMaxLevels % 3 when every level was launched)
+__global__ void k_build_check(Bufs B, uint32_t slot)
 {
-    const uint32_t *left = B.seg_count + 3u * (kMaxLevels % 3);
+    const uint32_t *left = B.seg_count + 3u * slot;
     if (threadIdx.x == 0 && (left[0] | left[1] | left[2]) != 0u) B.err[2] = 1u;
 }
 
@@ -1207,6 +1209,37 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     } while (0)
 #define TRY_LAUNCH(b, name) TRY_STEP(b, name, hipGetLastError())
 
+// The level kernels of a BVH build, level 0 up.  A capture records all kMaxLevels levels
+// (the graph is replayed for any tree); direct launches read the segment counts back every
+// 8 levels and stop once a level leaves no segments (trees end after 20-30 levels and each
+// empty level is three launches).  check_slot is the slot k_build_check reads.
+static hipError_t launch_levels(BatchedFrame *b, Bufs &B, hipStream_t s, uint32_t &check_slot)
+{
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    TRY(hipStreamIsCapturing(s, &cap));
+    const bool can_stop = cap == hipStreamCaptureStatusNone;
+    check_slot = kMaxLevels % 3;
+    for (int level = 0; level < kMaxLevels; level++) {
+        hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(kLevelThreads), 0, s, B, level);
+        if (b) TRY_LAUNCH(b, "k_bvh_level");
+        hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
+        if (b) TRY_LAUNCH(b, "k_bvh_level_small");
+        hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
+        if (b) TRY_LAUNCH(b, "k_bvh_level_tiny");
+        if (can_stop && level % 8 == 7 && level + 1 < kMaxLevels) {
+            uint32_t sc[3 * kSegClasses];
+            TRY(hipMemcpyAsync(sc, B.seg_count, sizeof sc, hipMemcpyDeviceToHost, s));
+            TRY(hipStreamSynchronize(s));
+            const uint32_t next = (uint32_t)(level + 1) % 3u;
+            if ((sc[3 * next] | sc[3 * next + 1] | sc[3 * next + 2]) == 0u) {
+                check_slot = next;
+                break;
+            }
+        }
+    }
+    return hipSuccess;
+}
+
 // Stage 0: reset and transport; 1: development and the developed-crystal BVH; 2: scan;
 // kStageResetKeepCounts: the reset of stage 0 without the latent counts and counters.
 constexpr int kStageResetKeepCounts = 3;
@@ -1249,15 +1282,9 @@ static hipError_t launch_stage(BatchedFrame *b, int stage, hipStream_t s)
         TRY_LAUNCH(b, "k_boost");
         hipLaunchKernelGGL(k_root_segments, dim3((b->n_tiles + 255) / 256), dim3(256), 0, s, B, b->n_tiles);
         TRY_LAUNCH(b, "k_root_segments");
-        for (int level = 0; level < kMaxLevels; level++) {
-            hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(kLevelThreads), 0, s, B, level);
-            TRY_LAUNCH(b, "k_bvh_level");
-            hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
-            TRY_LAUNCH(b, "k_bvh_level_small");
-            hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
-            TRY_LAUNCH(b, "k_bvh_level_tiny");
-    }
-    hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B);
+    uint32_t check_slot = kMaxLevels % 3;
+    TRY(launch_levels(b, B, s, check_slot));
+    hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B, check_slot);
     TRY_LAUNCH(b, "k_build_check");
     hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, 2 * n);
     TRY_LAUNCH(b, "k_leafcount");
@@ -1412,16 +1439,15 @@ hipError_t build_transport_bvhs(const TransportBuildIn &in, Node *nodes, Crystal
         e = hipMemcpyAsync(B.tiles, tiles.data(), n_tiles * sizeof(TileDev), hipMemcpyHostToDevice, s);
     const uint32_t nb = (uint32_t)((n + 255) / 256), nb2 = (uint32_t)((2 * n + 255) / 256);
     (void)hipGetLastError();
+    uint32_t check_slot = kMaxLevels % 3;
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_tb_init, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nb2, 8192))), dim3(256), 0, s,
                            B, (uint64_t)n, n_tiles);
         hipLaunchKernelGGL(k_root_segments, dim3((n_tiles + 255) / 256), dim3(256), 0, s, B, n_tiles);
-        for (int level = 0; level < kMaxLevels; level++) {
-            hipLaunchKernelGGL(k_bvh_level, dim3(2048), dim3(kLevelThreads), 0, s, B, level);
-            hipLaunchKernelGGL(k_bvh_level_small, dim3(2048), dim3(256), 0, s, B, level);
-            hipLaunchKernelGGL(k_bvh_level_tiny, dim3(2048), dim3(256), 0, s, B, level);
-        }
-        hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B);
+        e = launch_levels(nullptr, B, s, check_slot);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_build_check, dim3(1), dim3(64), 0, s, B, check_slot);
         hipLaunchKernelGGL(k_leafcount, dim3(nb2), dim3(256), 0, s, B, (uint64_t)(2 * n));
         hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, (uint64_t)(2 * n));
         hipLaunchKernelGGL(k_tb_records, dim3(nb), dim3(256), 0, s, B, in.quat, in.sens, in.morph, recs, leaf_slot,
