@@ -944,9 +944,10 @@ __global__ void __launch_bounds__(256) k_dev_records(Bufs B, const Crystal *recs
     Crystal r = recs[s];
     const double fe = B.fe[s];
     for (int a = 0; a < 3; a++) r.h[a] = (float)(B.dims64[3 * (size_t)s + a] * fe);
-    B.drecs[e] = r;
     float k[4];
     down_constants(r.q, r.h, r.morph, k);
+    r.sens = k[3];   // the scan reads the bounding-sphere radius with the centre
+    B.drecs[e] = r;
     B.dconst[2 * e] = make_float4(k[0], k[1], k[2], k[3]);
     const double *p = &B.pos64[3 * (size_t)s];
     B.dconst[2 * e + 1] = make_float4((float)(p[0] - (double)r.c[0]), (float)(p[1] - (double)r.c[1]),

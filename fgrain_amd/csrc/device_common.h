@@ -1058,12 +1058,15 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
             const uint32_t lf = lq[kQueueLanes * q];
             const uint32_t first = lf >> 4, cnt = lf & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
-                const float4 kc = buf_load4(consts_r, k << 5);
-                if (!sphere_may_hit_z_r(buf_load4(recs_r, k * 48u), kc.w, ox, oy)) continue;
-                const Rec r = load_rec_buf(recs_r, k);
-                const float4 cl = buf_load4(consts_r, (k << 5) + 16u);
+                // a scan record's fourth word holds its bounding-sphere radius
+                const float4 c0 = buf_load4(recs_r, k * 48u);
+                if (!sphere_may_hit_z_r(c0, c0.w, ox, oy)) continue;
+                const float4 kc = buf_load4(consts_r, k << 5), cl = buf_load4(consts_r, (k << 5) + 16u);
+                const float4 c1 = buf_load4(recs_r, k * 48u + 16u), c2 = buf_load4(recs_r, k * 48u + 32u);
+                const float cc[3] = {c0.x, c0.y, c0.z}, qq[4] = {c1.x, c1.y, c1.z, c1.w},
+                            hh[3] = {c2.x, c2.y, c2.z};
                 const float kk[4] = {kc.x, kc.y, kc.z, kc.w}, lo[3] = {cl.x, cl.y, cl.z};
-                P += crystal_chord_down(r.c, r.q, r.h, r.morph, kk, lo, ox, oy, oz);
+                P += crystal_chord_down(cc, qq, hh, __float_as_uint(c2.w), kk, lo, ox, oy, oz);
             }
         }
         nq = 0u;

@@ -95,7 +95,7 @@ struct ScanArgs {
 hipError_t launch_transport(const TransportArgs &a, hipStream_t s);
 // consts[2 k] = down_constants of record k (3 float4 per record), consts[2 k + 1] = lo[k],
 // the residual of its centre's rounding to f32 (x, y, z, 0); k < n
-hipError_t launch_down_constants(const float4 *recs, const float4 *lo, uint32_t n, float4 *consts, hipStream_t s);
+hipError_t launch_down_constants(float4 *recs, const float4 *lo, uint32_t n, float4 *consts, hipStream_t s);
 // Persistent transport of one tile (packets 0 .. a.n_photons - 1); work must point to a
 // zeroed 64-bit counter; grid = resident blocks of the device.
 hipError_t launch_transport_persistent(const TransportArgs &a, unsigned long long *work, int grid,

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
     if (valid && threadIdx.x % L == 0u) A.out[(size_t)py * A.ostride + px] = scan_value(T, A.mode);
 }
 
-__global__ void __launch_bounds__(256) down_constants_kernel(const float4 *recs, const float4 *lo, uint32_t n,
+__global__ void __launch_bounds__(256) down_constants_kernel(float4 *recs, const float4 *lo, uint32_t n,
                                                              float4 *consts)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,6 +64,7 @@ __global__ void __launch_bounds__(256) down_constants_kernel(const float4 *recs,
     down_constants(r.q, r.h, r.morph, c);
     consts[2 * k] = make_float4(c[0], c[1], c[2], c[3]);
     consts[2 * k + 1] = lo[k];
+    recs[3 * k] = make_float4(r.c[0], r.c[1], r.c[2], c[3]);   // radius in the sensitivity word
 }
 
 // ------------------------------------------------------- traversal for tests ---
@@ -214,7 +215,7 @@ hipError_t launch_transport(const TransportArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_down_constants(const float4 *recs, const float4 *lo, uint32_t n, float4 *consts, hipStream_t s)
+hipError_t launch_down_constants(float4 *recs, const float4 *lo, uint32_t n, float4 *consts, hipStream_t s)
 {
     (void)hipGetLastError();  // report only this launcher's launches
     if (n == 0) return hipSuccess;
