@@ -65,6 +65,7 @@ struct Bufs {
     BNode *bn;
     uint32_t *leafcnt;
     Node *dnodes;
+    ScanQNode *dqnodes;    // the scan's grid nodes, at the indices of dnodes
     Crystal *drecs;
     float4 *dconst;        // 2 per developed-crystal record: down_constants, centre residual
     Seg *seg[2][kSegClasses];   // [level parity][size class]
@@ -934,6 +935,17 @@ __global__ void __launch_bounds__(256) k_layout(Bufs B, uint64_t cap)
     if (g == 2ull * T.crystal_base) T.dev_nodes = nn;  // root: one writer per tile
 }
 
+// the scan's 16-byte grid node of every node of the developed-crystal trees
+__global__ void __launch_bounds__(256) k_scan_qnodes(Bufs B, uint64_t cap)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t t;
+    if (g >= cap || !node_valid(B, g, t)) return;
+    float bx, by, sc;
+    scan_qframe(B.dnodes[2ull * B.tiles[t].crystal_base], bx, by, sc);
+    B.dqnodes[g] = scan_qnode(B.dnodes[g], bx, by, sc);
+}
+
 __global__ void __launch_bounds__(256) k_dev_records(Bufs B, const Crystal *recs, uint64_t n)
 {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -963,9 +975,10 @@ __global__ void __launch_bounds__(256) k_scan(Bufs B, ScanCommon S, float *out, 
     const bool valid = pix < T.tw * T.th;
     const uint32_t px = valid ? pix % T.tw : 0u, py = valid ? pix / T.tw : 0u;
     const Node *nodes = B.dnodes + 2ull * T.crystal_base;
+    const ScanQNode *qnodes = B.dqnodes + 2ull * T.crystal_base;
     const float4 *recs = reinterpret_cast<const float4 *>(B.drecs + T.crystal_base);
     const float Tm = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, S.g, S.ss, T.pw, T.ph, S.z0,
-                                S.sigma, T.skey0, T.skey1, nodes, B.node_count[t], recs,
+                                S.sigma, T.skey0, T.skey1, nodes, qnodes, B.node_count[t], recs,
                                 B.dconst + 2ull * T.crystal_base);
     if (valid && threadIdx.x % L == 0u)
         out[T.out_base + (size_t)py * T.out_stride + px] = scan_value(Tm, S.mode);
@@ -1126,6 +1139,7 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.bn, 2 * n));
     TRY(alloc(b, B.leafcnt, 2 * n));
     TRY(alloc(b, B.dnodes, 2 * n));
+    TRY(alloc(b, B.dqnodes, 2 * n));
     TRY(alloc(b, B.drecs, n));
     TRY(alloc(b, B.dconst, 2 * n));
     // a level's segments partition its developed crystals: at most n / 65 have more
@@ -1291,6 +1305,8 @@ static hipError_t launch_stage(BatchedFrame *b, int stage, hipStream_t s)
     TRY_LAUNCH(b, "k_leafcount");
     hipLaunchKernelGGL(k_layout, dim3(nb2), dim3(256), 0, s, B, 2 * n);
     TRY_LAUNCH(b, "k_layout");
+    hipLaunchKernelGGL(k_scan_qnodes, dim3(nb2), dim3(256), 0, s, B, 2 * n);
+    TRY_LAUNCH(b, "k_scan_qnodes");
     hipLaunchKernelGGL(k_dev_records, dim3(nb), dim3(256), 0, s, B, in.recs, n);
     TRY_LAUNCH(b, "k_dev_records");
     return hipSuccess;

@@ -1013,10 +1013,14 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
 // origin; with outward-rounded float boxes and float origins the comparison is
 // exact, so the same crystals are tested as in the double-precision reference.
 // The walk is split into node steps and queued leaves, as in the transport: a lane
-// queues the leaves it hits (up to kScanQueue, in an LDS column of its own,
-// lq[q * kQueueLanes]) and keeps stepping; steps continue until 90 % of the walking lanes
-// have a full queue or have ended their walk, then every lane tests the crystals of its
-// queued leaves in the order found.  consts holds two float4 per record: its
+// queues the leaves it may hit (up to kScanQueue node indices, in an LDS column of its
+// own, lq[q * kQueueLanes]) and keeps stepping; steps continue until 90 % of the walking
+// lanes have a full queue or have ended their walk, then every lane tests the crystals of
+// its queued leaves in the order found.  The steps read the 16-byte grid nodes (one load
+// per step, ScanQNode), which enter every node the float test enters; a queued leaf is
+// confirmed before its crystals by the grid's narrow test or, in the grid's rounding
+// band, by its float box, so each ray meets the leaves of the float walk in its order.
+// consts holds two float4 per record: its
 // down_constants (the ray direction in the crystal frame is a constant of the crystal
 // for rays along -z) and its centre's rounding residual, with which crystal_chord_down
 // solves each crystal from the level of its centre.  The
@@ -1026,13 +1030,16 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
 // in no step.
 constexpr uint32_t kScanQueue = 16;
 
-__device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t n_nodes, const float4 *recs,
-                                                    const float4 *consts, float ox, float oy, float oz,
-                                                    bool active, uint32_t *lq)
+__device__ __forceinline__ float silver_path_queued(const Node *nodes, const ScanQNode *qnodes, uint32_t n_nodes,
+                                                    const float4 *recs, const float4 *consts, float ox, float oy,
+                                                    float oz, float bx, float by, float sc, bool active,
+                                                    uint32_t *lq)
 {
     float P = 0.0f;
     uint32_t i = active ? 0u : n_nodes, nq = 0;
-    const __amdgpu_buffer_rsrc_t nodes_r = buffer_of(nodes), recs_r = buffer_of(recs), consts_r = buffer_of(consts);
+    const __amdgpu_buffer_rsrc_t nodes_r = buffer_of(nodes), q_r = buffer_of(qnodes), recs_r = buffer_of(recs),
+                                 consts_r = buffer_of(consts);
+    const ScanQRay qr = scan_qray(ox, oy, bx, by, sc);   // the ray on the tree's grid (ScanQNode)
     for (;;) {
         const uint32_t walking = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(i < n_nodes));
         if (walking == 0u) break;
@@ -1043,19 +1050,30 @@ __device__ __forceinline__ float silver_path_queued(const Node *nodes, uint32_t 
             const uint32_t ns = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(stepping));
             if (ns == 0u || (it > 0u && ns <= stop)) break;
             if (stepping) {
-                const NodeView nd = load_node_buf(nodes_r, i);
-                const bool hit = nd.lo[0] < ox && ox < nd.hi[0] && nd.lo[1] < oy && oy < nd.hi[1];
-                if (hit && nd.prim != kInterior) {
-                    lq[kQueueLanes * nq++] = nd.prim;
-                    if (nq == kScanQueue) lim = 0u;
-                }
-                i = hit ? i + 1 : nd.skip;   // a leaf's skip link is its index + 1
+                const BufWords q = __builtin_amdgcn_raw_buffer_load_b128(q_r, (int)(i << 4), 0, 0);
+                const bool leaf = (uint32_t)q.z == kScanLeaf;
+                const bool hit = scan_q_may((uint32_t)q.x, (uint32_t)q.y, qr);
+                // a stepping lane has room in its queue, so the slot is written always and
+                // taken only by a leaf
+                lq[kQueueLanes * nq] = i;
+                nq += (hit && leaf) ? 1u : 0u;
+                lim = nq == kScanQueue ? 0u : lim;
+                // a leaf's skip link is its index + 1
+                i = (hit || leaf) ? i + 1 : (uint32_t)q.z;
             }
         }
 #pragma unroll 1
         for (uint32_t q = 0; q < kScanQueue; q++) {
             if (q >= nq) continue;
-            const uint32_t lf = lq[kQueueLanes * q];
+            const uint32_t j = lq[kQueueLanes * q];   // the leaf's node
+            const BufWords qn = __builtin_amdgcn_raw_buffer_load_b128(q_r, (int)(j << 4), 0, 0);
+            bool in = scan_q_surely((uint32_t)qn.x, (uint32_t)qn.y, qr);
+            if (!in) {
+                const NodeView nd = load_node_buf(nodes_r, j);
+                in = nd.lo[0] < ox && ox < nd.hi[0] && nd.lo[1] < oy && oy < nd.hi[1];
+            }
+            if (!in) continue;
+            const uint32_t lf = (uint32_t)qn.w;
             const uint32_t first = lf >> 4, cnt = lf & 15u;
             for (uint32_t k = first; k < first + cnt; k++) {
                 // a scan record's fourth word holds its bounding-sphere radius
@@ -1115,19 +1133,25 @@ FG_HD uint32_t scan_lanes(uint32_t rays, uint32_t rays_per_lane = 1u)
 __device__ __forceinline__ float scan_pixel(bool valid, uint32_t pix, uint32_t px, uint32_t py,
                                             uint32_t sub, uint32_t L, uint32_t g, uint32_t ss,
                                             float pw, float ph, float z0, float sigma, uint32_t key0,
-                                            uint32_t key1, const Node *nodes, uint32_t n_nodes,
-                                            const float4 *recs, const float4 *consts)
+                                            uint32_t key1, const Node *nodes, const ScanQNode *qnodes,
+                                            uint32_t n_nodes, const float4 *recs, const float4 *consts)
 {
     __shared__ uint32_t lq_all[kScanQueue][kQueueLanes];
     uint32_t *lq = &lq_all[0][threadIdx.x];
     const uint32_t R = g * g * ss * ss;
     float acc = 0.0f;
     if (valid) {
+        float bx = 0.0f, by = 0.0f, sc = 1.0f;   // no node: every walk ends at once
+        if (n_nodes > 0u) {
+            const Node root = nodes[0];
+            scan_qframe(root, bx, by, sc);
+        }
         for (uint32_t r = sub; r < R; r += L) {
             const uint32_t sx = r % g, sy = (r / g) % g, ax = (r / (g * g)) % ss, ay = r / (g * g * ss);
             float ox, oy;
             scan_origin(pix, px, py, ax, ay, sx, sy, g, ss, pw, ph, key0, key1, ox, oy);
-            acc += expf(-sigma * silver_path_queued(nodes, n_nodes, recs, consts, ox, oy, z0, true, lq));
+            acc += expf(-sigma * silver_path_queued(nodes, qnodes, n_nodes, recs, consts, ox, oy, z0, bx, by, sc,
+                                                    true, lq));
         }
     }
     for (uint32_t off = L >> 1; off >= 1u; off >>= 1) acc += __shfl_xor(acc, (int)off, 64);

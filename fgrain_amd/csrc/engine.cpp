@@ -72,7 +72,7 @@ extern "C" fg_status fg_engine_create(fg_engine **out)
 extern "C" void fg_engine_destroy(fg_engine *E)
 {
     if (!E) return;
-    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_recs, &E->scan_consts, &E->scan_lo, &E->out, &E->out2, &E->hp,
+    for (DevBuf *b : {&E->emap, &E->stats, &E->scan_nodes, &E->scan_qnodes, &E->scan_recs, &E->scan_consts, &E->scan_lo, &E->out, &E->out2, &E->hp,
                       &E->rays_o, &E->rays_d, &E->hits_n, &E->hits_i, &E->hits_t0, &E->hits_t1,
                       &E->dose64, &E->dose32, &E->prob, &E->dprob, &E->gprob, &E->gdose,
                       &E->gmap, &E->partials, &E->pgrad, &E->gout, &E->boost, &E->dboost,
@@ -640,8 +640,12 @@ static fg_status scan_enqueue(fg_engine *E, const Emulsion &em, const ScanBvh &s
                                   E->scan_consts.as<float4>(), E->stream), FG_ERR_PIPELINE);
     const size_t npx = (size_t)cfg.width_px * cfg.height_px;
     HIP_TRY(E->out.ensure(npx * sizeof(float)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(E->scan_qnodes.ensure(std::max<size_t>(nodes.size(), 1) * sizeof(ScanQNode)), FG_ERR_BUFFER_RANGE);
+    HIP_TRY(launch_scan_qnodes(E->scan_nodes.as<Node>(), (uint32_t)nodes.size(), E->scan_qnodes.as<ScanQNode>(),
+                               E->stream), FG_ERR_PIPELINE);
     ScanArgs a{};
     a.nodes = E->scan_nodes.as<Node>();
+    a.qnodes = E->scan_qnodes.as<ScanQNode>();
     a.recs = E->scan_recs.as<float4>();
     a.consts = E->scan_consts.as<float4>();
     a.n_nodes = (uint32_t)nodes.size();

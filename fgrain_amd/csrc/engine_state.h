@@ -80,7 +80,7 @@ struct fg_engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string name;
     std::mutex mu;
-    DevBuf emap, stats, scan_nodes, scan_recs, scan_consts, scan_lo, out, out2, hp, rays_o, rays_d, hits_n, hits_i,
+    DevBuf emap, stats, scan_nodes, scan_qnodes, scan_recs, scan_consts, scan_lo, out, out2, hp, rays_o, rays_d, hits_n, hits_i,
         hits_t0, hits_t1;
     // expected model and its adjoint
     DevBuf dose64, dose32, prob, dprob, gprob, gdose, gmap, partials, pgrad, gout, boost, dboost,

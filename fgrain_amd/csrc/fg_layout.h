@@ -56,6 +56,92 @@ struct alignas(16) Crystal {
 static_assert(sizeof(Node) == 32, "node must be 32 bytes");
 static_assert(sizeof(Crystal) == 48, "crystal record must be 48 bytes");
 
+// 16-byte node of the scan's walk along -z, one per Node at the same index: the node's
+// x-y box on a 16-bit grid of the tree (scan_qframe), rounded outward, and the skip link
+// of an interior node or kScanLeaf with the leaf's prim.  With X = (o - origin) * scale
+// for a ray through (ox, oy), a point strictly inside the node's float box has
+// lo < X < hi on both axes; the float evaluation x of X errs by less than 0.01 grid
+// units for every ray of a tile (relative error 2^-23, |X| far below 10^5), so
+// x >= lo - 0.5 and x <= hi + 0.5 hold for every such point (the walk may enter a few
+// more interior nodes, never fewer), and x >= lo + 1.5, x <= hi - 1.5 (lo and hi
+// within one unit of the exact box) prove the point strictly inside a leaf; in the band
+// between, the walk tests the leaf's float box itself.  The ray's bounds are rounded to
+// integers once (ScanQRay), so a node costs four integer comparisons.
+struct alignas(16) ScanQNode {
+    uint32_t lo;     // x | y << 16, rounded down
+    uint32_t hi;     // x | y << 16, rounded up
+    uint32_t word;   // skip link of an interior node, kScanLeaf for a leaf
+    uint32_t prim;   // the node's prim
+};
+static_assert(sizeof(ScanQNode) == 16, "scan node must be 16 bytes");
+constexpr uint32_t kScanLeaf = 0xFFFFFFFFu;
+
+// grid of a tree: origin at the root box's low x-y corner, 65534 units across its wider
+// x-y extent
+FG_HD void scan_qframe(const Node &root, float &bx, float &by, float &s)
+{
+    bx = root.lo[0];
+    by = root.lo[1];
+    const float ext = fmaxf(root.hi[0] - root.lo[0], root.hi[1] - root.lo[1]);
+    s = ext > 0.0f ? 65534.0f / ext : 1.0f;
+}
+
+// (v - b) * s is exact in double (float operands), so the rounding is exact too
+FG_HD uint32_t scan_qcoord(float v, float b, float s, bool up)
+{
+    const double x = ((double)v - (double)b) * (double)s;
+    const double r = up ? ceil(x) : floor(x);
+    return (uint32_t)fmin(fmax(r, 0.0), 65535.0);
+}
+
+// A ray on the grid: integer bounds for the two node tests.  For an integer l,
+// l <= floor(x + 0.5) exactly when x >= l - 0.5, and l < floor(x - 0.5) exactly when
+// x >= l + 1.5 (likewise ceil for the upper bounds); the float sums round by less than
+// 0.01 units.
+struct ScanQRay {
+    int32_t may_lo[2], may_hi[2];   // node enters when lo <= may_lo and hi >= may_hi
+    int32_t in_lo[2], in_hi[2];     // leaf surely holds the point when lo < in_lo and hi > in_hi
+};
+
+FG_HD int32_t scan_qint(float v) { return (int32_t)fminf(fmaxf(v, -1e9f), 1e9f); }
+
+FG_HD ScanQRay scan_qray(float ox, float oy, float bx, float by, float s)
+{
+    const float q[2] = {(ox - bx) * s, (oy - by) * s};
+    ScanQRay r;
+    for (int a = 0; a < 2; a++) {
+        r.may_lo[a] = scan_qint(floorf(q[a] + 0.5f));
+        r.may_hi[a] = scan_qint(ceilf(q[a] - 0.5f));
+        r.in_lo[a] = scan_qint(floorf(q[a] - 0.5f));
+        r.in_hi[a] = scan_qint(ceilf(q[a] + 0.5f));
+    }
+    return r;
+}
+
+// every point strictly inside the node's float box passes
+FG_HD bool scan_q_may(uint32_t lo, uint32_t hi, const ScanQRay &r)
+{
+    return (int32_t)(lo & 0xFFFFu) <= r.may_lo[0] && (int32_t)(hi & 0xFFFFu) >= r.may_hi[0] &&
+           (int32_t)(lo >> 16) <= r.may_lo[1] && (int32_t)(hi >> 16) >= r.may_hi[1];
+}
+
+// only points strictly inside the node's float box pass
+FG_HD bool scan_q_surely(uint32_t lo, uint32_t hi, const ScanQRay &r)
+{
+    return (int32_t)(lo & 0xFFFFu) < r.in_lo[0] && (int32_t)(hi & 0xFFFFu) > r.in_hi[0] &&
+           (int32_t)(lo >> 16) < r.in_lo[1] && (int32_t)(hi >> 16) > r.in_hi[1];
+}
+
+FG_HD ScanQNode scan_qnode(const Node &nd, float bx, float by, float s)
+{
+    ScanQNode o;
+    o.lo = scan_qcoord(nd.lo[0], bx, s, false) | scan_qcoord(nd.lo[1], by, s, false) << 16;
+    o.hi = scan_qcoord(nd.hi[0], bx, s, true) | scan_qcoord(nd.hi[1], by, s, true) << 16;
+    o.word = nd.prim == kInterior ? nd.skip : kScanLeaf;
+    o.prim = nd.prim;
+    return o;
+}
+
 // ------------------------------------------------------------------ Philox4x32-10
 FG_HD uint32_t mulhi32(uint32_t a, uint32_t b)
 {

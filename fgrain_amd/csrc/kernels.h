@@ -78,6 +78,7 @@ struct TileDev {
 
 struct ScanArgs {
     const Node *nodes;
+    const ScanQNode *qnodes;   // the scan's grid nodes of nodes (launch_scan_qnodes)
     const float4 *recs;
     const float4 *consts;    // 2 float4 per record (launch_down_constants)
     uint32_t n_nodes;
@@ -96,6 +97,8 @@ hipError_t launch_transport(const TransportArgs &a, hipStream_t s);
 // consts[2 k] = down_constants of record k (3 float4 per record), consts[2 k + 1] = lo[k],
 // the residual of its centre's rounding to f32 (x, y, z, 0); k < n
 hipError_t launch_down_constants(float4 *recs, const float4 *lo, uint32_t n, float4 *consts, hipStream_t s);
+// qnodes[i] = scan_qnode(nodes[i]) on the grid of nodes[0] (scan_qframe), i < n_nodes
+hipError_t launch_scan_qnodes(const Node *nodes, uint32_t n_nodes, ScanQNode *qnodes, hipStream_t s);
 // Persistent transport of one tile (packets 0 .. a.n_photons - 1); work must point to a
 // zeroed 64-bit counter; grid = resident blocks of the device.
 hipError_t launch_transport_persistent(const TransportArgs &a, unsigned long long *work, int grid,

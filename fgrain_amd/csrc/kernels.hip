@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
     const bool valid = pix < A.w_px * A.h_px;
     const uint32_t px = valid ? pix % A.w_px : 0u, py = valid ? pix / A.w_px : 0u;
     const float T = scan_pixel(valid, pix, px, py, threadIdx.x % L, L, A.g, A.ss, A.pw, A.ph, A.z0,
-                               A.sigma, A.key0, A.key1, A.nodes, A.n_nodes, A.recs, A.consts);
+                               A.sigma, A.key0, A.key1, A.nodes, A.qnodes, A.n_nodes, A.recs, A.consts);
     if (valid && threadIdx.x % L == 0u) A.out[(size_t)py * A.ostride + px] = scan_value(T, A.mode);
 }
 
@@ -65,6 +65,15 @@ __global__ void __launch_bounds__(256) down_constants_kernel(float4 *recs, const
     consts[2 * k] = make_float4(c[0], c[1], c[2], c[3]);
     consts[2 * k + 1] = lo[k];
     recs[3 * k] = make_float4(r.c[0], r.c[1], r.c[2], c[3]);   // radius in the sensitivity word
+}
+
+__global__ void __launch_bounds__(256) scan_qnodes_kernel(const Node *nodes, uint32_t n, ScanQNode *qnodes)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float bx, by, sc;
+    scan_qframe(nodes[0], bx, by, sc);
+    qnodes[i] = scan_qnode(nodes[i], bx, by, sc);
 }
 
 // ------------------------------------------------------- traversal for tests ---
@@ -203,6 +212,14 @@ hipError_t launch_valu_probe(float *out, uint32_t blocks, uint32_t iters, hipStr
     (void)hipGetLastError();  // report only this launcher's launches
     if (blocks == 0 || iters == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, s, out, iters, 1.0f, 0.0f);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_qnodes(const Node *nodes, uint32_t n_nodes, ScanQNode *qnodes, hipStream_t s)
+{
+    (void)hipGetLastError();  // report only this launcher's launches
+    if (n_nodes == 0) return hipSuccess;
+    hipLaunchKernelGGL(scan_qnodes_kernel, dim3((n_nodes + 255) / 256), dim3(256), 0, s, nodes, n_nodes, qnodes);
     return hipGetLastError();
 }
 
