@@ -66,6 +66,7 @@ struct Bufs {
     uint32_t *leafcnt;
     Node *dnodes;
     ScanQNode *dqnodes;    // the scan's grid nodes, at the indices of dnodes
+    PreRec *tpre;          // the transport's pre-test records, at the indices of the frame's records
     Crystal *drecs;
     float4 *dconst;        // 2 per developed-crystal record: down_constants, centre residual
     Seg *seg[2][kSegClasses];   // [level parity][size class]
@@ -168,8 +169,9 @@ __device__ __forceinline__ void transport_frame(const TransportArgs &C, const Bu
                                                 uint32_t n_tiles, uint64_t total_photons)
 {
     LaneCounters cnt;
-    transport_persistent<true, kCounts, kPre, kLQ, kCand>(C, B.tiles, n_tiles, nodes, recs, counts, emap, B.work,
-                                                          total_photons, cnt);
+    transport_persistent<true, kCounts, kPre, kLQ, kCand, true>(C, B.tiles, n_tiles, nodes, recs, counts, emap,
+                                                                B.work, total_photons, cnt, nullptr, nullptr,
+                                                                B.tpre);
     flush_lane_counters(cnt, C.stats);
 }
 
@@ -946,6 +948,15 @@ __global__ void __launch_bounds__(256) k_scan_qnodes(Bufs B, uint64_t cap)
     B.dqnodes[g] = scan_qnode(B.dnodes[g], bx, by, sc);
 }
 
+// the transport's pre-test record of every crystal of the frame (PreRec)
+__global__ void __launch_bounds__(256) k_pre_records(Bufs B, const Crystal *recs, uint64_t n)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Crystal r = recs[k];
+    B.tpre[k] = pre_record(r.c, r.h, r.morph);
+}
+
 __global__ void __launch_bounds__(256) k_dev_records(Bufs B, const Crystal *recs, uint64_t n)
 {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1156,6 +1167,9 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     TRY(alloc(b, B.devlist, n));
     TRY(alloc(b, B.dev_count, 1));
     TRY(alloc(b, B.tiles, b->n_tiles));
+    uint64_t n_trecs = 1;
+    for (const TileDev &t : in.tiles) n_trecs = std::max<uint64_t>(n_trecs, (uint64_t)t.crystal_base + t.n);
+    TRY(alloc(b, B.tpre, n_trecs));
     // host-side per-slot arrays
     std::vector<double> mdim(n);
     std::vector<uint32_t> tile_of(n);
@@ -1189,6 +1203,13 @@ hipError_t batched_create(const BatchedInput &in, hipStream_t s, BatchedFrame **
     b->total_blocks = blocks;
     b->in.tiles = tiles;
     TRY(hipMemcpyAsync(B.tiles, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, s));
+    if (in.recs) {
+        // the transport's pre-test records, once: the frame's records do not change after it is made
+        (void)hipGetLastError();
+        hipLaunchKernelGGL(k_pre_records, dim3((uint32_t)((n_trecs + 255) / 256)), dim3(256), 0, s, B, in.recs,
+                           n_trecs);
+        TRY(hipGetLastError());
+    }
     if (in.dev_pos) {
         (void)hipGetLastError();
         hipLaunchKernelGGL(k_gather_slots, dim3(std::max<uint32_t>(1u, (uint32_t)((n + 255) / 256))), dim3(256), 0, s,

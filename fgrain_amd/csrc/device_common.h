@@ -537,14 +537,18 @@ constexpr uint32_t kQueueLanes = 256;
 // (single tile only), with R the replay outputs and sums the lane's parameter sums.
 enum : int { kCounts = 0, kReplayDose = 1, kReplayGrad = 2 };
 
-template <bool kFrame, int kMode = kCounts, bool kPre = false, int kLQ = kLeafQueue, uint32_t kCand = kCandidates>
+// kPreRec (frame launches): the pre-test reads the crystals' 16-byte PreRec records pre
+// (one load per crystal) instead of the centre and semi-axes of the 48-byte records.
+template <bool kFrame, int kMode = kCounts, bool kPre = false, int kLQ = kLeafQueue, uint32_t kCand = kCandidates,
+          bool kPreRec = false>
 __device__ __forceinline__ void transport_persistent(const TransportArgs &C, const TileDev *tiles,
                                                      uint32_t n_tiles, const Node *nodes,
                                                      const float4 *recs, uint32_t *counts,
                                                      const float *emap, unsigned long long *work,
                                                      uint64_t total, LaneCounters &cnt,
                                                      const ReplayArgs *R = nullptr,
-                                                     ReplaySums *sums = nullptr)
+                                                     ReplaySums *sums = nullptr,
+                                                     const PreRec *pre = nullptr)
 {
     static_assert(kMode == kCounts || !kFrame, "replay runs on one tile");
     // the leaf cursor keeps the next queue slot (up to kLQ) in the top 4 bits of qk
@@ -568,7 +572,7 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
     const Node *nodes_b = kFrame ? nodes : C.nodes;
     const float4 *recs_b = kFrame ? recs : C.recs;
     uint32_t *counts_b = kFrame ? counts : C.counts;
-    const __amdgpu_buffer_rsrc_t nodes_r = buffer_of(nodes_b), recs_r = buffer_of(recs_b);
+    const __amdgpu_buffer_rsrc_t nodes_r = buffer_of(nodes_b), recs_r = buffer_of(recs_b), pre_r = buffer_of(pre);
     LaneTile T;
     PacketStart S0;
     if (!kFrame) lane_tile_single(C, T, S0);
@@ -832,14 +836,25 @@ __device__ __forceinline__ void transport_persistent(const TransportArgs &C, con
                             two = (lf & 15u) != 0u;   // a leaf of no crystals: one this step
                         }
                     }
-                    const uint32_t o = (T.crystal_base + kc) * 48u;
-                    const uint32_t o2 = two ? (T.crystal_base + k2) * 48u : o;
-                    const float4 ca = buf_load4(recs_r, o), ch = buf_load4(recs_r, o + 32u);
-                    const float4 ca2 = buf_load4(recs_r, o2), ch2 = buf_load4(recs_r, o2 + 32u);
-                    const float cc[3] = {ca.x, ca.y, ca.z}, hh[3] = {ch.x, ch.y, ch.z};
-                    const float cc2[3] = {ca2.x, ca2.y, ca2.z}, hh2[3] = {ch2.x, ch2.y, ch2.z};
-                    const bool m1 = crystal_may_absorb(cc, hh, __float_as_uint(ch.w), ox, oy, oz, dx, dy, dz, step);
-                    const bool m2 = two && crystal_may_absorb(cc2, hh2, __float_as_uint(ch2.w), ox, oy, oz, dx, dy, dz, step);
+                    bool m1, m2;
+                    if constexpr (kPreRec) {
+                        const uint32_t o = (T.crystal_base + kc) * 16u;
+                        const uint32_t o2 = two ? (T.crystal_base + k2) * 16u : o;
+                        const float4 a = buf_load4(pre_r, o), a2 = buf_load4(pre_r, o2);
+                        const PreRec r1 = {{a.x, a.y, a.z}, __float_as_uint(a.w)};
+                        const PreRec r2 = {{a2.x, a2.y, a2.z}, __float_as_uint(a2.w)};
+                        m1 = crystal_may_absorb_pre(r1, ox, oy, oz, dx, dy, dz, step);
+                        m2 = two && crystal_may_absorb_pre(r2, ox, oy, oz, dx, dy, dz, step);
+                    } else {
+                        const uint32_t o = (T.crystal_base + kc) * 48u;
+                        const uint32_t o2 = two ? (T.crystal_base + k2) * 48u : o;
+                        const float4 ca = buf_load4(recs_r, o), ch = buf_load4(recs_r, o + 32u);
+                        const float4 ca2 = buf_load4(recs_r, o2), ch2 = buf_load4(recs_r, o2 + 32u);
+                        const float cc[3] = {ca.x, ca.y, ca.z}, hh[3] = {ch.x, ch.y, ch.z};
+                        const float cc2[3] = {ca2.x, ca2.y, ca2.z}, hh2[3] = {ch2.x, ch2.y, ch2.z};
+                        m1 = crystal_may_absorb(cc, hh, __float_as_uint(ch.w), ox, oy, oz, dx, dy, dz, step);
+                        m2 = two && crystal_may_absorb(cc2, hh2, __float_as_uint(ch2.w), ox, oy, oz, dx, dy, dz, step);
+                    }
                     cq[kQueueLanes * nc] = m1 ? kc : k2;
                     nc += m1 ? 1u : 0u;
                     if (m1 && m2) cq[kQueueLanes * nc] = k2;

@@ -265,22 +265,22 @@ FG_HD float margin_rcp(float x)
 //    rotation and division errors).  Near tangency the roots spread by at most
 //    sqrt(error) / 2a <= 7.7e-4 rho hmax along the ray; the ends of the segment get 8e-3
 //    rho hmax.
-// Comparisons are written so that a NaN keeps the crystal.
-FG_HD bool crystal_may_absorb(const float c[3], const float h[3], uint32_t morph, float ox, float oy, float oz,
-                              float dx, float dy, float dz, float step)
+// Comparisons are written so that a NaN keeps the crystal.  Every quantity grows with
+// hmax and with 1 / hmin, so the test keeps at least the same segments when it is given a
+// larger hmax or a smaller hmin (crystal_may_absorb_pre).
+FG_HD bool crystal_may_absorb_hm(const float c[3], float hmax, float hmin, bool cube, float ox, float oy, float oz,
+                                 float dx, float dy, float dz, float step)
 {
     const float px = c[0] - ox, py = c[1] - oy, pz = c[2] - oz;
-    const float hmax = fmaxf(fmaxf(h[0], h[1]), h[2]);
     const float pn = fabsf(px) + fabsf(py) + fabsf(pz);
     float R, r2, mt;
-    if (morph == 0u) {
+    if (cube) {
         R = hmax * 1.7320525f;   // sqrt(3) (1 + 1e-6)
         const float m = 1e-5f * (pn + R);
         r2 = (R + m) * (R + m);
         mt = m;
     } else {
         R = hmax * 1.000001f;
-        const float hmin = fminf(fminf(h[0], h[1]), h[2]);
         const float rho = (pn + R) * margin_rcp(hmin);
         r2 = (R * R) * (1.0f + rho * (1e-5f + 6e-6f * rho));
         mt = hmax * (8e-3f * rho) + 1e-5f * (pn + R);
@@ -291,6 +291,67 @@ FG_HD bool crystal_may_absorb(const float c[3], const float h[3], uint32_t morph
     if (tp + (R + mt) <= 0.0f) return false;               // behind the origin
     if (tp - (R + mt) >= step) return false;               // beyond the segment's end
     return true;
+}
+
+FG_HD bool crystal_may_absorb(const float c[3], const float h[3], uint32_t morph, float ox, float oy, float oz,
+                              float dx, float dy, float dz, float step)
+{
+    return crystal_may_absorb_hm(c, fmaxf(fmaxf(h[0], h[1]), h[2]), fminf(fminf(h[0], h[1]), h[2]), morph == 0u, ox,
+                                 oy, oz, dx, dy, dz, step);
+}
+
+// binary16 bits of a float v >= 0, rounded up or down; only normal values, 0 and infinity
+// result (below 2^-14 v rounds down to 0 and up to 2^-14, above 65504 down to 65504 and up
+// to infinity)
+FG_HD uint32_t f16_bits_dir(float v, bool up)
+{
+    if (!(v > 0.0f)) return 0u;
+    if (v < 0x1p-14f) return up ? (1u << 10) : 0u;
+    if (v > 65504.0f) return up ? 0x7C00u : 0x7BFFu;
+    const uint32_t b = __builtin_bit_cast(uint32_t, v);
+    const uint32_t h = ((((b >> 23) & 255u) - 112u) << 10) | ((b >> 13) & 1023u);
+    return h + ((up && (b & 8191u)) ? 1u : 0u);   // a carry moves to the next binade exactly
+}
+
+// the value of binary16 bits made by f16_bits_dir (exact; on the device one conversion
+// instruction, which is exact for every binary16 value)
+FG_HD float f16_bits_value(uint32_t h)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+#endif
+    const uint32_t e = (h >> 10) & 31u, m = h & 1023u;
+    const uint32_t b = e == 0u ? 0u : (e == 31u ? 0x7F800000u : ((e + 112u) << 23) | (m << 13));
+    return __builtin_bit_cast(float, b);
+}
+
+// 16-byte record of the frame transport's absorption pre-test, one per crystal at the
+// record's index: the centre, the largest semi-axis rounded up and the smallest rounded
+// down to binary16, and whether the crystal takes the ellipsoid branch.  The pre-test on
+// it keeps every segment crystal_may_absorb keeps (larger hmax, smaller hmin), in one
+// load instead of two.
+struct alignas(16) PreRec {
+    float c[3];
+    uint32_t hw;   // up(hmax) | down(hmin) << 16 | (morph != 0) << 31
+};
+static_assert(sizeof(PreRec) == 16, "pre-test record must be 16 bytes");
+
+FG_HD PreRec pre_record(const float c[3], const float h[3], uint32_t morph)
+{
+    PreRec r;
+    r.c[0] = c[0];
+    r.c[1] = c[1];
+    r.c[2] = c[2];
+    r.hw = f16_bits_dir(fmaxf(fmaxf(h[0], h[1]), h[2]), true) |
+           f16_bits_dir(fminf(fminf(h[0], h[1]), h[2]), false) << 16 | (morph != 0u ? 1u : 0u) << 31;
+    return r;
+}
+
+FG_HD bool crystal_may_absorb_pre(const PreRec &r, float ox, float oy, float oz, float dx, float dy, float dz,
+                                  float step)
+{
+    return crystal_may_absorb_hm(r.c, f16_bits_value(r.hw & 0xFFFFu), f16_bits_value((r.hw >> 16) & 0x7FFFu),
+                                 (r.hw >> 31) == 0u, ox, oy, oz, dx, dy, dz, step);
 }
 
 // Constants of a crystal for rays along -z (the scan): the direction (0, 0, -1) rotated

@@ -1,4 +1,6 @@
-// Host check of the transport's absorption pre-test (crystal_may_absorb, fg_layout.h)
+// Host check of the transport's absorption pre-test (crystal_may_absorb, fg_layout.h),
+// and of its packed record (crystal_may_absorb_pre: it must keep every segment the float
+// pre-test keeps),
 // against the float crystal test itself (crystal_hit and the event conditions of the
 // transport, transport.rs:109-131): a pre-test that rejects a segment on which the float
 // test records an event is a violation.  Built with -ffp-contract=off from the same header
@@ -149,14 +151,17 @@ void run(uint64_t seed, long samples, Totals &T)
         const bool hit = fg::crystal_hit(cf, qf, hf, morph, ox, oy, oz, dx, dy, dz, t0, t1);
         const bool event = hit && t0 < step && fmaxf(fminf(t1, step) - t0, 0.0f) > 0.0f;
         const bool may = fg::crystal_may_absorb(cf, hf, morph, ox, oy, oz, dx, dy, dz, step);
+        // the frame transport's packed pre-test record must keep whatever the float one keeps
+        const bool may_pre = fg::crystal_may_absorb_pre(fg::pre_record(cf, hf, morph), ox, oy, oz, dx, dy, dz, step);
         n++;
         ev += event;
         rej += !may;
-        if (event && !may) {
+        if ((event && !may) || (may && !may_pre)) {
             if (bad < 5)
-                std::printf("violation: morph %u h %.9g %.9g %.9g c %.9g %.9g %.9g o %.9g %.9g %.9g d %.9g %.9g %.9g "
+                std::printf("violation%s: morph %u h %.9g %.9g %.9g c %.9g %.9g %.9g o %.9g %.9g %.9g d %.9g %.9g %.9g "
                             "step %.9g t0 %.9g t1 %.9g\n",
-                            morph, hf[0], hf[1], hf[2], cf[0], cf[1], cf[2], ox, oy, oz, dx, dy, dz, step, t0, t1);
+                            may ? " (packed record)" : "", morph, hf[0], hf[1], hf[2], cf[0], cf[1], cf[2], ox, oy, oz,
+                            dx, dy, dz, step, t0, t1);
             bad++;
         }
     }

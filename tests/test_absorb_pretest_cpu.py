@@ -60,3 +60,18 @@ def test_samples_find_violations_when_the_margins_are_cut(gxx, tmp_path):
     _build(inc, exe)
     r, (n, events, rejected, bad) = _run(exe, 40_000_000, 12)
     assert bad > 0 and r.returncode == 1, r.stdout
+
+
+def test_samples_find_violations_when_the_packed_record_rounds_the_wrong_way(gxx, tmp_path):
+    # the frame transport's packed pre-test record must round the largest semi-axis up;
+    # rounded down it rejects segments the float pre-test keeps
+    src = open(os.path.join(CSRC, "fg_layout.h")).read()
+    old = "f16_bits_dir(fmaxf(fmaxf(h[0], h[1]), h[2]), true)"
+    assert src.count(old) == 1
+    inc = tmp_path / "weak"
+    inc.mkdir()
+    (inc / "fg_layout.h").write_text(src.replace(old, old.replace("true)", "false)")))
+    exe = tmp_path / "absorb_pretest_packed_weak"
+    _build(inc, exe)
+    r, (n, events, rejected, bad) = _run(exe, 4_000_000, 13)
+    assert bad > 0 and r.returncode == 1 and "(packed record)" in r.stdout, r.stdout
