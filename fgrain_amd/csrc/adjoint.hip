@@ -250,8 +250,8 @@ template <bool kPre>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) replay_persistent_dose_kernel(ReplayArgs R, unsigned long long *work)
 {
     LaneCounters cnt;
-    transport_persistent<false, kReplayDose, kPre>(R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work,
-                                             R.t.n_photons, cnt, &R, nullptr);
+    transport_persistent<false, kReplayDose, kPre, kLeafQueue, kCandidates, true>(
+        R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work, R.t.n_photons, cnt, &R, nullptr, R.t.pre);
     if (R.stats) {
         const unsigned long long e = wave_sum((unsigned long long)cnt.abs);
         if ((threadIdx.x & 63) == 0) {
@@ -267,8 +267,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))
     __shared__ double red[4][3];
     LaneCounters cnt;
     ReplaySums sums;
-    transport_persistent<false, kReplayGrad>(R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work,
-                                             R.t.n_photons, cnt, &R, &sums);
+    transport_persistent<false, kReplayGrad, false, kLeafQueue, kCandidates, true>(
+        R.t, nullptr, 0u, nullptr, nullptr, nullptr, nullptr, work, R.t.n_photons, cnt, &R, &sums, R.t.pre);
     if (R.stats) {
         const unsigned long long e = wave_sum((unsigned long long)cnt.abs);
         if ((threadIdx.x & 63) == 0) {
@@ -552,6 +552,7 @@ hipError_t launch_replay_persistent_dose(const ReplayArgs &a, unsigned long long
 {
     (void)hipGetLastError();  // report only this launcher's launches
     if (a.t.n_photons == 0) return hipSuccess;
+    if (!a.t.pre) return hipErrorInvalidValue;   // the walk's pre-test reads these records
     // the f64 deposits are always pre-reduced across the wave: with them the dose pass of
     // the 512 x 512 adjoint tile takes 72 ms instead of 81 (the counts transport, whose
     // u32 atomics are cheaper, leaves it to the tuning)
@@ -563,6 +564,7 @@ hipError_t launch_replay_persistent_grad(const ReplayArgs &a, unsigned long long
 {
     (void)hipGetLastError();  // report only this launcher's launches
     if (a.t.n_photons == 0) return hipSuccess;
+    if (!a.t.pre) return hipErrorInvalidValue;   // the walk's pre-test reads these records
     replay_persistent_grad_kernel<<<grid, 256, 0, s>>>(a, work);
     return hipGetLastError();
 }

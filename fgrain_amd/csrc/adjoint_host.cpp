@@ -121,6 +121,7 @@ fg_status setup(fg_engine *E, Emulsion &em, const fg_expected_model &m, const fl
     TransportArgs &a = S.r.t;
     a.nodes = d.nodes.as<Node>();
     a.recs = d.recs.as<float4>();
+    a.pre = d.pre.as<PreRec>();
     a.n_nodes = d.n_nodes;
     fill_transport(a, ec, m.exposure, tw, th, m.expose_seed, em.reach, E->tuning);
     HIP_TRY(E->emap.ensure(S.nmap * sizeof(float)), FG_ERR_BUFFER_RANGE);

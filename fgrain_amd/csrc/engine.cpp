@@ -329,6 +329,7 @@ static void free_device(Emulsion &em)
     if (em.dev) {
         em.dev->nodes.release();
         em.dev->recs.release();
+        em.dev->pre.release();
         em.dev->counts.release();
         em.dev->exp_nodes.release();
         em.dev->exp_recs.release();
@@ -430,6 +431,14 @@ fg_status ensure_device(fg_engine *E, Emulsion &em)
     if (!em.recs.empty())
         HIP_TRY(hipMemcpyAsync(d.recs.p, em.recs.data(), em.recs.size() * sizeof(Crystal),
                                hipMemcpyHostToDevice, E->stream), FG_ERR_BUFFER_MAP);
+    HIP_TRY(d.pre.ensure(std::max<size_t>(em.recs.size(), 1) * sizeof(PreRec)), FG_ERR_BUFFER_RANGE);
+    if (!em.recs.empty()) {
+        std::vector<PreRec> pre(em.recs.size());
+        for (size_t k = 0; k < pre.size(); k++) pre[k] = pre_record(em.recs[k].c, em.recs[k].h, em.recs[k].morph);
+        HIP_TRY(hipMemcpyAsync(d.pre.p, pre.data(), pre.size() * sizeof(PreRec), hipMemcpyHostToDevice, E->stream),
+                FG_ERR_BUFFER_MAP);
+        HIP_TRY(hipStreamSynchronize(E->stream), FG_ERR_CHANNEL);   // before pre goes out of scope
+    }
     return FG_OK;
 }
 

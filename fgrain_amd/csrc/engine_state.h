@@ -58,6 +58,7 @@ inline bool offsets_fit_32(uint64_t n_nodes, uint64_t n_records)
 namespace fg {
 struct DeviceEmulsion {
     DevBuf nodes, recs, counts;
+    DevBuf pre;   // the transport's pre-test records (PreRec), for the replay kernels
     uint32_t n_nodes = 0;
     int64_t n = 0;
     // expected-model scan BVH: every crystal, boxes for the largest boosted expansion,

@@ -12,6 +12,7 @@ namespace fg {
 struct TransportArgs {
     const Node *nodes;
     const float4 *recs;      // 3 float4 per crystal record, leaf order
+    const PreRec *pre;       // pre-test record per crystal (the persistent replay kernels)
     uint32_t n_nodes;
     uint32_t chunk;          // packets a wave takes from the work counter at a time (>= 64)
     const float *emap;       // exposure map (may be null: weight 1)
